@@ -1,0 +1,22 @@
+// Device-side PCS commit building blocks (commit.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flock_gpu.h"
+#include "ctx.h"
+
+namespace flock {
+
+// Forward interleaved additive NTT over 2^k positions x num_lanes lanes, from
+// `start_layer`, fusing up to `max_fuse` layers per pass.
+int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k, int num_lanes, int start_layer,
+                                   int max_fuse);
+// Flat BLAKE3 Merkle tree of n_leaves leaves of leaf_bytes each.
+int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t leaf_bytes, uint64_t n_leaves,
+                       void *d_tree);
+// Full commit; writes the root to host memory and synchronizes `st`.
+int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
+                  uint8_t root[32], hipStream_t st);
+
+}  // namespace flock

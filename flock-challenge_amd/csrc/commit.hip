@@ -1,0 +1,220 @@
+// PCS commit on gfx950: replicate -> interleaved additive NTT -> BLAKE3 Merkle.
+//
+// Layout in HBM (the reference's, crates/flock-core/src/pcs/commit.rs:235-245):
+// codeword[pos * NL + lane] F128, NL = 2^log_batch_size interleaved RS lanes;
+// Merkle leaf p = the NL*16 contiguous bytes of position p.
+//
+// NTT pass kernel.  A pass applies F consecutive layers [l, l+F) of the
+// neighbors-last forward transform (ntt/additive_ntt_f128.rs:1281-1320).
+// The 2^F positions that exchange data in those layers share every position
+// bit except F contiguous ones, so they form a "group"
+//     pos(e) = (B << (k-l)) | (e << s) | r,   s = k - l - F,  e in [0, 2^F).
+// One wavefront owns one group for NL = 64: lane = RS lane, and each lane
+// keeps its 2^F elements in VGPRs, so every butterfly is lane-local and the
+// global loads/stores are 1 KiB-contiguous per position (fully coalesced).
+// The twiddle of a butterfly depends only on (layer, block), i.e. it is
+// wavefront-uniform: its product runs through a 2 x 256-entry byte table of
+// that twiddle built cooperatively in LDS (gf128_dev.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/flock_gpu.h"
+#include "blake3_dev.h"
+#include "commit.h"
+#include "ctx.h"
+#include "gf128_dev.h"
+
+namespace flock {
+
+template <int F>
+__global__ __launch_bounds__(64) void ntt_pass_kernel(f128 *__restrict__ data, const f128 *__restrict__ tw,
+                                                      int k, int l) {
+  __shared__ f128 tab[512];
+  const uint32_t lane = threadIdx.x;
+  const int s = k - l - F;
+  const uint64_t g = blockIdx.x;
+  const uint64_t r = g & ((1ull << s) - 1ull);
+  const uint64_t B = g >> s;
+  const uint64_t base = (B << (k - l)) | r;
+  f128 v[1 << F];
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) v[e] = data[((base | ((uint64_t)e << s)) << 6) + lane];
+
+#pragma unroll
+  for (int j = 0; j < F; ++j) {
+    const uint64_t tw_base = (1ull << (l + j)) - 1ull + (B << j);
+#pragma unroll
+    for (int c = 0; c < (1 << j); ++c) {
+      const f128 t = tw[tw_base + c];
+      __syncthreads();
+      f128_build_bytetab(t, tab, lane);
+      __syncthreads();
+#pragma unroll
+      for (int el = 0; el < (1 << (F - 1 - j)); ++el) {
+        const int e0 = (c << (F - j)) | el;
+        const int e1 = e0 | (1 << (F - 1 - j));
+        f128 u = f128_xor(v[e0], f128_mul_bytetab(v[e1], tab));
+        v[e0] = u;
+        f128_xor_into(v[e1], u);
+        // Keep one product's 16 table loads in flight at a time; letting the
+        // scheduler hoist loads across butterflies blows the VGPR budget.
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
+}
+
+// Replicate the message into every 2^log_inv_rate sub-block: the exact state
+// after the first log_inv_rate layers (pcs/commit.rs:451-467).
+__global__ void replicate_kernel(const uint4 *__restrict__ msg, uint4 *__restrict__ cw, uint64_t msg_len,
+                                 uint64_t cw_len) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < cw_len; i += stride) cw[i] = msg[i & (msg_len - 1)];
+}
+
+// Leaf = non-root BLAKE3 chunk chaining value of leaf_bytes (<= 1024) bytes
+// (merkle.rs:200-205): counter 0, CHUNK_START on the first block, CHUNK_END on
+// the last, final block zero-padded with its true length.
+__global__ __launch_bounds__(256) void merkle_leaf_kernel(const uint4 *__restrict__ data, uint32_t leaf_bytes,
+                                                          uint64_t n_leaves, uint4 *__restrict__ out) {
+  uint64_t leaf = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= n_leaves) return;
+  const uint32_t leaf_u4 = leaf_bytes >> 4;
+  const uint4 *p = data + leaf * leaf_u4;
+  uint32_t cv[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) cv[i] = b3::IV[i];
+  const uint32_t nblocks = leaf_bytes == 0 ? 1u : (leaf_bytes + 63u) >> 6;
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    uint32_t m[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t idx = b * 4u + q;
+      uint4 w = idx < leaf_u4 ? p[idx] : make_uint4(0u, 0u, 0u, 0u);
+      m[4 * q] = w.x; m[4 * q + 1] = w.y; m[4 * q + 2] = w.z; m[4 * q + 3] = w.w;
+    }
+    uint32_t len = leaf_bytes - 64u * b;
+    len = len > 64u ? 64u : len;
+    uint32_t flags = (b == 0 ? b3::CHUNK_START : 0u) | (b + 1 == nblocks ? b3::CHUNK_END : 0u);
+    b3::compress_cv(cv, m, 0, len, flags);
+  }
+  out[2 * leaf] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+  out[2 * leaf + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+}
+
+// Parent level: out[i] = compress(IV, in[2i] || in[2i+1], 0, 64, PARENT)[0..8]
+// (merkle.rs:207-210).  Each workgroup folds up to `levels` levels through LDS.
+__global__ __launch_bounds__(256) void merkle_parent_kernel(const uint4 *__restrict__ in, uint64_t n_out,
+                                                            uint4 *__restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  uint32_t m[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint4 w = in[4 * i + q];
+    m[4 * q] = w.x; m[4 * q + 1] = w.y; m[4 * q + 2] = w.z; m[4 * q + 3] = w.w;
+  }
+  uint32_t cv[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) cv[t] = b3::IV[t];
+  b3::compress_cv(cv, m, 0, 64, b3::PARENT);
+  out[2 * i] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+  out[2 * i + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+}
+
+static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k, int l, int F) {
+  ScopedPhase ph(ctx, "ntt_pass_kernel", st);
+  uint64_t groups = 1ull << (k - F);
+  dim3 grid((unsigned)groups), block(64);
+  switch (F) {
+    case 1: hipLaunchKernelGGL(ntt_pass_kernel<1>, grid, block, 0, st, data, tw, k, l); break;
+    case 2: hipLaunchKernelGGL(ntt_pass_kernel<2>, grid, block, 0, st, data, tw, k, l); break;
+    case 3: hipLaunchKernelGGL(ntt_pass_kernel<3>, grid, block, 0, st, data, tw, k, l); break;
+    case 4: hipLaunchKernelGGL(ntt_pass_kernel<4>, grid, block, 0, st, data, tw, k, l); break;
+    case 5: hipLaunchKernelGGL(ntt_pass_kernel<5>, grid, block, 0, st, data, tw, k, l); break;
+    default: ctx->last_error = "ntt_pass: unsupported F"; return FLOCK_E_ARG;
+  }
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k, int num_lanes, int start_layer,
+                                   int max_fuse) {
+  if (num_lanes != 64) {
+    ctx->last_error = "ntt: only 64 interleaved lanes are implemented";
+    return FLOCK_E_SHAPE;
+  }
+  const F128 *tw = ctx->device_twiddles(k);
+  if (!tw) return FLOCK_E_DEVICE;
+  int l = start_layer;
+  while (l < k) {
+    int F = std::min(max_fuse, k - l);
+    int rc = ntt_pass(ctx, st, (f128 *)d_data, (const f128 *)tw, k, l, F);
+    if (rc) return rc;
+    l += F;
+  }
+  return FLOCK_OK;
+}
+
+int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t leaf_bytes, uint64_t n_leaves,
+                       void *d_tree) {
+  if (leaf_bytes > 1024 || (leaf_bytes & 15) || n_leaves == 0 || (n_leaves & (n_leaves - 1))) {
+    ctx->last_error = "merkle: leaf must be a multiple of 16 bytes, at most 1024, power-of-two leaf count";
+    return FLOCK_E_SHAPE;
+  }
+  uint4 *tree = (uint4 *)d_tree;
+  {
+  ScopedPhase ph(ctx, "merkle_leaf_kernel", st);
+  hipLaunchKernelGGL(merkle_leaf_kernel, dim3((unsigned)((n_leaves + 255) / 256)), dim3(256), 0, st,
+                     (const uint4 *)d_data, leaf_bytes, n_leaves, tree);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  }
+  ScopedPhase ph(ctx, "merkle_parents", st);
+  uint64_t rs = 0, rl = n_leaves;
+  while (rl > 1) {
+    uint64_t nl = rl >> 1;
+    hipLaunchKernelGGL(merkle_parent_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st,
+                       (const uint4 *)(tree + 2 * rs), nl, tree + 2 * (rs + rl));
+    FLOCK_HIP_CHECK(ctx, hipGetLastError());
+    rs += rl;
+    rl = nl;
+  }
+  return FLOCK_OK;
+}
+
+int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
+                  uint8_t root[32], hipStream_t st) {
+  if (p->merkle_hash != 1) {
+    ctx->last_error = "only the BLAKE3 Merkle hash is implemented";
+    return FLOCK_E_ARG;
+  }
+  if (p->m < 7 + p->log_batch_size || p->log_inv_rate < 1) {
+    ctx->last_error = "commit: m too small or log_inv_rate < 1";
+    return FLOCK_E_SHAPE;
+  }
+  const int log_msg = (int)p->m - 7;
+  const int log_dim = log_msg - (int)p->log_batch_size;
+  const int k = log_dim + (int)p->log_inv_rate;
+  const uint64_t msg_len = 1ull << log_msg;
+  const uint64_t cw_len = msg_len << p->log_inv_rate;
+  ScopedPhase ph(ctx, "commit", st);
+  hipLaunchKernelGGL(replicate_kernel, dim3(2048), dim3(256), 0, st, (const uint4 *)d_z, (uint4 *)d_codeword,
+                     msg_len, cw_len);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  int rc = ntt_forward_interleaved_device(ctx, st, d_codeword, k, 1 << p->log_batch_size, (int)p->log_inv_rate, 4);
+  if (rc) return rc;
+  rc = merkle_tree_device(ctx, st, d_codeword, 16u << p->log_batch_size, 1ull << k, d_tree);
+  if (rc) return rc;
+  const uint64_t root_idx = (2ull << k) - 2ull;
+  FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(root, (const uint8_t *)d_tree + 32 * root_idx, 32, hipMemcpyDeviceToHost, st));
+  ph.end();
+  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  return FLOCK_OK;
+}
+
+}  // namespace flock

@@ -1,0 +1,123 @@
+// Context, scratch arena and constant tables.
+#include "ctx.h"
+
+#include "../../include/flock_gpu.h"
+
+namespace flock {
+
+const Phi8 &phi8_table() {
+  static const Phi8 t;
+  return t;
+}
+
+// Normalized subspace-polynomial rows evals[i][k] = W_i(b_{i+k}) / W_i(b_i)
+// for the standard basis b_i = x^i, then the breadth-first span table
+// (ntt/additive_ntt_f128.rs:57-89, 106-128).
+std::vector<F128> ntt_twiddles(int dim) {
+  std::vector<std::vector<F128>> ev(dim);
+  ev[0].resize(dim);
+  for (int i = 0; i < dim; ++i) ev[0][i] = i < 64 ? F128(1ull << i, 0) : F128(0, 1ull << (i - 64));
+  for (int i = 1; i < dim; ++i) {
+    ev[i].resize(dim - i);
+    for (int k = 1; k <= dim - i; ++k) {
+      F128 p = ev[i - 1][k];
+      ev[i][k - 1] = p * (p + ev[i - 1][0]);
+    }
+  }
+  for (int i = 0; i < dim; ++i) {
+    F128 iv = inv(ev[i][0]);
+    for (auto &x : ev[i]) x = x * iv;
+  }
+  std::vector<F128> tw(dim ? ((size_t)1 << dim) - 1 : 0);
+  for (int layer = 0; layer < dim; ++layer) {
+    size_t start = ((size_t)1 << layer) - 1;
+    const auto &row = ev[dim - layer - 1];
+    tw[start] = F128();
+    for (int bit = 0; bit < layer; ++bit) {
+      size_t half = (size_t)1 << bit;
+      for (size_t b = 0; b < half; ++b) tw[start + half + b] = tw[start + b] + row[1 + bit];
+    }
+  }
+  return tw;
+}
+
+const std::vector<F128> &Ctx::host_twiddles(int dim) {
+  auto it = twiddles_host.find(dim);
+  if (it != twiddles_host.end()) return it->second;
+  return twiddles_host[dim] = ntt_twiddles(dim);
+}
+
+const F128 *Ctx::device_twiddles(int dim) {
+  auto it = twiddles_dev.find(dim);
+  if (it != twiddles_dev.end()) return it->second;
+  const auto &h = host_twiddles(dim);
+  F128 *d = nullptr;
+  size_t bytes = std::max<size_t>(1, h.size()) * sizeof(F128);
+  if (hipMalloc(&d, bytes) != hipSuccess) {
+    last_error = "hipMalloc twiddles failed";
+    return nullptr;
+  }
+  if (!h.empty() && hipMemcpy(d, h.data(), h.size() * sizeof(F128), hipMemcpyHostToDevice) != hipSuccess) {
+    last_error = "hipMemcpy twiddles failed";
+    hipFree(d);
+    return nullptr;
+  }
+  twiddles_dev[dim] = d;
+  return d;
+}
+
+void *Ctx::get_scratch(const std::string &name, size_t bytes) {
+  auto &b = scratch[name];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.ptr, bytes) != hipSuccess) {
+    b.ptr = nullptr;
+    last_error = "hipMalloc scratch '" + name + "' failed";
+    return nullptr;
+  }
+  b.bytes = bytes;
+  return b.ptr;
+}
+
+void Ctx::phase_begin(const std::string &name, hipStream_t st) {
+  PhaseEvent e;
+  e.name = name;
+  hipEventCreate(&e.begin);
+  hipEventCreate(&e.end);
+  hipEventRecord(e.begin, st);
+  open_phases.push_back(phase_events.size());
+  phase_events.push_back(e);
+}
+
+void Ctx::phase_end(hipStream_t st) {
+  if (open_phases.empty()) return;
+  hipEventRecord(phase_events[open_phases.back()].end, st);
+  open_phases.pop_back();
+}
+
+void Ctx::resolve_phases() {
+  for (auto &e : phase_events) {
+    hipEventSynchronize(e.end);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.begin, e.end) == hipSuccess) {
+      auto &t = phase_totals[e.name];
+      t.first += ms;
+      t.second += 1;
+    }
+    hipEventDestroy(e.begin);
+    hipEventDestroy(e.end);
+  }
+  phase_events.clear();
+}
+
+Ctx::~Ctx() {
+  hipSetDevice(device);
+  for (auto &kv : twiddles_dev) hipFree(kv.second);
+  for (auto &kv : scratch)
+    if (kv.second.ptr) hipFree(kv.second.ptr);
+  if (stream) hipStreamDestroy(stream);
+}
+
+}  // namespace flock

@@ -1,0 +1,84 @@
+// Per-device prover context: stream, error text, cached constant tables and
+// a grow-only device arena.  One context per GPU; calls on distinct contexts
+// are independent, calls on one context are serialized by the caller.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "host_field.h"
+
+namespace flock {
+
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Device-side phase timing with HIP events on the launching stream.
+struct PhaseEvent {
+  std::string name;
+  hipEvent_t begin, end;
+};
+
+struct Ctx {
+  int device = 0;
+  bool timing = false;
+  std::vector<PhaseEvent> phase_events;
+  std::vector<size_t> open_phases;
+  std::map<std::string, std::pair<double, long>> phase_totals;  // ms, count
+  void phase_begin(const std::string &name, hipStream_t st);
+  void phase_end(hipStream_t st);
+  void resolve_phases();  // after the stream has been synchronized
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // Breadth-first NTT twiddle table per domain dimension (device copy).
+  std::map<int, F128 *> twiddles_dev;
+  std::map<int, std::vector<F128>> twiddles_host;
+  // Named grow-only scratch buffers.
+  std::map<std::string, DeviceBuffer> scratch;
+
+  void *get_scratch(const std::string &name, size_t bytes);
+  const std::vector<F128> &host_twiddles(int dim);
+  const F128 *device_twiddles(int dim);
+  ~Ctx();
+};
+
+struct ScopedPhase {
+  Ctx *ctx;
+  hipStream_t st;
+  bool open;
+  ScopedPhase(Ctx *c, const char *name, hipStream_t s) : ctx(c), st(s), open(c->timing) {
+    if (open) ctx->phase_begin(name, st);
+  }
+  void end() {
+    if (open) ctx->phase_end(st);
+    open = false;
+  }
+  ~ScopedPhase() { end(); }
+};
+
+// Error plumbing: every HIP failure is turned into a return code plus text.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define FLOCK_HIP_CHECK(ctx, expr)                                                          \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      (ctx)->last_error = std::string(#expr) + ": " + hipGetErrorString(_e);                \
+      return FLOCK_E_DEVICE;                                                                \
+    }                                                                                       \
+  } while (0)
+
+// Standard-basis additive-NTT twiddles for domain dimension `dim`
+// (crates/flock-core/src/ntt/additive_ntt_f128.rs:57-128): layer l, block b at
+// index (1 << l) - 1 + b.
+std::vector<F128> ntt_twiddles(int dim);
+
+}  // namespace flock

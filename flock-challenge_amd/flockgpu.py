@@ -1,0 +1,137 @@
+"""Host-side Python mirror of the reference prover interface, over the C ABI
+in include/flock_gpu.h (libflockgpu.so).
+
+Mirrors the names of crates/flock-prover/src/r1cs_hashes/blake3.rs
+(`Blake3Setup::new`, `prove_fast`, `verify`) and crates/flock-core/src/pcs/
+commit.rs (`PcsParams`, `commit`).  Every call runs the HIP kernels; there is
+no CPU fallback: if the library is missing or no GPU is present, the calls
+raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libflockgpu.so")
+
+FLOCK_OK = 0
+ERRORS = {-1: "FLOCK_E_SHAPE", -2: "FLOCK_E_OOM", -3: "FLOCK_E_DEVICE", -4: "FLOCK_E_CAP", -5: "FLOCK_E_ARG",
+          -6: "FLOCK_E_VERIFY"}
+
+PROFILE_FAST, PROFILE_SLIM, PROFILE_SECURE = 0, 1, 2
+HASH_SHA256, HASH_BLAKE3 = 0, 1
+
+
+class FlockError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class PcsParams(ctypes.Structure):
+    """PcsParams (crates/flock-core/src/pcs/commit.rs:32-50)."""
+    _fields_ = [("m", ctypes.c_uint32), ("log_inv_rate", ctypes.c_uint32), ("log_batch_size", ctypes.c_uint32),
+                ("profile", ctypes.c_uint32), ("merkle_hash", ctypes.c_uint32)]
+
+    def log_msg_len(self):
+        return self.m - 7
+
+    def log_dim(self):
+        return self.log_msg_len() - self.log_batch_size
+
+    def k_code(self):
+        return self.log_dim() + self.log_inv_rate
+
+    def n_positions(self):
+        return 1 << self.k_code()
+
+    def num_ntts(self):
+        return 1 << self.log_batch_size
+
+    def codeword_len_f128(self):
+        return self.n_positions() * self.num_ntts()
+
+
+_lib = None
+
+
+def lib():
+    """Load libflockgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python flock-challenge_amd/build.py` first")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.flock_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(P)]
+        L.flock_ctx_destroy.argtypes = [P]
+        L.flock_last_error.restype = ctypes.c_char_p
+        L.flock_last_error.argtypes = [P]
+        L.flock_commit.argtypes = [P, ctypes.POINTER(PcsParams), P, ctypes.c_size_t, P]
+        L.flock_commit_device.argtypes = [P, ctypes.POINTER(PcsParams), P, P, P, P, P]
+        L.flock_timing_enable.argtypes = [P, ctypes.c_int]
+        L.flock_timing_report.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names the header declares, for the ABI smoke test."""
+    return ["flock_ctx_create", "flock_ctx_destroy", "flock_last_error", "flock_commit", "flock_commit_device",
+            "flock_timing_enable", "flock_timing_report"]
+
+
+class Context:
+    """One GPU context (stream, constant tables, device arena)."""
+
+    def __init__(self, device=0):
+        self.handle = ctypes.c_void_p()
+        rc = lib().flock_ctx_create(device, ctypes.byref(self.handle))
+        if rc != FLOCK_OK:
+            raise FlockError(rc, f"flock_ctx_create(device={device}) failed")
+
+    def _check(self, rc):
+        if rc != FLOCK_OK:
+            raise FlockError(rc, lib().flock_last_error(self.handle).decode())
+
+    def close(self):
+        if self.handle:
+            lib().flock_ctx_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- commit
+    def commit(self, params, z_packed):
+        """`pcs::commit(z_packed, params).root` (pcs/commit.rs:250).  z_packed:
+        numpy uint64 array of 2*2^(m-7) words (lo, hi per F128)."""
+        z = np.ascontiguousarray(z_packed)
+        root = (ctypes.c_uint8 * 32)()
+        self._check(lib().flock_commit(self.handle, ctypes.byref(params), z.ctypes.data, z.nbytes, root))
+        return bytes(root)
+
+    def commit_device(self, params, z_ptr, codeword_ptr, tree_ptr, stream=None):
+        root = (ctypes.c_uint8 * 32)()
+        self._check(lib().flock_commit_device(self.handle, ctypes.byref(params), ctypes.c_void_p(z_ptr),
+                                              ctypes.c_void_p(codeword_ptr), ctypes.c_void_p(tree_ptr), root,
+                                              ctypes.c_void_p(stream) if stream else None))
+        return bytes(root)
+
+    # ---------------------------------------------------------------- timing
+    def timing(self, on=True):
+        self._check(lib().flock_timing_enable(self.handle, 1 if on else 0))
+
+    def timing_report(self):
+        """{phase: (count, total_ms)} from HIP events on the launching stream."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(lib().flock_timing_report(self.handle, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.rsplit(" ", 2)
+            out[name] = (int(cnt), float(ms))
+        return out
