@@ -14,6 +14,55 @@ struct flock_ctx {
 
 using flock::Ctx;
 
+extern "C" int flock_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int flock_device_alloc(flock_ctx *ctx, size_t bytes, void **d_ptr) {
+  if (!ctx || !d_ptr) return FLOCK_E_ARG;
+  *d_ptr = nullptr;
+  if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
+  if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) {
+    ctx->c.last_error = "hipMalloc failed";
+    *d_ptr = nullptr;
+    return FLOCK_E_OOM;
+  }
+  return FLOCK_OK;
+}
+
+extern "C" int flock_device_free(flock_ctx *ctx, void *d_ptr) {
+  if (!ctx) return FLOCK_E_ARG;
+  hipSetDevice(ctx->c.device);
+  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  FLOCK_HIP_CHECK(&ctx->c, hipFree(d_ptr));
+  return FLOCK_OK;
+}
+
+extern "C" int flock_memcpy_h2d(flock_ctx *ctx, void *d_dst, const void *src, size_t bytes) {
+  if (!ctx || (!d_dst && bytes) || (!src && bytes)) return FLOCK_E_ARG;
+  hipSetDevice(ctx->c.device);
+  FLOCK_HIP_CHECK(&ctx->c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return FLOCK_OK;
+}
+
+extern "C" int flock_memcpy_d2h(flock_ctx *ctx, void *dst, const void *d_src, size_t bytes) {
+  if (!ctx || (!d_src && bytes) || (!dst && bytes)) return FLOCK_E_ARG;
+  hipSetDevice(ctx->c.device);
+  FLOCK_HIP_CHECK(&ctx->c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return FLOCK_OK;
+}
+
+extern "C" int flock_synchronize(flock_ctx *ctx) {
+  if (!ctx) return FLOCK_E_ARG;
+  hipSetDevice(ctx->c.device);
+  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return FLOCK_OK;
+}
+
 extern "C" int flock_ctx_create(int device, flock_ctx **out) {
   if (!out) return FLOCK_E_ARG;
   *out = nullptr;

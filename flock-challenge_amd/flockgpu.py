@@ -70,6 +70,12 @@ def lib():
         L.flock_last_error.argtypes = [P]
         L.flock_commit.argtypes = [P, ctypes.POINTER(PcsParams), P, ctypes.c_size_t, P]
         L.flock_commit_device.argtypes = [P, ctypes.POINTER(PcsParams), P, P, P, P, P]
+        L.flock_device_count.argtypes = []
+        L.flock_device_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
+        L.flock_device_free.argtypes = [P, P]
+        L.flock_memcpy_h2d.argtypes = [P, P, P, ctypes.c_size_t]
+        L.flock_memcpy_d2h.argtypes = [P, P, P, ctypes.c_size_t]
+        L.flock_synchronize.argtypes = [P]
         L.flock_timing_enable.argtypes = [P, ctypes.c_int]
         L.flock_timing_report.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
@@ -79,7 +85,47 @@ def lib():
 def exported_symbols():
     """Names the header declares, for the ABI smoke test."""
     return ["flock_ctx_create", "flock_ctx_destroy", "flock_last_error", "flock_commit", "flock_commit_device",
-            "flock_timing_enable", "flock_timing_report"]
+            "flock_timing_enable", "flock_timing_report", "flock_device_count", "flock_device_alloc",
+            "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_synchronize"]
+
+
+def device_count():
+    return lib().flock_device_count()
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context (freed on close/GC)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = ctypes.c_void_p()
+        ctx._check(lib().flock_device_alloc(ctx.handle, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.ctx._check(lib().flock_memcpy_h2d(self.ctx.handle, ctypes.c_void_p(self.ptr), a.ctypes.data, a.nbytes))
+        return self
+
+    def download(self, dtype=np.uint8, count=None):
+        n = self.nbytes // np.dtype(dtype).itemsize if count is None else count
+        out = np.empty(n, dtype)
+        self.ctx._check(lib().flock_memcpy_d2h(self.ctx.handle, out.ctypes.data, ctypes.c_void_p(self.ptr),
+                                               out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            lib().flock_device_free(self.ctx.handle, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -105,6 +151,16 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        return DeviceBuffer(self, a.nbytes).upload(a)
+
+    def synchronize(self):
+        self._check(lib().flock_synchronize(self.handle))
 
     # ---------------------------------------------------------------- commit
     def commit(self, params, z_packed):

@@ -51,6 +51,9 @@ typedef struct {
     uint32_t flags;
 } flock_compression;
 
+/* Number of visible GPUs (0 when none), without creating a context. */
+int flock_device_count(void);
+
 /* Context on GPU `device`.  Replaces the process-global state the reference
  * keeps in rayon pools and scratch caches (crates/flock-core/src/lib.rs:79). */
 int flock_ctx_create(int device, flock_ctx **out);
@@ -70,6 +73,15 @@ int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, const void *z_p
  * `stream` is a hipStream_t or NULL for the context's stream. */
 int flock_commit_device(flock_ctx *ctx, const flock_pcs_params *params, const void *d_z, void *d_codeword,
                         void *d_tree, uint8_t root[32], void *stream);
+
+/* Device memory helpers for callers without their own HIP runtime handle
+ * (tests, benchmarks, FFI hosts).  Pointers are device pointers on the
+ * context's GPU; copies are synchronous with respect to the context stream. */
+int flock_device_alloc(flock_ctx *ctx, size_t bytes, void **d_ptr);
+int flock_device_free(flock_ctx *ctx, void *d_ptr);
+int flock_memcpy_h2d(flock_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+int flock_memcpy_d2h(flock_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int flock_synchronize(flock_ctx *ctx);
 
 /* Device phase timing (HIP events on the launching stream), the counterpart
  * of the reference's FLOCK_PHASE_TIMING / prove_fast_timed

@@ -30,8 +30,9 @@ def commit_message(log_len, seed=0xCA5E10CAC0111701):
 
 
 def gpu_available():
+    """Through the library's own HIP runtime (never torch's: a second HIP
+    runtime in the process would contend for the device)."""
     try:
-        import torch
-        return torch.cuda.is_available()
+        return flockgpu.device_count() > 0
     except Exception:
         return False

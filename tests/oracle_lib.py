@@ -89,6 +89,17 @@ def _declare(L):
     L.omerkle_multi_proof.argtypes = [P, SZ, P, SZ, P, SZ]
     L.omerkle_verify_multi.argtypes = [P, SZ, P, P, SZ, P, SZ]
     L.o_pcs_commit.argtypes = [ctypes.POINTER(PcsParams), P, P, P, P]
+    L.o_prove.argtypes = [P, SZ, P, SZ, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), ctypes.POINTER(SZ), P]
+    L.o_verify.argtypes = [P, SZ, SZ, P, SZ, P]
+    L.o_statement_digest.argtypes = [ctypes.c_int, P]
+    L.o_block_witness.argtypes = [P, P, P, P]
+    L.o_witness_packed.argtypes = [P, SZ, ctypes.c_int, P, P, P]
+    L.o_r1cs_matrices.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+    L.o_apply_block.argtypes = [P, P, P]
+    L.o_ext_matrix.argtypes = [P]
+    L.o_lagrange_s.argtypes = [F128, P]
+    L.o_lagrange_lambda.argtypes = [F128, P]
+    L.free.argtypes = [P]
 
 
 def ptr(a):
@@ -184,3 +195,39 @@ def pcs_commit(m, log_inv_rate, log_batch_size, z_packed_u64, want_codeword=Fals
                             ptr(tree) if tree is not None else None, ptr(root))
     assert rc == 0
     return root.tobytes(), cw, tree
+
+
+DOMAIN = b"flock-bench-v0"
+
+
+def prove(blocks, domain=DOMAIN):
+    """o_prove over a COMPRESSION_DTYPE array; returns the proof bytes."""
+    L = lib()
+    out = ctypes.POINTER(ctypes.c_uint8)()
+    n = SZ()
+    d = np.frombuffer(domain, np.uint8).copy()
+    rc = L.o_prove(ptr(blocks), blocks.size, ptr(d), len(domain), ctypes.byref(out), ctypes.byref(n), None)
+    assert rc == 0, rc
+    data = ctypes.string_at(out, n.value)
+    libc = ctypes.CDLL(None)
+    libc.free(out)
+    return data
+
+
+def verify(proof, n_blocks, blocks=None, domain=DOMAIN):
+    b = np.frombuffer(proof, np.uint8).copy()
+    d = np.frombuffer(domain, np.uint8).copy()
+    return lib().o_verify(ptr(b), b.size, n_blocks, ptr(d), len(domain), ptr(blocks) if blocks is not None else None)
+
+
+def statement_digest(m):
+    out = np.zeros(32, np.uint8)
+    lib().o_statement_digest(m, ptr(out))
+    return out.tobytes()
+
+
+def block_witness(comp):
+    z = np.zeros(256, np.uint64); a = np.zeros(256, np.uint64); b = np.zeros(256, np.uint64)
+    c = np.array([comp], dtype=COMPRESSION_DTYPE)
+    lib().o_block_witness(ptr(c), ptr(z), ptr(a), ptr(b))
+    return z, a, b

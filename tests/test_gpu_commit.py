@@ -26,19 +26,17 @@ def test_commit_root_matches_oracle(ctx, m, rate):
 
 
 def test_commit_codeword_and_tree_match_oracle(ctx):
-    import torch
     m, rate = 19, 1
     z = gpu_util.commit_message(m - 7, seed=77)
     want_root, want_cw, want_tree = O.pcs_commit(m, rate, 6, z, want_codeword=True)
     p = flockgpu.PcsParams(m, rate, 6, 0, 1)
-    dz = torch.from_numpy(z.view(np.int64)).cuda()
-    dcw = torch.empty(p.codeword_len_f128() * 2, dtype=torch.int64, device="cuda")
-    dtree = torch.empty((2 * p.n_positions() - 1) * 4, dtype=torch.int64, device="cuda")
-    torch.cuda.synchronize()
-    root = ctx.commit_device(p, dz.data_ptr(), dcw.data_ptr(), dtree.data_ptr())
+    dz = ctx.upload(z)
+    dcw = ctx.alloc(p.codeword_len_f128() * 16)
+    dtree = ctx.alloc((2 * p.n_positions() - 1) * 32)
+    root = ctx.commit_device(p, dz.ptr, dcw.ptr, dtree.ptr)
     assert root == want_root
-    assert np.array_equal(dcw.cpu().numpy().view(np.uint64), want_cw)
-    assert dtree.cpu().numpy().view(np.uint8).tobytes() == want_tree.tobytes()
+    assert np.array_equal(dcw.download(np.uint64), want_cw)
+    assert dtree.download().tobytes() == want_tree.tobytes()
 
 
 def test_commit_config2_shape(ctx):
