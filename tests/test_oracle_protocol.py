@@ -250,3 +250,23 @@ def test_wrong_statement_is_rejected(proof22):
     assert O.verify(pf, blocks.size, other) != 0
     assert O.verify(pf + b"\x00", blocks.size, blocks) != 0
     assert O.verify(pf[:-1], blocks.size, blocks) != 0
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not os.environ.get("FLOCK_SLOW_TESTS"), reason="3-minute m=32 oracle proof; set FLOCK_SLOW_TESTS=1")
+def test_ranked_canary_length_matches_reference():
+    """The reference's ranked canary (blake3.rs:4910-4939) asserts a 437,551-byte
+    proof for this input; multiproof sizes depend on every transcript byte before
+    the queries, so the length fingerprints the whole transcript."""
+    import json
+    sys_path = os.path.join(os.path.dirname(__file__), "golden")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(sys_path, "make_canary_golden.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    gold = json.load(open(os.path.join(sys_path, "canary_m32.json")))
+    blocks = mk.canary_blocks()
+    pf = O.prove(blocks)
+    assert len(pf) == 437551 == gold["reference_expected_len"]
+    assert O.blake3(pf).hex() == gold["proof_blake3"]
+    assert O.verify(pf, blocks.size, None) == 0
