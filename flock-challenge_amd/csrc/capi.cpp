@@ -2,11 +2,15 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <string>
 #include <new>
 
 #include "../../include/flock_gpu.h"
 #include "commit.h"
 #include "ctx.h"
+#include "ligerito_configs.h"
+#include "prover.h"
+#include "r1cs.h"
 
 struct flock_ctx {
   flock::Ctx c;
@@ -118,6 +122,137 @@ extern "C" int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, cons
   if (!dz || !dcw || !dtree) return FLOCK_E_OOM;
   FLOCK_HIP_CHECK(c, hipMemcpyAsync(dz, z_packed, msg_bytes, hipMemcpyHostToDevice, c->stream));
   return flock::commit_device(c, params, dz, dcw, dtree, root, c->stream);
+}
+
+// ------------------------------------------------------------------ prover
+struct flock_setup {
+  flock_ctx *ctx;
+  uint64_t n_blocks;
+  uint32_t m;
+  uint8_t digest[32];
+};
+
+extern "C" int flock_setup_create(flock_ctx *ctx, uint64_t n_blocks, uint32_t profile, uint32_t merkle_hash,
+                                  flock_setup **out) {
+  if (!ctx || !out) return FLOCK_E_ARG;
+  *out = nullptr;
+  if (profile != 0 || merkle_hash != 1) {
+    ctx->c.last_error = "setup: only the Fast profile with the Blake3 Merkle hash is implemented";
+    return FLOCK_E_ARG;
+  }
+  if (n_blocks == 0 || n_blocks > (1ull << 21)) {
+    ctx->c.last_error = "setup: n_blocks must be in 1..2^21";
+    return FLOCK_E_SHAPE;
+  }
+  uint32_t n_log = 0;
+  while ((1ull << n_log) < (n_blocks < 8 ? 8 : n_blocks)) ++n_log;
+  const uint32_t m = flock::r1cs::K_LOG + n_log;
+  bool found = false;
+  for (int i = 0; i < FLOCK_LIG_N_CONFIGS; ++i)
+    if (FLOCK_LIG_CONFIGS[i].m == (int)m && FLOCK_LIG_CONFIGS[i].profile == 0) found = true;
+  if (!found) {
+    ctx->c.last_error = "setup: no Ligerito configuration for m = " + std::to_string(m) + " (supported: 22..35)";
+    return FLOCK_E_SHAPE;
+  }
+  flock_setup *s = new (std::nothrow) flock_setup();
+  if (!s) return FLOCK_E_OOM;
+  s->ctx = ctx;
+  s->n_blocks = n_blocks;
+  s->m = m;
+  flock::r1cs::matrices();
+  flock::r1cs::statement_digest((int)m, s->digest);
+  *out = s;
+  return FLOCK_OK;
+}
+
+extern "C" void flock_setup_destroy(flock_setup *setup) { delete setup; }
+
+extern "C" int flock_setup_info(const flock_setup *setup, uint32_t *m, uint8_t digest[32]) {
+  if (!setup) return FLOCK_E_ARG;
+  if (m) *m = setup->m;
+  if (digest) memcpy(digest, setup->digest, 32);
+  return FLOCK_OK;
+}
+
+static int finish_proof(Ctx *c, std::vector<uint8_t> &bytes, uint8_t *out, size_t cap, size_t *out_len) {
+  c->last_proof.swap(bytes);
+  if (out_len) *out_len = c->last_proof.size();
+  if (!out || cap < c->last_proof.size()) {
+    c->last_error = "proof needs " + std::to_string(c->last_proof.size()) + " bytes";
+    return FLOCK_E_CAP;
+  }
+  memcpy(out, c->last_proof.data(), c->last_proof.size());
+  return FLOCK_OK;
+}
+
+static int check_prove_args(flock_setup *setup, uint64_t n_blocks, const uint8_t *domain, size_t domain_len) {
+  if (!setup || !setup->ctx || (!domain && domain_len)) return FLOCK_E_ARG;
+  if (n_blocks != setup->n_blocks) {
+    setup->ctx->c.last_error = "prove: n_blocks " + std::to_string(n_blocks) + " != setup's " +
+                               std::to_string(setup->n_blocks);
+    return FLOCK_E_SHAPE;
+  }
+  if (hipSetDevice(setup->ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
+  return FLOCK_OK;
+}
+
+extern "C" int flock_prove_blake3_device(flock_setup *setup, const flock_compression *d_blocks, uint64_t n_blocks,
+                                         const uint8_t *domain, size_t domain_len, uint8_t *out, size_t cap,
+                                         size_t *out_len, void *stream) {
+  int rc = check_prove_args(setup, n_blocks, domain, domain_len);
+  if (rc) return rc;
+  if (!d_blocks) return FLOCK_E_ARG;
+  Ctx *c = &setup->ctx->c;
+  std::vector<uint8_t> bytes;
+  rc = flock::prove_blake3_device(c, pick_stream(setup->ctx, stream), d_blocks, n_blocks, domain, domain_len, bytes);
+  if (rc) return rc;
+  return finish_proof(c, bytes, out, cap, out_len);
+}
+
+extern "C" int flock_prove_blake3(flock_setup *setup, const flock_compression *blocks, uint64_t n_blocks,
+                                  const uint8_t *domain, size_t domain_len, uint8_t *out, size_t cap,
+                                  size_t *out_len) {
+  int rc = check_prove_args(setup, n_blocks, domain, domain_len);
+  if (rc) return rc;
+  if (!blocks) return FLOCK_E_ARG;
+  Ctx *c = &setup->ctx->c;
+  void *d = c->get_scratch("prove.blocks", n_blocks * sizeof(flock_compression));
+  if (!d) return FLOCK_E_OOM;
+  FLOCK_HIP_CHECK(c, hipMemcpyAsync(d, blocks, n_blocks * sizeof(flock_compression), hipMemcpyHostToDevice, c->stream));
+  std::vector<uint8_t> bytes;
+  rc = flock::prove_blake3_device(c, c->stream, (const flock_compression *)d, n_blocks, domain, domain_len, bytes);
+  if (rc) return rc;
+  return finish_proof(c, bytes, out, cap, out_len);
+}
+
+extern "C" int flock_generate_compressions_device(flock_ctx *ctx, uint32_t log2, uint64_t seed,
+                                                  flock_compression *d_out) {
+  if (!ctx || !d_out || log2 > 30) return FLOCK_E_ARG;
+  if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
+  return flock::generate_compressions_device(&ctx->c, ctx->c.stream, log2, seed, d_out);
+}
+
+extern "C" int flock_prove_blake3_seeded(flock_setup *setup, uint32_t log2, uint64_t seed, const uint8_t *domain,
+                                         size_t domain_len, uint8_t *out, size_t cap, size_t *out_len) {
+  if (log2 > 30) return FLOCK_E_ARG;
+  int rc = check_prove_args(setup, 1ull << log2, domain, domain_len);
+  if (rc) return rc;
+  Ctx *c = &setup->ctx->c;
+  void *d = c->get_scratch("prove.blocks", (1ull << log2) * sizeof(flock_compression));
+  if (!d) return FLOCK_E_OOM;
+  if ((rc = flock::generate_compressions_device(c, c->stream, log2, seed, (flock_compression *)d))) return rc;
+  std::vector<uint8_t> bytes;
+  rc = flock::prove_blake3_device(c, c->stream, (const flock_compression *)d, 1ull << log2, domain, domain_len, bytes);
+  if (rc) return rc;
+  return finish_proof(c, bytes, out, cap, out_len);
+}
+
+extern "C" int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len) {
+  if (!ctx) return FLOCK_E_ARG;
+  if (out_len) *out_len = ctx->c.last_proof.size();
+  if (!out || cap < ctx->c.last_proof.size()) return FLOCK_E_CAP;
+  memcpy(out, ctx->c.last_proof.data(), ctx->c.last_proof.size());
+  return FLOCK_OK;
 }
 
 extern "C" int flock_timing_enable(flock_ctx *ctx, int on) {

@@ -15,6 +15,9 @@ int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k
 // Flat BLAKE3 Merkle tree of n_leaves leaves of leaf_bytes each.
 int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t leaf_bytes, uint64_t n_leaves,
                        void *d_tree);
+// Fill out with out_len/msg_len copies of msg (F128 counts, powers of two).
+int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len_f128, void *out,
+                     uint64_t out_len_f128);
 // Full commit; writes the root to host memory and synchronizes `st`.
 int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
                   uint8_t root[32], hipStream_t st);

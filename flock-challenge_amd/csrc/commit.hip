@@ -127,6 +127,14 @@ __global__ __launch_bounds__(256) void merkle_parent_kernel(const uint4 *__restr
   out[2 * i + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
+int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len_f128, void *out,
+                     uint64_t out_len_f128) {
+  hipLaunchKernelGGL(replicate_kernel, dim3(2048), dim3(256), 0, st, (const uint4 *)msg, (uint4 *)out, msg_len_f128,
+                     out_len_f128);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
 static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k, int l, int F) {
   ScopedPhase ph(ctx, "ntt_pass_kernel", st);
   uint64_t groups = 1ull << (k - F);

@@ -69,7 +69,10 @@ const F128 *Ctx::device_twiddles(int dim) {
 void *Ctx::get_scratch(const std::string &name, size_t bytes) {
   auto &b = scratch[name];
   if (b.bytes >= bytes && b.ptr) return b.ptr;
-  if (b.ptr) hipFree(b.ptr);
+  if (b.ptr) {
+    hipDeviceSynchronize();  // queued kernels may still use the old buffer
+    hipFree(b.ptr);
+  }
   b.ptr = nullptr;
   b.bytes = 0;
   if (hipMalloc(&b.ptr, bytes) != hipSuccess) {

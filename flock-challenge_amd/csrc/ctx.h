@@ -40,6 +40,10 @@ struct Ctx {
   std::map<int, std::vector<F128>> twiddles_host;
   // Named grow-only scratch buffers.
   std::map<std::string, DeviceBuffer> scratch;
+  // The R1CS transposes live in named scratch buffers once uploaded.
+  bool csc_uploaded = false;
+  // Bytes of the last proof made on this context.
+  std::vector<uint8_t> last_proof;
 
   void *get_scratch(const std::string &name, size_t bytes);
   const std::vector<F128> &host_twiddles(int dim);

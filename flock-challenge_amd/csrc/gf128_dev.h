@@ -100,3 +100,58 @@ __device__ __forceinline__ void f128_build_bytetab(f128 t, f128 *__restrict__ ta
     dst[k] = val;
   }
 }
+
+// a * x^4 mod P: the 4 bits shifted out fold back as h * 0x87 (< 2^11).
+__device__ __forceinline__ f128 f128_shl4(f128 a) {
+  uint32_t h = a.w >> 28;
+  f128 r;
+  r.w = __builtin_amdgcn_alignbit(a.w, a.z, 28);
+  r.z = __builtin_amdgcn_alignbit(a.z, a.y, 28);
+  r.y = __builtin_amdgcn_alignbit(a.y, a.x, 28);
+  r.x = (a.x << 4) ^ fold87(h);
+  return r;
+}
+
+// General product a * b with both operands varying per lane.  Each lane owns
+// a 16-entry 4-bit window table of a in LDS (entry e at tab[e * 64 + lane],
+// lane = 0..63 within the wavefront, so a wave's reads of one entry index
+// are contiguous), then runs Horner over the 32 nibbles of b.  No cross-lane
+// traffic: a lane only reads back what it wrote, so no barrier is needed.
+__device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab, uint32_t lane) {
+  f128 t1 = a, t2 = f128_mulx(a), t4 = f128_mulx(t2), t8 = f128_mulx(t4);
+  f128 t3 = f128_xor(t1, t2), t5 = f128_xor(t4, t1), t6 = f128_xor(t4, t2), t7 = f128_xor(t4, t3);
+  f128 *T = tab + lane;
+  T[0] = f128_zero();
+  T[64] = t1;
+  T[128] = t2;
+  T[192] = t3;
+  T[256] = t4;
+  T[320] = t5;
+  T[384] = t6;
+  T[448] = t7;
+  T[512] = t8;
+  T[576] = f128_xor(t8, t1);
+  T[640] = f128_xor(t8, t2);
+  T[704] = f128_xor(t8, t3);
+  T[768] = f128_xor(t8, t4);
+  T[832] = f128_xor(t8, t5);
+  T[896] = f128_xor(t8, t6);
+  T[960] = f128_xor(t8, t7);
+  f128 acc = f128_zero();
+  uint32_t w0 = b.w, w1 = b.z, w2 = b.y, w3 = b.x;
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int k = 7; k >= 0; --k) {
+      acc = f128_shl4(acc);
+      f128_xor_into(acc, T[((w0 >> (4 * k)) & 15u) << 6]);
+    }
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+  }
+  return acc;
+}
+
+// LDS bytes a wavefront needs for f128_mul scratch.
+#define F128_MUL_LDS_PER_WAVE (16 * 64 * 16)

@@ -1,0 +1,40 @@
+// Device building blocks of the prover (one launcher per kernel family).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flock_gpu.h"
+#include "ctx.h"
+
+namespace flock {
+
+int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_real, int n_log, void *d_z,
+                   void *d_a, void *d_b);
+
+}  // namespace flock
+
+namespace flock {
+size_t zc_tables_bytes();
+int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out);
+int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *c, int m,
+                     const void *d_tables, const void *d_ehi, void *d_out128);
+int fold_at_z_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_w64, uint64_t rows, void *A,
+                     void *B);
+int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out);
+int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2);
+int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
+int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3);
+int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);
+int lc_zfold_device(Ctx *ctx, hipStream_t st, const void *z, const void *eq_outer, uint64_t n_blocks, void *out);
+int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
+                   const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb);
+int rs_fold1b_device(Ctx *ctx, hipStream_t st, const void *z, const void *e0, const void *e1, uint64_t n, void *out256);
+int rs_basis_device(Ctx *ctx, hipStream_t st, const void *e0, const void *e1, const void *tables, uint64_t n, void *out);
+int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed);
+int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
+                    void *d_best);
+int gather_device(Ctx *ctx, hipStream_t st, const void *src, const uint64_t *d_idx, uint32_t n, uint32_t unit_bytes,
+                  void *out);
+int scatter_device(Ctx *ctx, hipStream_t st, void *dst, const uint64_t *d_idx, const void *d_val, uint32_t n);
+int fold_replicas_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t n, uint32_t reps, void *out);
+}  // namespace flock

@@ -1,0 +1,814 @@
+// The BLAKE3 proof on one MI355X: Blake3Setup::prove_fast
+// (crates/flock-prover/src/r1cs_hashes/blake3.rs:3405) composed as
+// prove_ligerito (crates/flock-prover/src/prover.rs:194-295).  All bulk work
+// runs in HIP kernels on the context stream; the host keeps the Fiat-Shamir
+// transcript and the O(1)-size algebra between rounds.
+#include "prover.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "commit.h"
+#include "ligerito_configs.h"
+#include "prove_kernels.h"
+#include "r1cs.h"
+#include "transcript.h"
+#include "zc_tables.h"
+
+namespace flock {
+
+namespace {
+
+const F128 ONE = F128::one();
+
+// ------------------------------------------------------------------ host algebra
+void build_eq(const F128 *r, int n, F128 *t) {
+  t[0] = ONE;
+  for (int i = 0; i < n; ++i) {
+    size_t half = (size_t)1 << i;
+    for (size_t x = 0; x < half; ++x) {
+      F128 v = t[x];
+      F128 vr = v * r[i];
+      t[x + half] = vr;
+      t[x] = v + vr;
+    }
+  }
+}
+
+std::vector<F128> eq_table(const F128 *r, int n) {
+  std::vector<F128> t((size_t)1 << n);
+  build_eq(r, n, t.data());
+  return t;
+}
+
+// Lagrange weights at z over nodes phi8(first .. first + count) within the
+// node set phi8(set_first .. set_first + set_count)
+// (zerocheck/multilinear.rs:370-389, 509-563).
+void lagrange(F128 z, int first, int count, int set_first, int set_count, F128 *w) {
+  for (int i = 0; i < count; ++i) {
+    F128 si = phi8((uint8_t)(first + i)), num = ONE, den = ONE;
+    for (int j = set_first; j < set_first + set_count; ++j) {
+      if (j == first + i) continue;
+      F128 sj = phi8((uint8_t)j);
+      num = num * (z + sj);
+      den = den * (si + sj);
+    }
+    w[i] = num * inv(den);
+  }
+}
+
+F128 dot(const F128 *a, const F128 *b, size_t n) {
+  F128 s;
+  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+uint8_t f8_inv(uint8_t a) {
+  uint8_t r = 1, sq = a;
+  for (int i = 0; i < 8; ++i) {
+    if ((0xFE >> i) & 1) r = f8_mul(r, sq);
+    sq = f8_mul(sq, sq);
+  }
+  return r;
+}
+
+// Medium constants beta_i = g / (1 + g), g = x^(2^(i-1))
+// (univariate_skip_optimized.rs:119-148).
+F128 medium(int i) {
+  F128 g(1ull << (1u << i), 0);
+  return g * inv(ONE + g);
+}
+const uint8_t SMALL_CHAL[3] = {0xF7, 0x53, 0xB5};
+
+const ZcTables &zc_tables() {
+  static ZcTables T;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    memset(&T, 0, sizeof T);
+    // S -> Lambda extension: value at 64 + j of the degree-<64 polynomial
+    // through (s, bit_s), s = 0..63 (the inverse-S / forward-Lambda LCH NTT
+    // pair of zerocheck/univariate_skip.rs:106-120 computes the same map).
+    static uint8_t M[64][64];
+    for (int j = 0; j < 64; ++j)
+      for (int s = 0; s < 64; ++s) {
+        uint8_t num = 1, den = 1;
+        for (int t = 0; t < 64; ++t) {
+          if (t == s) continue;
+          num = f8_mul(num, (uint8_t)((64 + j) ^ t));
+          den = f8_mul(den, (uint8_t)(s ^ t));
+        }
+        M[j][s] = f8_mul(num, f8_inv(den));
+      }
+    for (int p = 0; p < 16; ++p)
+      for (int v = 0; v < 16; ++v)
+        for (int jq = 0; jq < 16; ++jq) {
+          uint32_t word = 0;
+          for (int k = 0; k < 4; ++k) {
+            uint8_t acc = 0;
+            for (int t = 0; t < 4; ++t)
+              if ((v >> t) & 1) acc ^= M[4 * jq + k][4 * p + t];
+            word |= (uint32_t)acc << (8 * k);
+          }
+          T.ext[(p * 16 + v) * 16 + jq] = word;
+        }
+    uint8_t x = 1;
+    for (int i = 0; i < 255; ++i) {
+      T.log8[x] = (uint16_t)i;
+      x = f8_mul(x, 3);
+    }
+    T.log8[0] = 1024;
+    for (int i = 0; i < 3 * 1024 + 4; ++i) {
+      uint8_t e = 1;
+      if (i < 1024) {
+        for (int k = 0; k < i % 255; ++k) e = f8_mul(e, 3);
+        T.exp8[i] = e;
+      } else {
+        T.exp8[i] = 0;
+      }
+    }
+    for (int xl = 0; xl < 8; ++xl) {
+      uint8_t eps = 1;
+      for (int i = 0; i < 3; ++i) eps = f8_mul(eps, ((xl >> i) & 1) ? SMALL_CHAL[i] : (uint8_t)(SMALL_CHAL[i] ^ 1));
+      T.log_eps[xl] = T.log8[eps];
+    }
+    F128 beta[4];
+    for (int i = 0; i < 4; ++i) beta[i] = medium(i);
+    for (int xm = 0; xm < 16; ++xm) {
+      F128 e4 = ONE;
+      for (int i = 0; i < 4; ++i) e4 = e4 * (((xm >> i) & 1) ? beta[i] : ONE + beta[i]);
+      for (int t = 0; t < 8; ++t) {
+        F128 v = e4 * phi8((uint8_t)(1 << t));
+        memcpy(T.pt[xm][t], &v, 16);
+      }
+    }
+  });
+  return T;
+}
+
+int ceil_log2(size_t n) {
+  int l = 0;
+  if (n <= 1) return 0;
+  --n;
+  while (n) {
+    ++l;
+    n >>= 1;
+  }
+  return l;
+}
+
+std::vector<size_t> sample_queries(Transcript &t, size_t block_len, size_t count) {
+  std::vector<size_t> out;
+  while (out.size() < count) {
+    F128 v = t.sample();
+    size_t q = (size_t)(v.lo % block_len);
+    if (std::find(out.begin(), out.end(), q) == out.end()) out.push_back(q);
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+// Node indices of merkle_multi_proof (merkle.rs:879-925) in emission order.
+std::vector<uint64_t> multiproof_nodes(size_t num_leaves, const std::vector<size_t> &pos) {
+  std::vector<uint64_t> nodes;
+  if (pos.empty() || num_leaves == 1) return nodes;
+  std::vector<size_t> act = pos;  // already sorted + distinct
+  size_t ls = 0, ll = num_leaves;
+  while (ll > 1) {
+    std::vector<size_t> nxt;
+    for (size_t i = 0; i < act.size();) {
+      size_t p = act[i];
+      if (i + 1 < act.size() && act[i + 1] == (p ^ 1)) {
+        i += 2;
+      } else {
+        nodes.push_back(ls + (p ^ 1));
+        i += 1;
+      }
+      nxt.push_back(p >> 1);
+    }
+    act.swap(nxt);
+    ls += ll;
+    ll >>= 1;
+  }
+  return nodes;
+}
+
+struct Bytes {
+  std::vector<uint8_t> v;
+  void put(const void *p, size_t n) { v.insert(v.end(), (const uint8_t *)p, (const uint8_t *)p + n); }
+  void u64(uint64_t x) { put(&x, 8); }
+  void u32(uint32_t x) { put(&x, 4); }
+  void f(F128 x) { put(&x, 16); }
+  void fvec(const F128 *p, size_t n) {
+    u64(n);
+    put(p, 16 * n);
+  }
+};
+
+struct Opening {
+  std::vector<F128> rows;  // n x lanes
+  size_t lanes = 0, n = 0;
+  std::vector<uint8_t> hashes;  // 32 x k
+};
+
+void put_opening(Bytes &b, const Opening &o) {
+  b.u64(o.n);
+  for (size_t i = 0; i < o.n; ++i) b.fvec(&o.rows[i * o.lanes], o.lanes);
+  b.u64(o.hashes.size() / 32);
+  b.put(o.hashes.data(), o.hashes.size());
+}
+
+F128 fv(const uint4 &u) { return F128((uint64_t)u.x | ((uint64_t)u.y << 32), (uint64_t)u.z | ((uint64_t)u.w << 32)); }
+
+// A Ligero commitment on the device (ligero_commit, pcs/ligerito.rs:3340-3560).
+struct Witness {
+  void *cw = nullptr, *tree = nullptr;
+  size_t block_len = 0, lanes = 0;
+  int dim = 0, rate = 0;
+  uint8_t root[32];
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ the prover
+struct Prover {
+  Ctx *ctx;
+  hipStream_t st;
+  int m, L;
+  const flock_lig_config *cfg = nullptr;
+  std::vector<const char *> scratch_names;
+
+  void *dev(const char *name, size_t bytes) { return ctx->get_scratch(name, bytes); }
+
+  int d2h(void *h, const void *d, size_t n) {
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, st));
+    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return FLOCK_OK;
+  }
+  int h2d(void *d, const void *h, size_t n) {
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st));
+    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return FLOCK_OK;
+  }
+
+  // eq table over n variables into device buffer `out` (LSB-first).
+  int eq_device(const F128 *r, int n, void *out) {
+    int k = n < 13 ? n : 13;
+    std::vector<F128> lo = eq_table(r, k), hi = eq_table(r + k, n - k);
+    void *dlo = dev("eq.lo", lo.size() * 16), *dhi = dev("eq.hi", hi.size() * 16);
+    if (!dlo || !dhi) return FLOCK_E_OOM;
+    int rc = h2d(dlo, lo.data(), lo.size() * 16);
+    if (!rc) rc = h2d(dhi, hi.data(), hi.size() * 16);
+    if (!rc) rc = eq_expand_device(ctx, st, dlo, k, dhi, n - k, out);
+    return rc;
+  }
+
+  int grind(Transcript &t, uint32_t bits, uint64_t *nonce_out) {
+    uint64_t nonce = 0;
+    if (bits) {
+      ScopedPhase ph(ctx, "pow", st);
+      uint8_t dg[32];
+      t.state_digest(dg);
+      unsigned long long *best = (unsigned long long *)dev("pow.best", 8);
+      if (!best) return FLOCK_E_OOM;
+      uint64_t start = 0, window = 1ull << std::min<uint32_t>(bits + 2, 26);
+      for (;;) {
+        unsigned long long none = ~0ull;
+        int rc = h2d(best, &none, 8);
+        if (!rc) rc = pow_scan_device(ctx, st, dg, start, window, bits, best);
+        unsigned long long got = 0;
+        if (!rc) rc = d2h(&got, best, 8);
+        if (rc) return rc;
+        if (got != ~0ull) {
+          nonce = got;
+          break;
+        }
+        start += window;
+      }
+    }
+    t.observe_bytes(&nonce, 8);
+    *nonce_out = nonce;
+    return FLOCK_OK;
+  }
+
+  int commit_level(const void *f, int log_msg_cols, int k, int rate, const char *tag, Witness &w) {
+    ScopedPhase ph(ctx, "ligero_commit", st);
+    w.lanes = (size_t)1 << k;
+    w.dim = log_msg_cols + rate;
+    w.rate = rate;
+    w.block_len = (size_t)1 << w.dim;
+    std::string cwn = std::string("lig.cw.") + tag, trn = std::string("lig.tree.") + tag;
+    w.cw = dev(cwn.c_str(), w.block_len * w.lanes * 16);
+    w.tree = dev(trn.c_str(), (2 * w.block_len - 1) * 32);
+    if (!w.cw || !w.tree) return FLOCK_E_OOM;
+    const uint64_t msg = (uint64_t)1 << (log_msg_cols + k);
+    int rc = replicate_device(ctx, st, f, msg, w.cw, w.block_len * w.lanes);
+    if (rc) return rc;
+    const F128 *tw = ctx->device_twiddles(w.dim);
+    if (!tw) return FLOCK_E_DEVICE;
+    for (int l = rate; l < w.dim; ++l)
+      if ((rc = ntt_layer_device(ctx, st, w.cw, tw, w.dim, l, k, 0))) return rc;
+    if ((rc = merkle_tree_device(ctx, st, w.cw, (uint32_t)(16 * w.lanes), w.block_len, w.tree))) return rc;
+    return d2h(w.root, (const uint8_t *)w.tree + 32 * (2 * w.block_len - 2), 32);
+  }
+
+  int open(const Witness &w, const std::vector<size_t> &q, Opening &o) {
+    ScopedPhase ph(ctx, "open_rows", st);
+    o.n = q.size();
+    o.lanes = w.lanes;
+    o.rows.assign(o.n * o.lanes, F128());
+    std::vector<uint64_t> idx(q.begin(), q.end());
+    std::vector<uint64_t> nodes = multiproof_nodes(w.block_len, q);
+    size_t ni = idx.size() + nodes.size();
+    uint64_t *didx = (uint64_t *)dev("open.idx", ni * 8 + 8);
+    void *drows = dev("open.rows", o.rows.size() * 16 + 16), *dnodes = dev("open.nodes", nodes.size() * 32 + 32);
+    if (!didx || !drows || !dnodes) return FLOCK_E_OOM;
+    std::vector<uint64_t> all(idx);
+    all.insert(all.end(), nodes.begin(), nodes.end());
+    int rc = h2d(didx, all.data(), all.size() * 8);
+    if (!rc) rc = gather_device(ctx, st, w.cw, didx, (uint32_t)idx.size(), (uint32_t)(16 * w.lanes), drows);
+    if (!rc) rc = gather_device(ctx, st, w.tree, didx + idx.size(), (uint32_t)nodes.size(), 32, dnodes);
+    if (!rc) rc = d2h(o.rows.data(), drows, o.rows.size() * 16);
+    o.hashes.resize(nodes.size() * 32);
+    if (!rc && !nodes.empty()) rc = d2h(o.hashes.data(), dnodes, o.hashes.size());
+    return rc;
+  }
+
+  // induce_sumcheck_poly (pcs/ligerito.rs:1989-2160) as a transposed NTT:
+  // basis = R^T F^T s where s[q_i] = eq(alpha, i) and F R is the RS encoding
+  // of the Ligero level (replicate, layers rate..dim-1).
+  int induce(const Witness &w, const std::vector<size_t> &q, const std::vector<F128> &alpha, int n_cols, void *basis) {
+    ScopedPhase ph(ctx, "induce_basis", st);
+    std::vector<F128> aw = eq_table(alpha.data(), (int)alpha.size());
+    aw.resize(q.size());
+    void *sp = dev("induce.sparse", w.block_len * 16);
+    uint64_t *didx = (uint64_t *)dev("induce.idx", q.size() * 8 + 8);
+    void *dval = dev("induce.val", q.size() * 16 + 16);
+    if (!sp || !didx || !dval) return FLOCK_E_OOM;
+    FLOCK_HIP_CHECK(ctx, hipMemsetAsync(sp, 0, w.block_len * 16, st));
+    std::vector<uint64_t> idx(q.begin(), q.end());
+    int rc = h2d(didx, idx.data(), idx.size() * 8);
+    if (!rc) rc = h2d(dval, aw.data(), aw.size() * 16);
+    if (!rc) rc = scatter_device(ctx, st, sp, didx, dval, (uint32_t)q.size());
+    const F128 *tw = ctx->device_twiddles(w.dim);
+    if (!tw) return FLOCK_E_DEVICE;
+    for (int l = w.dim - 1; !rc && l >= w.rate; --l) rc = ntt_layer_device(ctx, st, sp, tw, w.dim, l, 0, 1);
+    if (!rc) rc = fold_replicas_device(ctx, st, sp, (uint64_t)1 << n_cols, 1u << w.rate, basis);
+    return rc;
+  }
+
+  int msg(const void *f, const void *b, uint64_t len, int with_eval, F128 out[3]) {
+    uint4 *d = (uint4 *)dev("lig.msg.out", 48);
+    if (!d) return FLOCK_E_OOM;
+    int rc = lig_msg_device(ctx, st, f, b, len / 2, with_eval, d);
+    uint4 h[3];
+    if (!rc) rc = d2h(h, d, 48);
+    for (int i = 0; i < 3; ++i) out[i] = fv(h[i]);
+    return rc;
+  }
+};
+
+int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
+                        const uint8_t *domain, size_t domain_len, std::vector<uint8_t> &out) {
+  if (n_blocks == 0) {
+    ctx->last_error = "prove: need at least one compression";
+    return FLOCK_E_SHAPE;
+  }
+  int n_log = 0;
+  while ((1ull << n_log) < std::max<uint64_t>(n_blocks, 8)) ++n_log;
+  const int m = r1cs::K_LOG + n_log;
+  const flock_lig_config *cfg = nullptr;
+  for (int i = 0; i < FLOCK_LIG_N_CONFIGS; ++i)
+    if (FLOCK_LIG_CONFIGS[i].m == m && FLOCK_LIG_CONFIGS[i].profile == 0) cfg = &FLOCK_LIG_CONFIGS[i];
+  if (!cfg) {
+    ctx->last_error = "prove: no Ligerito configuration for m = " + std::to_string(m) + " (supported: 22..35)";
+    return FLOCK_E_SHAPE;
+  }
+  Prover P{ctx, st, m, m - 7, cfg, {}};
+  ScopedPhase ph_all(ctx, "prove", st);
+  int rc = 0;
+  const int L = m - 7, n = m - 6;
+  const size_t words = (size_t)1 << (m - 6);  // uint64 words per bit-vector
+
+  // ---- witness + commit
+  void *z = P.dev("w.z", words * 8), *a = P.dev("w.a", words * 8), *b = P.dev("w.b", words * 8);
+  if (!z || !a || !b) return FLOCK_E_OOM;
+  if ((rc = witness_device(ctx, st, d_blocks, n_blocks, n_log, z, a, b))) return rc;
+  flock_pcs_params pp{(uint32_t)m, 1, 6, 0, 1};
+  const int k0 = L - 6 + 1;
+  const size_t npos0 = (size_t)1 << k0;
+  void *cw0 = P.dev("c.codeword", npos0 * 64 * 16), *tree0 = P.dev("c.tree", (2 * npos0 - 1) * 32);
+  if (!cw0 || !tree0) return FLOCK_E_OOM;
+  uint8_t root[32];
+  if ((rc = commit_device(ctx, &pp, z, cw0, tree0, root, st))) return rc;
+
+  Transcript t(domain, domain_len);
+  {
+    uint8_t dig[32];
+    r1cs::statement_digest(m, dig);
+    t.label("flock-r1cs-v0");
+    t.observe_bytes(dig, 32);
+    t.observe_bytes(root, 32);
+  }
+
+  Bytes B;
+  B.put("FLOCK", 5);
+  uint8_t hv[2] = {4, 2};
+  B.put(hv, 2);
+  B.put(root, 32);
+  B.u64(pp.m);
+  B.u64(pp.log_inv_rate);
+  B.u64(pp.log_batch_size);
+  B.u32(pp.profile);
+  B.u32(pp.merkle_hash);
+
+  // ---- zerocheck (zerocheck.rs:482-1364)
+  t.label("flock-zerocheck-v0");
+  std::vector<F128> r(m);
+  {
+    std::vector<F128> rs = t.sample_vec(6), ro = t.sample_vec(m - 13);
+    for (int i = 0; i < 6; ++i) r[i] = rs[i];
+    for (int i = 0; i < 3; ++i) r[6 + i] = phi8(SMALL_CHAL[i]);
+    for (int i = 0; i < 4; ++i) r[9 + i] = medium(i);
+    for (int i = 0; i < m - 13; ++i) r[13 + i] = ro[i];
+  }
+  F128 r1ab[64], r1c[64];
+  {
+    void *dt = P.dev("zc.tables", sizeof(ZcTables));
+    void *dehi = P.dev("zc.ehi", ((size_t)1 << (m - 13)) * 16);
+    void *dout = P.dev("zc.r1out", 128 * 16);
+    if (!dt || !dehi || !dout) return FLOCK_E_OOM;
+    if ((rc = P.h2d(dt, &zc_tables(), sizeof(ZcTables)))) return rc;
+    if ((rc = P.eq_device(&r[13], m - 13, dehi))) return rc;
+    if ((rc = zc_round1_device(ctx, st, a, b, z, m, dt, dehi, dout))) return rc;
+    uint4 h[128];
+    if ((rc = P.d2h(h, dout, sizeof h))) return rc;
+    for (int j = 0; j < 64; ++j) {
+      r1ab[j] = fv(h[j]);
+      r1c[j] = fv(h[64 + j]);
+    }
+  }
+  t.observe_slice(r1ab, 64);
+  t.observe_slice(r1c, 64);
+  const F128 zz = t.sample();
+  F128 wl[64], ws[64];
+  lagrange(zz, 64, 64, 64, 64, wl);
+  const F128 final_c = dot(wl, r1c, 64);
+  lagrange(zz, 0, 64, 0, 64, ws);
+  std::vector<F128> rhos;
+  std::vector<std::pair<F128, F128>> zc_rounds;
+  F128 final_a, final_b;
+  {
+    const size_t rows = words;
+    void *A = P.dev("zc.A", rows * 16), *Bv = P.dev("zc.B", rows * 16);
+    void *A2 = P.dev("zc.A2", rows * 8), *B2 = P.dev("zc.B2", rows * 8);
+    void *E = P.dev("zc.E", rows * 8), *dw = P.dev("zc.w", 64 * 16), *dmsg = P.dev("zc.msg", 32);
+    if (!A || !Bv || !A2 || !B2 || !E || !dw || !dmsg) return FLOCK_E_OOM;
+    if ((rc = P.h2d(dw, ws, sizeof ws))) return rc;
+    if ((rc = fold_at_z_device(ctx, st, a, b, dw, rows, A, Bv))) return rc;
+    size_t len = rows;
+    for (int i = 0; i < n; ++i) {
+      const size_t half = len / 2;
+      if ((rc = P.eq_device(&r[6 + i + 1], n - i - 1, E))) return rc;
+      if ((rc = zc_round_device(ctx, st, A, Bv, E, half, dmsg))) return rc;
+      uint4 h[2];
+      if ((rc = P.d2h(h, dmsg, 32))) return rc;
+      F128 g1 = fv(h[0]), gi = fv(h[1]);
+      zc_rounds.push_back({g1, gi});
+      t.observe(g1);
+      t.observe(gi);
+      F128 rho = t.sample();
+      rhos.push_back(rho);
+      if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2))) return rc;
+      std::swap(A, A2);
+      std::swap(Bv, B2);
+      len = half;
+    }
+    uint4 h[2];
+    if ((rc = P.d2h(&h[0], A, 16)) || (rc = P.d2h(&h[1], Bv, 16))) return rc;
+    final_a = fv(h[0]);
+    final_b = fv(h[1]);
+    t.observe(final_a);
+    t.observe(final_b);
+  }
+  B.fvec(r1ab, 64);
+  B.fvec(r1c, 64);
+  B.u64(zc_rounds.size());
+  for (auto &pr : zc_rounds) {
+    B.f(pr.first);
+    B.f(pr.second);
+  }
+  B.f(final_a);
+  B.f(final_b);
+  B.f(final_c);
+
+  // ---- lincheck (lincheck.rs:1480-1664)
+  F128 lc_rounds[8][2], zpart[64], r_inner_skip, w_claim, r_inner_rest[8];
+  {
+    ScopedPhase ph(ctx, "lincheck", st);
+    t.label("flock-lincheck-v0");
+    const F128 alpha = t.sample(), beta = t.sample();
+    const r1cs::Matrices &mx = r1cs::matrices();
+    uint32_t *dap = (uint32_t *)P.dev("lc.ap", mx.a_t.ptr.size() * 4), *dar = (uint32_t *)P.dev("lc.ar", mx.a_t.cols.size() * 4);
+    uint32_t *dbp = (uint32_t *)P.dev("lc.bp", mx.b_t.ptr.size() * 4), *dbr = (uint32_t *)P.dev("lc.br", mx.b_t.cols.size() * 4);
+    void *deq = P.dev("lc.eq", 16 << 14), *dcomb = P.dev("lc.comb", 16 << 14), *dzv = P.dev("lc.zv", 16 << 14);
+    void *deqo = P.dev("lc.eqo", 16ull << (m - 14));
+    if (!dap || !dar || !dbp || !dbr || !deq || !dcomb || !dzv || !deqo) return FLOCK_E_OOM;
+    if (!ctx->csc_uploaded) {
+      if ((rc = P.h2d(dap, mx.a_t.ptr.data(), mx.a_t.ptr.size() * 4)) || (rc = P.h2d(dar, mx.a_t.cols.data(), mx.a_t.cols.size() * 4)) ||
+          (rc = P.h2d(dbp, mx.b_t.ptr.data(), mx.b_t.ptr.size() * 4)) || (rc = P.h2d(dbr, mx.b_t.cols.data(), mx.b_t.cols.size() * 4)))
+        return rc;
+      ctx->csc_uploaded = true;
+    }
+    // quirky eq table (lincheck.rs:1127-1142): index i_skip + 64 i_rest
+    std::vector<F128> eqi(1 << 14);
+    {
+      F128 lam[64];
+      lagrange(zz, 0, 64, 0, 64, lam);
+      std::vector<F128> er = eq_table(rhos.data(), 8);
+      for (int ir = 0; ir < 256; ++ir)
+        for (int is = 0; is < 64; ++is) eqi[is + 64 * ir] = lam[is] * er[ir];
+    }
+    if ((rc = P.h2d(deq, eqi.data(), eqi.size() * 16))) return rc;
+    if ((rc = lc_comb_device(ctx, st, dap, dar, dbp, dbr, deq, alpha, beta, r1cs::Z_CONST, dcomb))) return rc;
+    if ((rc = P.eq_device(&rhos[8], m - 14, deqo))) return rc;
+    if ((rc = lc_zfold_device(ctx, st, z, deqo, 1ull << (m - 14), dzv))) return rc;
+    std::vector<F128> comb(1 << 14), zv(1 << 14);
+    if ((rc = P.d2h(comb.data(), dcomb, comb.size() * 16)) || (rc = P.d2h(zv.data(), dzv, zv.size() * 16))) return rc;
+    size_t len = 1 << 14;
+    F128 rr[8];
+    for (int rnd = 0; rnd < 8; ++rnd) {
+      size_t half = len / 2;
+      F128 e1, einf;
+      for (size_t i = 0; i < half; ++i) {
+        e1 += comb[i + half] * zv[i + half];
+        einf += (comb[i] + comb[i + half]) * (zv[i] + zv[i + half]);
+      }
+      lc_rounds[rnd][0] = e1;
+      lc_rounds[rnd][1] = einf;
+      t.observe(e1);
+      t.observe(einf);
+      F128 rv = t.sample();
+      rr[rnd] = rv;
+      for (size_t i = 0; i < half; ++i) {
+        comb[i] = comb[i] + rv * (comb[i + half] + comb[i]);
+        zv[i] = zv[i] + rv * (zv[i + half] + zv[i]);
+      }
+      len = half;
+    }
+    for (int i = 0; i < 64; ++i) zpart[i] = zv[i];
+    t.observe_slice(zpart, 64);
+    r_inner_skip = t.sample();
+    F128 lam[64];
+    lagrange(r_inner_skip, 0, 64, 0, 64, lam);
+    w_claim = dot(lam, zpart, 64);
+    for (int i = 0; i < 8; ++i) r_inner_rest[i] = rr[7 - i];
+  }
+  B.u64(8);
+  for (int i = 0; i < 8; ++i) {
+    B.f(lc_rounds[i][0]);
+    B.f(lc_rounds[i][1]);
+  }
+  B.fvec(zpart, 64);
+
+  // ---- claims (r1cs.rs:286-330): x_full = x_inner_rest ++ x_outer
+  std::vector<F128> xf[2];
+  xf[0].assign(r_inner_rest, r_inner_rest + 8);
+  xf[0].insert(xf[0].end(), rhos.begin() + 8, rhos.end());
+  xf[1].assign(r.begin() + 6, r.end());
+
+  // ---- PCS open: ring switch (ring_switch.rs:2708-2780, 3040-3117)
+  t.label("flock-pcs-open-batch-v0");
+  const size_t N = (size_t)1 << L;
+  void *e0 = P.dev("rs.e0", N * 16), *e1 = P.dev("rs.e1", N * 16), *cb = P.dev("lig.cb", N * 16);
+  void *d256 = P.dev("rs.s", 256 * 16), *dtab = P.dev("rs.tab", 2 * 32 * 16 * 16);
+  if (!e0 || !e1 || !cb || !d256 || !dtab) return FLOCK_E_OOM;
+  F128 shv[2][128], sclaim[2], eqr[2][128];
+  if ((rc = P.eq_device(&xf[0][1], L, e0)) || (rc = P.eq_device(&xf[1][1], L, e1))) return rc;
+  if ((rc = rs_fold1b_device(ctx, st, z, e0, e1, N, d256))) return rc;
+  {
+    uint4 h[256];
+    if ((rc = P.d2h(h, d256, sizeof h))) return rc;
+    for (int i = 0; i < 128; ++i) {
+      shv[0][i] = fv(h[i]);
+      shv[1][i] = fv(h[128 + i]);
+    }
+  }
+  for (int k = 0; k < 2; ++k) {
+    t.label("flock-ring-switch-v0");
+    t.observe_slice(shv[k], 128);
+    std::vector<F128> rd = t.sample_vec(7);
+    build_eq(rd.data(), 7, eqr[k]);
+    // s_hat_u = bit transpose of s_hat_v (ring_switch.rs:1654-1718)
+    F128 u[128];
+    for (int bb = 0; bb < 128; ++bb) {
+      uint64_t lo = 0, hi = 0;
+      for (int i = 0; i < 128; ++i) {
+        uint64_t bit = bb < 64 ? (shv[k][i].lo >> bb) & 1 : (shv[k][i].hi >> (bb - 64)) & 1;
+        if (i < 64) lo |= bit << i;
+        else hi |= bit << (i - 64);
+      }
+      u[bb] = F128(lo, hi);
+    }
+    sclaim[k] = dot(u, eqr[k], 128);
+  }
+  F128 gam[2] = {t.sample(), t.sample()};
+  {
+    // nibble tables of gamma_k eq(r''_k)
+    std::vector<F128> tab(2 * 32 * 16);
+    for (int k = 0; k < 2; ++k) {
+      F128 wv[128];
+      for (int bb = 0; bb < 128; ++bb) wv[bb] = gam[k] * eqr[k][bb];
+      for (int p = 0; p < 32; ++p)
+        for (int v = 0; v < 16; ++v) {
+          F128 s;
+          for (int q = 0; q < 4; ++q)
+            if ((v >> q) & 1) s += wv[4 * p + q];
+          tab[k * 512 + p * 16 + v] = s;
+        }
+    }
+    if ((rc = P.h2d(dtab, tab.data(), tab.size() * 16))) return rc;
+    if ((rc = rs_basis_device(ctx, st, e0, e1, dtab, N, cb))) return rc;
+  }
+  const F128 target = gam[0] * sclaim[0] + gam[1] * sclaim[1];
+  B.u64(2);
+  B.fvec(shv[0], 128);
+  B.fvec(shv[1], 128);
+
+  // ---- Ligerito (pcs/ligerito.rs:6222-6980)
+  std::vector<F128> msgs;
+  std::vector<uint64_t> nonces, fold_nonces;
+  std::vector<F128> oods, yr;
+  std::vector<std::array<uint8_t, 32>> roots;
+  std::vector<Opening> rec_open;
+  Opening open0, open_last;
+  {
+    ScopedPhase ph(ctx, "ligerito", st);
+    t.label("flock-ligerito-basis-v0");
+    t.observe(target);
+    t.observe_bytes(root, 32);
+    void *f = z, *fb = P.dev("lig.f.a", N * 8), *fc = P.dev("lig.f.b", N * 8);
+    void *cbb = P.dev("lig.cb.b", N * 8), *cbc = P.dev("lig.cb.c", N * 8);
+    void *basis = P.dev("lig.basis", N * 8);
+    if (!fb || !fc || !cbb || !cbc || !basis) return FLOCK_E_OOM;
+    void *cur_cb = cb;
+    size_t len = N;
+    F128 o3[3];
+    auto push = [&](F128 u0, F128 u2) {
+      msgs.push_back(u0);
+      msgs.push_back(u2);
+      t.observe(u0);
+      t.observe(u2);
+    };
+    auto fold = [&](F128 rv) -> int {
+      void *nf = (f == fb) ? fc : fb, *ncb = (cur_cb == cbb) ? cbc : cbb;
+      int e = fold_pair_device(ctx, st, f, cur_cb, len / 2, rv, nf, ncb);
+      if (e) return e;
+      f = nf;
+      cur_cb = ncb;
+      len /= 2;
+      if ((e = P.msg(f, cur_cb, len, 0, o3))) return e;
+      push(o3[0], o3[1]);
+      return FLOCK_OK;
+    };
+    auto glue = [&](const void *bn, F128 be) { return axpy_device(ctx, st, cur_cb, bn, len, be); };
+    if ((rc = P.msg(f, cur_cb, len, 0, o3))) return rc;
+    push(o3[0], o3[1]);
+    const int ik = cfg->initial_k;
+    std::vector<F128> ris;
+    for (int j = 0; j < ik; ++j) {
+      int bits = cfg->levels[0].fold_grinding_bits - j;
+      if (bits > 0) {
+        uint64_t nc;
+        if ((rc = P.grind(t, (uint32_t)bits, &nc))) return rc;
+        fold_nonces.push_back(nc);
+      }
+      F128 rj = t.sample();
+      ris.push_back(rj);
+      if ((rc = fold(rj))) return rc;
+    }
+    const int n1 = L - ik;
+    Witness prev;
+    prev.cw = cw0;
+    prev.tree = tree0;
+    prev.lanes = 64;
+    prev.block_len = npos0;
+    prev.dim = k0;
+    prev.rate = 1;
+    Witness nxt;
+    if ((rc = P.commit_level(f, n1 - cfg->levels[1].k_recursive, cfg->levels[1].k_recursive,
+                             cfg->levels[1].log_inv_rate, "1", nxt)))
+      return rc;
+    std::array<uint8_t, 32> rt;
+    memcpy(rt.data(), nxt.root, 32);
+    roots.push_back(rt);
+    t.observe_bytes(nxt.root, 32);
+    auto ood = [&](int count, int nvars) -> int {
+      void *deq = P.dev("lig.eqz", (16ull << nvars));
+      if (!deq) return FLOCK_E_OOM;
+      for (int o = 0; o < count; ++o) {
+        std::vector<F128> zo = t.sample_vec(nvars);
+        int e = P.eq_device(zo.data(), nvars, deq);
+        if (!e) e = P.msg(f, deq, len, 1, o3);
+        if (e) return e;
+        t.observe(o3[2]);
+        oods.push_back(o3[2]);
+        push(o3[0], o3[1]);
+        F128 be = t.sample();
+        if ((e = glue(deq, be))) return e;
+      }
+      return FLOCK_OK;
+    };
+    if ((rc = ood(cfg->levels[1].ood_samples, n1))) return rc;
+    uint64_t nc;
+    if ((rc = P.grind(t, (uint32_t)cfg->levels[0].grinding_bits, &nc))) return rc;
+    nonces.push_back(nc);
+    std::vector<size_t> q = sample_queries(t, prev.block_len, (size_t)cfg->levels[0].queries);
+    std::vector<F128> al = t.sample_vec((size_t)ceil_log2(q.size()));
+    if ((rc = P.open(prev, q, open0))) return rc;
+    if ((rc = P.induce(prev, q, al, n1, basis))) return rc;
+    if ((rc = P.msg(f, basis, len, 0, o3))) return rc;
+    push(o3[0], o3[1]);
+    F128 be = t.sample();
+    if ((rc = glue(basis, be))) return rc;
+    prev = nxt;
+    const int R = cfg->n_levels - 1;
+    for (int i = 0; i < R; ++i) {
+      const int ki = cfg->levels[i + 1].k_recursive;
+      std::vector<F128> lr;
+      for (int j = 0; j < ki; ++j) {
+        int bits = cfg->levels[i + 1].fold_grinding_bits - j;
+        if (bits > 0) {
+          if ((rc = P.grind(t, (uint32_t)bits, &nc))) return rc;
+          fold_nonces.push_back(nc);
+        }
+        F128 rj = t.sample();
+        lr.push_back(rj);
+        if ((rc = fold(rj))) return rc;
+      }
+      if (i == R - 1) {
+        yr.resize(len);
+        if ((rc = P.d2h(yr.data(), f, len * 16))) return rc;
+        for (auto &v : yr) t.observe(v);
+        if ((rc = P.grind(t, (uint32_t)cfg->levels[i + 1].grinding_bits, &nc))) return rc;
+        nonces.push_back(nc);
+        q = sample_queries(t, prev.block_len, (size_t)cfg->levels[i + 1].queries);
+        if ((rc = P.open(prev, q, open_last))) return rc;
+        break;
+      }
+      int nn = 0;
+      while (((size_t)1 << nn) < len) ++nn;
+      const int kn = cfg->levels[i + 2].k_recursive;
+      Witness w2;
+      const char *tag = (i % 2 == 0) ? "2" : "1";
+      if ((rc = P.commit_level(f, nn - kn, kn, cfg->levels[i + 2].log_inv_rate, tag, w2))) return rc;
+      memcpy(rt.data(), w2.root, 32);
+      roots.push_back(rt);
+      t.observe_bytes(w2.root, 32);
+      if ((rc = ood(cfg->levels[i + 2].ood_samples, nn))) return rc;
+      if ((rc = P.grind(t, (uint32_t)cfg->levels[i + 1].grinding_bits, &nc))) return rc;
+      nonces.push_back(nc);
+      q = sample_queries(t, prev.block_len, (size_t)cfg->levels[i + 1].queries);
+      al = t.sample_vec((size_t)ceil_log2(q.size()));
+      Opening op;
+      if ((rc = P.open(prev, q, op))) return rc;
+      rec_open.push_back(op);
+      if ((rc = P.induce(prev, q, al, nn, basis))) return rc;
+      if ((rc = P.msg(f, basis, len, 0, o3))) return rc;
+      push(o3[0], o3[1]);
+      be = t.sample();
+      if ((rc = glue(basis, be))) return rc;
+      prev = w2;
+    }
+  }
+  // LigeritoProof (pcs/ligerito.rs:1504-1545)
+  {
+    uint8_t root0[32];
+    memcpy(root0, root, 32);
+    B.put(root0, 32);
+    put_opening(B, open0);
+    B.u64(roots.size());
+    for (auto &rt : roots) B.put(rt.data(), 32);
+    B.u64(rec_open.size());
+    for (auto &o : rec_open) put_opening(B, o);
+    B.fvec(yr.data(), yr.size());
+    put_opening(B, open_last);
+    B.u64(msgs.size() / 2);
+    B.put(msgs.data(), msgs.size() * 16);
+    B.u64(nonces.size());
+    B.put(nonces.data(), nonces.size() * 8);
+    B.fvec(oods.data(), oods.size());
+    B.u64(fold_nonces.size());
+    B.put(fold_nonces.data(), fold_nonces.size() * 8);
+  }
+  ph_all.end();
+  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  out.swap(B.v);
+  return FLOCK_OK;
+}
+
+}  // namespace flock

@@ -1,0 +1,432 @@
+// GF(2^128) vector kernels for the sumcheck-style phases of the prover:
+// multilinear zerocheck rounds, eq-table expansion, lincheck folds, ring
+// switch, the Ligerito sumcheck, a generic additive-NTT layer (forward and
+// transposed), the BLAKE3 proof-of-work scan and index gathers.
+//
+// Reductions write one partial per workgroup; xor_rows_device (zerocheck.hip)
+// or the host XORs them.  Kernels whose products have both operands varying
+// per lane use f128_mul with a 16 KiB per-wavefront LDS table; products by a
+// workgroup-uniform scalar use the byte tables of gf128_dev.h.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "../../include/flock_gpu.h"
+#include "blake3_dev.h"
+#include "ctx.h"
+#include "gf128_dev.h"
+#include "prove_kernels.h"
+
+namespace flock {
+
+#define MUL_WAVES 2
+#define MUL_THREADS (64 * MUL_WAVES)
+
+__device__ __forceinline__ void wg_xor_reduce(f128 v, f128 *red, f128 *out) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = f128_xor(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- eq tables
+// out[i] = lo[i & (2^k - 1)] * hi[i >> k].  Each workgroup covers a range
+// inside one hi index, so hi is a workgroup-uniform multiplier.
+__global__ __launch_bounds__(256) void eq_expand_kernel(const f128 *__restrict__ lo, int k, const f128 *__restrict__ hi,
+                                                        uint64_t per_wg, f128 *__restrict__ out) {
+  __shared__ f128 tab[512];
+  const uint64_t lo_n = 1ull << k;
+  const uint64_t chunks = lo_n / per_wg;
+  const uint64_t h = blockIdx.x / chunks, c0 = (blockIdx.x % chunks) * per_wg;
+  if (threadIdx.x < 64) f128_build_bytetab(hi[h], tab, threadIdx.x);
+  __syncthreads();
+  for (uint64_t i = c0 + threadIdx.x; i < c0 + per_wg; i += 256) out[(h << k) + i] = f128_mul_bytetab(lo[i], tab);
+}
+
+int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out) {
+  uint64_t lo_n = 1ull << k;
+  uint64_t per = lo_n < 4096 ? lo_n : 4096;
+  uint64_t wgs = (lo_n / per) << n_hi_log;
+  hipLaunchKernelGGL(eq_expand_kernel, dim3((unsigned)wgs), dim3(256), 0, st, (const f128 *)lo, k, (const f128 *)hi,
+                     per, (f128 *)out);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- zerocheck rounds
+// (G(1), G(inf)) of round_pair_naive (zerocheck/multilinear.rs:639-680):
+// sum_x' E[x'] A1 B1 and sum_x' E[x'] (A0 + A1)(B0 + B1).
+__global__ __launch_bounds__(MUL_THREADS) void zc_round_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
+                                                               const f128 *__restrict__ E, uint64_t half,
+                                                               f128 *__restrict__ partial) {
+  __shared__ f128 mt[MUL_WAVES][1024];
+  __shared__ f128 red[MUL_THREADS];
+  const uint32_t lane = threadIdx.x & 63;
+  f128 *tab = mt[threadIdx.x >> 6];
+  f128 g1 = f128_zero(), gi = f128_zero();
+  for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
+    const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1], e = E[x];
+    const f128 p1 = f128_mul(a1, b1, tab, lane);
+    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, lane);
+    f128_xor_into(g1, f128_mul(e, p1, tab, lane));
+    f128_xor_into(gi, f128_mul(e, pi, tab, lane));
+  }
+  wg_xor_reduce(g1, red, &partial[2 * blockIdx.x]);
+  wg_xor_reduce(gi, red, &partial[2 * blockIdx.x + 1]);
+}
+
+static unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
+  uint64_t g = (work + threads - 1) / threads;
+  if (g > cap) g = cap;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half,
+                    void *out2) {
+  ScopedPhase ph(ctx, "zc_round", st);
+  unsigned g = grid_for(half, MUL_THREADS, 4096);
+  f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL(zc_round_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
+                     (const f128 *)E, half, part);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return xor_rows_device(ctx, st, part, g, 2, out2);
+}
+
+// v'[x] = v[2x] + r (v[2x] + v[2x+1]) for two vectors (fold_in_place_pair,
+// multilinear.rs:3209-3230; partial_eval_lsb, pcs/ligerito.rs:1673).
+__global__ __launch_bounds__(256) void fold_pair_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
+                                                        uint64_t half, f128 r, f128 *__restrict__ A2,
+                                                        f128 *__restrict__ B2) {
+  __shared__ f128 tab[512];
+  if (threadIdx.x < 64) f128_build_bytetab(r, tab, threadIdx.x);
+  __syncthreads();
+  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < half; x += (uint64_t)gridDim.x * 256) {
+    const f128 a0 = A[2 * x], a1 = A[2 * x + 1];
+    A2[x] = f128_xor(a0, f128_mul_bytetab(f128_xor(a0, a1), tab));
+    if (B) {
+      const f128 b0 = B[2 * x], b1 = B[2 * x + 1];
+      B2[x] = f128_xor(b0, f128_mul_bytetab(f128_xor(b0, b1), tab));
+    }
+  }
+}
+
+int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2) {
+  ScopedPhase ph(ctx, "fold_pair", st);
+  f128 rr = make_uint4((uint32_t)r.lo, (uint32_t)(r.lo >> 32), (uint32_t)r.hi, (uint32_t)(r.hi >> 32));
+  hipLaunchKernelGGL(fold_pair_kernel, dim3(grid_for(half, 256, 8192)), dim3(256), 0, st, (const f128 *)A,
+                     (const f128 *)B, half, rr, (f128 *)A2, (f128 *)B2);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- Ligerito sumcheck
+// round_msg_lsb (pcs/ligerito.rs:3725-3772): u0 = sum f0 b0,
+// u2 = sum (f0 + f1)(b0 + b1); with_eval adds y = sum f0 b0 + f1 b1
+// (round_msg_and_eval_lsb, :3775-3816).
+__global__ __launch_bounds__(MUL_THREADS) void lig_msg_kernel(const f128 *__restrict__ f, const f128 *__restrict__ b,
+                                                              uint64_t half, int with_eval,
+                                                              f128 *__restrict__ partial) {
+  __shared__ f128 mt[MUL_WAVES][1024];
+  __shared__ f128 red[MUL_THREADS];
+  const uint32_t lane = threadIdx.x & 63;
+  f128 *tab = mt[threadIdx.x >> 6];
+  f128 u0 = f128_zero(), u2 = f128_zero(), y = f128_zero();
+  for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
+    const f128 f0 = f[2 * x], f1 = f[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+    const f128 e0 = f128_mul(f0, b0, tab, lane);
+    f128_xor_into(u0, e0);
+    f128_xor_into(u2, f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, lane));
+    if (with_eval) f128_xor_into(y, f128_xor(e0, f128_mul(f1, b1, tab, lane)));
+  }
+  wg_xor_reduce(u0, red, &partial[3 * blockIdx.x]);
+  wg_xor_reduce(u2, red, &partial[3 * blockIdx.x + 1]);
+  wg_xor_reduce(y, red, &partial[3 * blockIdx.x + 2]);
+}
+
+int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3) {
+  ScopedPhase ph(ctx, "lig_msg", st);
+  unsigned g = grid_for(half, MUL_THREADS, 4096);
+  f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL(lig_msg_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b, half,
+                     with_eval, part);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return xor_rows_device(ctx, st, part, g, 3, out3);
+}
+
+// acc[i] += beta * x[i]  (SumcheckProver::glue, pcs/ligerito.rs:5766-5790)
+__global__ __launch_bounds__(256) void axpy_kernel(f128 *__restrict__ acc, const f128 *__restrict__ x, uint64_t n,
+                                                   f128 beta) {
+  __shared__ f128 tab[512];
+  if (threadIdx.x < 64) f128_build_bytetab(beta, tab, threadIdx.x);
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    f128_xor_into(acc[i], f128_mul_bytetab(x[i], tab));
+}
+
+int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta) {
+  f128 bb = make_uint4((uint32_t)beta.lo, (uint32_t)(beta.lo >> 32), (uint32_t)beta.hi, (uint32_t)(beta.hi >> 32));
+  hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (f128 *)acc, (const f128 *)x, n, bb);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- lincheck
+// z_vec[i] = sum_blk eq_outer[blk] * bit_i(z block blk), i < 2^14
+// (partial_fold_packed_z, lincheck.rs:660-845).  Workgroup = 4 words of the
+// 256-word block x a range of blocks; results XORed into out with atomics.
+__global__ __launch_bounds__(256) void lc_zfold_kernel(const uint64_t *__restrict__ z, const f128 *__restrict__ eq,
+                                                       uint64_t n_blocks, uint64_t blocks_per_wg,
+                                                       uint32_t *__restrict__ out) {
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const uint64_t b0 = (uint64_t)blockIdx.y * blocks_per_wg;
+  uint64_t b1 = b0 + blocks_per_wg;
+  if (b1 > n_blocks) b1 = n_blocks;
+  f128 acc = f128_zero();
+  for (uint64_t blk = b0; blk < b1; ++blk) {
+    const uint64_t word = z[blk * 256 + w];
+    const uint32_t mask = 0u - (uint32_t)((word >> lane) & 1);
+    const f128 e = eq[blk];
+    acc.x ^= e.x & mask;
+    acc.y ^= e.y & mask;
+    acc.z ^= e.z & mask;
+    acc.w ^= e.w & mask;
+  }
+  uint32_t *o = out + 4 * (64 * w + lane);
+  atomicXor(o + 0, acc.x);
+  atomicXor(o + 1, acc.y);
+  atomicXor(o + 2, acc.z);
+  atomicXor(o + 3, acc.w);
+}
+
+int lc_zfold_device(Ctx *ctx, hipStream_t st, const void *z, const void *eq_outer, uint64_t n_blocks, void *out) {
+  ScopedPhase ph(ctx, "lc_zfold", st);
+  FLOCK_HIP_CHECK(ctx, hipMemsetAsync(out, 0, 16 << 14, st));
+  uint64_t ranges = n_blocks < 64 ? 1 : 64;
+  uint64_t per = (n_blocks + ranges - 1) / ranges;
+  hipLaunchKernelGGL(lc_zfold_kernel, dim3(64, (unsigned)ranges), dim3(256), 0, st, (const uint64_t *)z,
+                     (const f128 *)eq_outer, n_blocks, per, (uint32_t *)out);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// comb[c] = alpha (A_0^T eq)[c] + (B_0^T eq)[c] (+ beta at the pinned column)
+// (fold_alpha_batched, lincheck.rs:1158-1240; const pin, :1530-1545).
+__global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict__ ap, const uint32_t *__restrict__ ar,
+                                                      const uint32_t *__restrict__ bp, const uint32_t *__restrict__ br,
+                                                      const f128 *__restrict__ eq, f128 alpha, f128 beta, uint32_t pin,
+                                                      f128 *__restrict__ comb) {
+  __shared__ f128 tab[512];
+  if (threadIdx.x < 64) f128_build_bytetab(alpha, tab, threadIdx.x);
+  __syncthreads();
+  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+  f128 ta = f128_zero(), tb = f128_zero();
+  for (uint32_t k = ap[c]; k < ap[c + 1]; ++k) f128_xor_into(ta, eq[ar[k]]);
+  for (uint32_t k = bp[c]; k < bp[c + 1]; ++k) f128_xor_into(tb, eq[br[k]]);
+  f128 v = f128_xor(f128_mul_bytetab(ta, tab), tb);
+  if (c == pin) f128_xor_into(v, beta);
+  comb[c] = v;
+}
+
+int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
+                   const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb) {
+  ScopedPhase ph(ctx, "lc_comb", st);
+  auto cv = [](F128 v) { return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32)); };
+  hipLaunchKernelGGL(lc_comb_kernel, dim3((1 << 14) / 256), dim3(256), 0, st, ap, ar, bp, br, (const f128 *)eq,
+                     cv(alpha), cv(beta), pin, (f128 *)comb);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- ring switch
+// s_hat_v[k][r] = sum_i bit_r(z[i]) eq_k[i] for two claims (fold_1b_rows,
+// ring_switch.rs:1568-1618).  Thread = bit r; workgroup = element range.
+__global__ __launch_bounds__(128) void rs_fold1b_kernel(const f128 *__restrict__ z, const f128 *__restrict__ e0,
+                                                        const f128 *__restrict__ e1, uint64_t n, uint64_t per_wg,
+                                                        f128 *__restrict__ partial) {
+  const uint32_t r = threadIdx.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * per_wg;
+  uint64_t i1 = i0 + per_wg;
+  if (i1 > n) i1 = n;
+  f128 s0 = f128_zero(), s1 = f128_zero();
+  const uint32_t word = r >> 5, sh = r & 31;
+  for (uint64_t i = i0; i < i1; ++i) {
+    const f128 zi = z[i];
+    const uint32_t w = word == 0 ? zi.x : word == 1 ? zi.y : word == 2 ? zi.z : zi.w;
+    const uint32_t mask = 0u - ((w >> sh) & 1u);
+    const f128 a = e0[i], b = e1[i];
+    s0.x ^= a.x & mask; s0.y ^= a.y & mask; s0.z ^= a.z & mask; s0.w ^= a.w & mask;
+    s1.x ^= b.x & mask; s1.y ^= b.y & mask; s1.z ^= b.z & mask; s1.w ^= b.w & mask;
+  }
+  partial[(uint64_t)blockIdx.x * 256 + r] = s0;
+  partial[(uint64_t)blockIdx.x * 256 + 128 + r] = s1;
+}
+
+int rs_fold1b_device(Ctx *ctx, hipStream_t st, const void *z, const void *e0, const void *e1, uint64_t n, void *out256) {
+  ScopedPhase ph(ctx, "rs_fold1b", st);
+  uint64_t wgs = n < 4096 ? 1 : 4096;
+  uint64_t per = (n + wgs - 1) / wgs;
+  wgs = (n + per - 1) / per;
+  f128 *part = (f128 *)ctx->get_scratch("rs.fold1b.partial", wgs * 256 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL(rs_fold1b_kernel, dim3((unsigned)wgs), dim3(128), 0, st, (const f128 *)z, (const f128 *)e0,
+                     (const f128 *)e1, n, per, part);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return xor_rows_device(ctx, st, part, wgs, 256, out256);
+}
+
+// b[i] = sum_k sum_bits bit_b(eq_k[i]) w_k[b] with w_k = gamma_k eq(r''_k)
+// (fold_b128_elems, ring_switch.rs:1720-1760, combined as in pcs.rs:1025-1100)
+// through nibble tables T[k][p][v] (2 x 32 x 16 F128, host-built).
+__global__ __launch_bounds__(256) void rs_basis_kernel(const f128 *__restrict__ e0, const f128 *__restrict__ e1,
+                                                       const f128 *__restrict__ T, uint64_t n, f128 *__restrict__ out) {
+  __shared__ f128 tab[2 * 32 * 16];
+  for (uint32_t i = threadIdx.x; i < 2 * 32 * 16; i += 256) tab[i] = T[i];
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const f128 a = e0[i], b = e1[i];
+    f128 s = f128_zero();
+#pragma unroll
+    for (int p = 0; p < 32; ++p) {
+      const uint32_t wa = p < 8 ? a.x : p < 16 ? a.y : p < 24 ? a.z : a.w;
+      const uint32_t wb = p < 8 ? b.x : p < 16 ? b.y : p < 24 ? b.z : b.w;
+      f128_xor_into(s, tab[p * 16 + ((wa >> (4 * (p & 7))) & 15)]);
+      f128_xor_into(s, tab[512 + p * 16 + ((wb >> (4 * (p & 7))) & 15)]);
+    }
+    out[i] = s;
+  }
+}
+
+int rs_basis_device(Ctx *ctx, hipStream_t st, const void *e0, const void *e1, const void *tables, uint64_t n, void *out) {
+  ScopedPhase ph(ctx, "rs_basis", st);
+  hipLaunchKernelGGL(rs_basis_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (const f128 *)e0,
+                     (const f128 *)e1, (const f128 *)tables, n, (f128 *)out);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- generic NTT layer
+// One butterfly layer over 2^k positions x NL lanes; twiddle per block from
+// the breadth-first table.  Forward (additive_ntt_f128.rs:1281-1320):
+// u += t v, v += u.  Transposed: s = u + v, u = s, v = t s + v.
+__global__ __launch_bounds__(MUL_THREADS) void ntt_layer_kernel(f128 *__restrict__ d, const f128 *__restrict__ tw,
+                                                                int k, int l, int log_nl, int transposed) {
+  __shared__ f128 mt[MUL_WAVES][1024];
+  const uint32_t lane = threadIdx.x & 63;
+  f128 *tab = mt[threadIdx.x >> 6];
+  const uint64_t total = 1ull << (k - 1 + log_nl);
+  for (uint64_t t = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; t < total; t += (uint64_t)gridDim.x * MUL_THREADS) {
+    const uint64_t ln = t & ((1ull << log_nl) - 1), bf = t >> log_nl;
+    const int sh = k - l - 1;
+    const uint64_t block = bf >> sh, row = bf & ((1ull << sh) - 1);
+    const uint64_t top = (block << (sh + 1)) + row, bot = top + (1ull << sh);
+    const f128 w = tw[(1ull << l) - 1 + block];
+    f128 u = d[(top << log_nl) + ln], v = d[(bot << log_nl) + ln];
+    if (!transposed) {
+      f128_xor_into(u, f128_mul(v, w, tab, lane));
+      f128_xor_into(v, u);
+    } else {
+      const f128 s = f128_xor(u, v);
+      u = s;
+      v = f128_xor(f128_mul(s, w, tab, lane), v);
+    }
+    d[(top << log_nl) + ln] = u;
+    d[(bot << log_nl) + ln] = v;
+  }
+}
+
+int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed) {
+  const uint64_t total = 1ull << (k - 1 + log_nl);
+  hipLaunchKernelGGL(ntt_layer_kernel, dim3(grid_for(total, MUL_THREADS, 8192)), dim3(MUL_THREADS), 0, st, (f128 *)d,
+                     (const f128 *)tw, k, l, log_nl, transposed);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- proof of work
+// BLAKE3(digest | nonce_le | 0^24) as one CHUNK_START|CHUNK_END|ROOT block; the
+// leading `bits` bits of the digest bytes must be zero (challenger.rs:1246-1300).
+// The smallest satisfying nonce of the window wins (atomicMin).
+__global__ __launch_bounds__(256) void pow_kernel(uint4 d0, uint4 d1, uint64_t start, uint64_t count, uint32_t bits,
+                                                  unsigned long long *__restrict__ best) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  const uint64_t nonce = start + t;
+  uint32_t m[16] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w, (uint32_t)nonce, (uint32_t)(nonce >> 32),
+                    0, 0, 0, 0, 0, 0};
+  uint32_t cv[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) cv[i] = b3::IV[i];
+  b3::compress_cv(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT);
+  const uint64_t be = ((uint64_t)__builtin_bswap32(cv[0]) << 32) | __builtin_bswap32(cv[1]);
+  const bool ok = bits == 0 || (bits <= 64 && (be >> (64 - bits)) == 0);
+  if (ok) atomicMin(best, (unsigned long long)nonce);
+}
+
+int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
+                    void *d_best) {
+  uint4 d0, d1;
+  memcpy(&d0, digest, 16);
+  memcpy(&d1, digest + 16, 16);
+  hipLaunchKernelGGL(pow_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, d0, d1, start, count, bits,
+                     (unsigned long long *)d_best);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------- gathers
+__global__ void gather_kernel(const uint4 *__restrict__ src, const uint64_t *__restrict__ idx, uint32_t n,
+                              uint32_t unit_u4, uint4 *__restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)n * unit_u4) return;
+  const uint64_t i = t / unit_u4, j = t % unit_u4;
+  out[t] = src[idx[i] * unit_u4 + j];
+}
+
+int gather_device(Ctx *ctx, hipStream_t st, const void *src, const uint64_t *d_idx, uint32_t n, uint32_t unit_bytes,
+                  void *out) {
+  const uint32_t u4 = unit_bytes / 16;
+  const uint64_t total = (uint64_t)n * u4;
+  if (!total) return FLOCK_OK;
+  hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, (const uint4 *)src, d_idx,
+                     n, u4, (uint4 *)out);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// replicate + scatter helpers
+__global__ void scatter_kernel(f128 *__restrict__ dst, const uint64_t *__restrict__ idx, const f128 *__restrict__ val,
+                               uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[idx[t]] = val[t];
+}
+
+int scatter_device(Ctx *ctx, hipStream_t st, void *dst, const uint64_t *d_idx, const void *d_val, uint32_t n) {
+  if (!n) return FLOCK_OK;
+  hipLaunchKernelGGL(scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, (f128 *)dst, d_idx, (const f128 *)d_val, n);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// out[j] = xor over replicas r of in[j + r * n]  (transpose of replicate)
+__global__ void fold_replicas_kernel(const f128 *__restrict__ in, uint64_t n, uint32_t reps, f128 *__restrict__ out) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    f128 s = f128_zero();
+    for (uint32_t r = 0; r < reps; ++r) f128_xor_into(s, in[j + r * n]);
+    out[j] = s;
+  }
+}
+
+int fold_replicas_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t n, uint32_t reps, void *out) {
+  hipLaunchKernelGGL(fold_replicas_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (const f128 *)in, n, reps,
+                     (f128 *)out);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+}  // namespace flock

@@ -78,6 +78,16 @@ def lib():
         L.flock_synchronize.argtypes = [P]
         L.flock_timing_enable.argtypes = [P, ctypes.c_int]
         L.flock_timing_report.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+        SZ, U64, U32 = ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint32
+        PSZ = ctypes.POINTER(ctypes.c_size_t)
+        L.flock_setup_create.argtypes = [P, U64, U32, U32, ctypes.POINTER(P)]
+        L.flock_setup_destroy.argtypes = [P]
+        L.flock_setup_info.argtypes = [P, ctypes.POINTER(U32), P]
+        L.flock_prove_blake3.argtypes = [P, P, U64, P, SZ, P, SZ, PSZ]
+        L.flock_prove_blake3_device.argtypes = [P, P, U64, P, SZ, P, SZ, PSZ, P]
+        L.flock_prove_blake3_seeded.argtypes = [P, U32, U64, P, SZ, P, SZ, PSZ]
+        L.flock_generate_compressions_device.argtypes = [P, U32, U64, P]
+        L.flock_last_proof.argtypes = [P, P, SZ, PSZ]
         _lib = L
     return _lib
 
@@ -86,11 +96,24 @@ def exported_symbols():
     """Names the header declares, for the ABI smoke test."""
     return ["flock_ctx_create", "flock_ctx_destroy", "flock_last_error", "flock_commit", "flock_commit_device",
             "flock_timing_enable", "flock_timing_report", "flock_device_count", "flock_device_alloc",
-            "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_synchronize"]
+            "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_synchronize", "flock_setup_create",
+            "flock_setup_destroy", "flock_setup_info", "flock_prove_blake3", "flock_prove_blake3_device",
+            "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof"]
 
 
 def device_count():
     return lib().flock_device_count()
+
+
+# The benchmark's Fiat-Shamir domain separator (the worker's FsChallenger).
+BENCH_DOMAIN = b"flock-bench-v0"
+
+# One compression: the reference `Compression` tuple (blake3.rs:1087).
+COMPRESSION_DTYPE = np.dtype([("cv", "<u4", (8,)), ("m", "<u4", (16,)), ("counter", "<u8"), ("block_len", "<u4"),
+                              ("flags", "<u4")])
+assert COMPRESSION_DTYPE.itemsize == 112
+
+PROOF_CAP = 4 << 20
 
 
 class DeviceBuffer:
@@ -191,3 +214,75 @@ class Context:
             name, cnt, ms = line.rsplit(" ", 2)
             out[name] = (int(cnt), float(ms))
         return out
+
+
+class Blake3Setup:
+    """`Blake3Setup` (crates/flock-prover/src/r1cs_hashes/blake3.rs:3185-3722)
+    for `n_blocks` compressions on one GPU context: Fast profile, rate 1/2,
+    BLAKE3 Merkle hash."""
+
+    def __init__(self, ctx, n_blocks, profile=PROFILE_FAST, merkle_hash=HASH_BLAKE3):
+        self.ctx = ctx
+        self.n_blocks = n_blocks
+        self.handle = ctypes.c_void_p()
+        ctx._check(lib().flock_setup_create(ctx.handle, n_blocks, profile, merkle_hash, ctypes.byref(self.handle)))
+        m = ctypes.c_uint32()
+        dg = (ctypes.c_uint8 * 32)()
+        ctx._check(lib().flock_setup_info(self.handle, ctypes.byref(m), dg))
+        self._m = m.value
+        self.statement_digest = bytes(dg)
+
+    def m(self):
+        return self._m
+
+    def n_blocks_log(self):
+        return self._m - 14
+
+    def close(self):
+        if self.handle:
+            lib().flock_setup_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _out(self, call):
+        buf = np.empty(PROOF_CAP, np.uint8)
+        n = ctypes.c_size_t()
+        rc = call(buf.ctypes.data, buf.nbytes, ctypes.byref(n))
+        if rc == -4:  # FLOCK_E_CAP: fetch the retained bytes
+            buf = np.empty(n.value, np.uint8)
+            rc = lib().flock_last_proof(self.ctx.handle, buf.ctypes.data, buf.nbytes, ctypes.byref(n))
+        self.ctx._check(rc)
+        return buf[:n.value].tobytes()
+
+    def prove_fast(self, blocks, domain=BENCH_DOMAIN):
+        """`prove_fast` + `R1csProofBundleLigerito::to_bytes`: proof bundle
+        bytes for host compressions (numpy array of COMPRESSION_DTYPE)."""
+        b = np.ascontiguousarray(blocks, dtype=COMPRESSION_DTYPE)
+        d = bytes(domain)
+        return self._out(lambda o, c, n: lib().flock_prove_blake3(self.handle, b.ctypes.data, b.size, d, len(d), o,
+                                                                  c, n))
+
+    def prove_fast_device(self, d_blocks_ptr, n_blocks, domain=BENCH_DOMAIN, stream=None):
+        d = bytes(domain)
+        return self._out(lambda o, c, n: lib().flock_prove_blake3_device(
+            self.handle, ctypes.c_void_p(d_blocks_ptr), n_blocks, d, len(d), o, c, n,
+            ctypes.c_void_p(stream) if stream else None))
+
+    def prove_seeded(self, log2, seed, domain=BENCH_DOMAIN):
+        """The worker's seed -> proof path (worker/src/main.rs:36-45)."""
+        d = bytes(domain)
+        return self._out(lambda o, c, n: lib().flock_prove_blake3_seeded(self.handle, log2, seed, d, len(d), o, c,
+                                                                         n))
+
+
+def generate_compressions_device(ctx, log2, seed):
+    """`generate_compressions(log2, seed)` (benchmark-tools/common/src/lib.rs:
+    20-33) into a new device buffer."""
+    buf = ctx.alloc((1 << log2) * COMPRESSION_DTYPE.itemsize)
+    ctx._check(lib().flock_generate_compressions_device(ctx.handle, log2, seed, ctypes.c_void_p(buf.ptr)))
+    return buf

@@ -83,6 +83,52 @@ int flock_memcpy_h2d(flock_ctx *ctx, void *d_dst, const void *src, size_t bytes)
 int flock_memcpy_d2h(flock_ctx *ctx, void *dst, const void *d_src, size_t bytes);
 int flock_synchronize(flock_ctx *ctx);
 
+/* ------------------------------------------------------------------ prover */
+
+/* A prover setup for n_blocks compressions: Blake3Setup::with_profile
+ * (crates/flock-prover/src/r1cs_hashes/blake3.rs:3312-3347) with the
+ * benchmark parameters (rate 1/2, 64 lanes, BLAKE3 Merkle hash).  Builds the
+ * R1CS matrices and statement digest.  Device buffers are reserved in the
+ * context by the first prove and reused by later ones of the same size.
+ * profile: only 0 (Fast); merkle_hash: only 1 (Blake3).  m = 14 +
+ * ceil(log2(max(n_blocks, 8))) must have a Ligerito configuration (22..35). */
+typedef struct flock_setup flock_setup;
+int flock_setup_create(flock_ctx *ctx, uint64_t n_blocks, uint32_t profile, uint32_t merkle_hash,
+                       flock_setup **out);
+void flock_setup_destroy(flock_setup *setup);
+/* m of the setup and its R1CS statement digest (BlockR1cs::statement_digest,
+ * crates/flock-core/src/r1cs.rs:343-365). */
+int flock_setup_info(const flock_setup *setup, uint32_t *m, uint8_t digest[32]);
+
+/* Blake3Setup::prove_fast (blake3.rs:3405) followed by the worker's
+ * R1csProofBundleLigerito::to_bytes (crates/flock-core/src/proof_io.rs:161-208,
+ * worker/src/main.rs:42-45): the proof bundle bytes ("FLOCK" 04 02 + bincode)
+ * for `n_blocks` compressions under a BLAKE3 Fiat-Shamir transcript with
+ * domain separator `domain` (FsChallenger::new, crates/flock-core/src/
+ * challenger.rs:246).  n_blocks must equal the setup's (the reference
+ * asserts, blake3.rs:3520).  If cap is too small, returns FLOCK_E_CAP with
+ * *out_len set; the bytes stay available through flock_last_proof. */
+int flock_prove_blake3(flock_setup *setup, const flock_compression *blocks, uint64_t n_blocks,
+                       const uint8_t *domain, size_t domain_len, uint8_t *out, size_t cap, size_t *out_len);
+
+/* Same with the compressions already in device memory (d_blocks, n_blocks x
+ * 112 B) and the work issued on `stream` (NULL: the context stream). */
+int flock_prove_blake3_device(flock_setup *setup, const flock_compression *d_blocks, uint64_t n_blocks,
+                              const uint8_t *domain, size_t domain_len, uint8_t *out, size_t cap, size_t *out_len,
+                              void *stream);
+
+/* The worker path from a seed (worker/src/main.rs:36-45): expand
+ * generate_compressions(log2, seed) (benchmark-tools/common/src/lib.rs:20-33)
+ * on the device, then prove.  The setup must be for 2^log2 blocks. */
+int flock_prove_blake3_seeded(flock_setup *setup, uint32_t log2, uint64_t seed, const uint8_t *domain,
+                              size_t domain_len, uint8_t *out, size_t cap, size_t *out_len);
+
+/* generate_compressions(log2, seed) into device memory d_out (2^log2 x 112 B). */
+int flock_generate_compressions_device(flock_ctx *ctx, uint32_t log2, uint64_t seed, flock_compression *d_out);
+
+/* Copy the bytes of the context's last proof (kept after FLOCK_E_CAP). */
+int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len);
+
 /* Device phase timing (HIP events on the launching stream), the counterpart
  * of the reference's FLOCK_PHASE_TIMING / prove_fast_timed
  * (crates/flock-prover/src/prover.rs:1267-1284).  Enabling resets the totals.

@@ -1,0 +1,122 @@
+"""Parity of the GPU prover (libflockgpu.so, through the C ABI) with the CPU
+oracle: identical proof bundle bytes for identical compressions, accepted by
+the oracle's verifier.  The m=32 case (2^18 compressions, the benchmark size)
+is checked against the committed canary fixture: its length equals the
+reference's own canary assertion (crates/flock-prover/src/r1cs_hashes/
+blake3.rs:4910-4939) and its BLAKE3 digest equals the oracle's proof.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gpu_util
+import flockgpu
+import oracle_lib as O
+import proof_format as PF
+
+pytestmark = pytest.mark.gpu
+
+WORKER_SEED = 0x00C0FFEEBEEFD15C  # worker/src/main.rs:28
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not gpu_util.gpu_available():
+        pytest.skip("no GPU")
+    c = flockgpu.Context(0)
+    yield c
+    c.close()
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _gpu_prove(ctx, blocks):
+    s = flockgpu.Blake3Setup(ctx, blocks.size)
+    try:
+        return s.prove_fast(blocks)
+    finally:
+        s.close()
+
+
+def _check_same(gpu, ref):
+    diff = PF.first_difference(gpu, ref)
+    assert diff is None, diff
+
+
+@pytest.mark.parametrize("log2,seed", [(8, WORKER_SEED), (9, 7), (10, 1)])
+def test_proof_bytes_match_oracle(ctx, log2, seed):
+    _say(f"prove log2={log2} seed={seed:#x}")
+    blocks = O.generate_compressions(log2, seed)
+    gpu = _gpu_prove(ctx, blocks)
+    ref = O.prove(blocks)
+    _check_same(gpu, ref)
+    assert O.verify(gpu, blocks.size, blocks) == 0
+
+
+@pytest.mark.parametrize("n", [200, 3000])
+def test_non_power_of_two_batches_pad_like_the_oracle(ctx, n):
+    _say(f"prove n={n}")
+    log2 = int(np.ceil(np.log2(n)))
+    blocks = O.generate_compressions(log2, 0x9A110F11)[:n].copy()
+    gpu = _gpu_prove(ctx, blocks)
+    ref = O.prove(blocks)
+    _check_same(gpu, ref)
+    assert O.verify(gpu, n, blocks) == 0
+
+
+def test_config0_worker_seed_matches_oracle(ctx):
+    """2^12 compressions (m = 26), the worker warm-up seed."""
+    _say("prove log2=12 worker seed")
+    blocks = O.generate_compressions(12, WORKER_SEED)
+    gpu = _gpu_prove(ctx, blocks)
+    ref = O.prove(blocks)
+    _check_same(gpu, ref)
+
+
+def test_seeded_and_device_paths_match_host_blocks(ctx):
+    _say("seeded/device paths log2=11")
+    log2, seed = 11, 0x1234
+    blocks = O.generate_compressions(log2, seed)
+    s = flockgpu.Blake3Setup(ctx, 1 << log2)
+    try:
+        host = s.prove_fast(blocks)
+        seeded = s.prove_seeded(log2, seed)
+        d = flockgpu.generate_compressions_device(ctx, log2, seed)
+        assert np.array_equal(d.download(np.uint8), blocks.view(np.uint8).reshape(-1))
+        dev = s.prove_fast_device(d.ptr, 1 << log2)
+        again = s.prove_fast(blocks)
+    finally:
+        s.close()
+    assert seeded == host and dev == host and again == host
+
+
+def test_canary_m32_matches_fixture(ctx):
+    """2^18 compressions from the reference canary seed: 437,551 bytes, and
+    the same bytes as the oracle (BLAKE3 digest of its proof)."""
+    g = json.load(open(os.path.join(GOLDEN, "canary_m32.json")))
+    _say("prove canary m=32")
+    s = flockgpu.Blake3Setup(ctx, 1 << g["log2"])
+    try:
+        proof = s.prove_seeded(g["log2"], g["raw_seed"] ^ (g["log2"] << 29))
+    finally:
+        s.close()
+    assert len(proof) == g["reference_expected_len"]
+    assert O.blake3(proof).hex() == g["proof_blake3"]
+
+
+def test_shape_errors_are_reported(ctx):
+    with pytest.raises(flockgpu.FlockError) as e:
+        flockgpu.Blake3Setup(ctx, 4)  # m = 17: no Ligerito configuration
+    assert e.value.code == -1
+    s = flockgpu.Blake3Setup(ctx, 256)
+    try:
+        with pytest.raises(flockgpu.FlockError) as e:
+            s.prove_fast(O.generate_compressions(7, 1))
+        assert e.value.code == -1
+    finally:
+        s.close()

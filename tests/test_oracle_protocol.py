@@ -270,3 +270,20 @@ def test_ranked_canary_length_matches_reference():
     assert len(pf) == 437551 == gold["reference_expected_len"]
     assert O.blake3(pf).hex() == gold["proof_blake3"]
     assert O.verify(pf, blocks.size, None) == 0
+
+
+def test_proof_format_parses_every_byte():
+    """The field map covers an oracle proof exactly and names the first
+    differing field of a tampered copy."""
+    import proof_format as PF
+    blocks = O.generate_compressions(8, 0x00C0FFEEBEEFD15C)
+    p = O.prove(blocks)
+    fs = PF.fields(p)
+    assert fs[0] == ("magic", 0, 5) and sum(n for _, _, n in fs) == len(p)
+    names = [n for n, _, _ in fs]
+    off = dict((n, o) for n, o, _ in fs)["lincheck.z_partial"]
+    q = bytearray(p)
+    q[off + 3] ^= 1
+    assert "lincheck.z_partial" in PF.first_difference(p, bytes(q))
+    assert PF.first_difference(p, p) is None
+    assert "ligerito.fold_nonces" in names
