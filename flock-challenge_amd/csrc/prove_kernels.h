@@ -16,8 +16,16 @@ int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, 
 namespace flock {
 size_t zc_tables_bytes();
 int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out);
-int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *c, int m,
-                     const void *d_tables, const void *d_ehi, void *d_out128);
+// P^AB of zerocheck round 1 (64 F128) for m-variable a, b bit-vectors.
+int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int m, const void *d_tables,
+                     const void *d_ehi, void *d_out64);
+// Q_k[il][r] = sum_ih bit_r(elem[il + 2^il_log ih]) hi_k[ih] over 128-bit
+// elements (2^il_log x n_ih of them); hi1 may be null (one claim).  out:
+// [2^(il_log-4)][n_claims][16][128] F128.
+int bitfold_device(Ctx *ctx, hipStream_t st, const void *elems, int il_log, uint64_t n_ih, const void *hi0,
+                   const void *hi1, void *out);
+// out[y][w] = xor_x in[y][x][w] (ny stacks of nx rows of width F128).
+int xor_reduce_device(Ctx *ctx, hipStream_t st, const void *in, uint32_t ny, uint32_t nx, uint32_t width, void *out);
 int fold_at_z_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_w64, uint64_t rows, void *A,
                      void *B);
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out);
@@ -25,10 +33,8 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3);
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);
-int lc_zfold_device(Ctx *ctx, hipStream_t st, const void *z, const void *eq_outer, uint64_t n_blocks, void *out);
 int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
                    const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb);
-int rs_fold1b_device(Ctx *ctx, hipStream_t st, const void *z, const void *e0, const void *e1, uint64_t n, void *out256);
 int rs_basis_device(Ctx *ctx, hipStream_t st, const void *e0, const void *e1, const void *tables, uint64_t n, void *out);
 int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed);
 int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,

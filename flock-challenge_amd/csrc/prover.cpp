@@ -85,15 +85,18 @@ F128 medium(int i) {
 }
 const uint8_t SMALL_CHAL[3] = {0xF7, 0x53, 0xB5};
 
+// S -> Lambda extension matrix: M[j][s] = value at 64 + j of the degree-<64
+// polynomial through (s, [s == t]) over GF(2^8) (the inverse-S / forward-
+// Lambda LCH NTT pair of zerocheck/univariate_skip.rs:106-120 computes the
+// same map).  Filled with the round-1 tables.
+uint8_t EXT_M[64][64];
+
 const ZcTables &zc_tables() {
   static ZcTables T;
   static std::once_flag once;
   std::call_once(once, [] {
     memset(&T, 0, sizeof T);
-    // S -> Lambda extension: value at 64 + j of the degree-<64 polynomial
-    // through (s, bit_s), s = 0..63 (the inverse-S / forward-Lambda LCH NTT
-    // pair of zerocheck/univariate_skip.rs:106-120 computes the same map).
-    static uint8_t M[64][64];
+    auto &M = EXT_M;
     for (int j = 0; j < 64; ++j)
       for (int s = 0; s < 64; ++s) {
         uint8_t num = 1, den = 1;
@@ -438,18 +441,40 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   }
   F128 r1ab[64], r1c[64];
   {
+    ScopedPhase ph(ctx, "zc_round1_all", st);
     void *dt = P.dev("zc.tables", sizeof(ZcTables));
     void *dehi = P.dev("zc.ehi", ((size_t)1 << (m - 13)) * 16);
-    void *dout = P.dev("zc.r1out", 128 * 16);
-    if (!dt || !dehi || !dout) return FLOCK_E_OOM;
+    void *dout = P.dev("zc.r1out", 64 * 16);
+    void *dhic = P.dev("zc.hic", ((size_t)1 << (m - 11)) * 16), *dq = P.dev("zc.cq", 2048 * 16);
+    if (!dt || !dehi || !dout || !dhic || !dq) return FLOCK_E_OOM;
     if ((rc = P.h2d(dt, &zc_tables(), sizeof(ZcTables)))) return rc;
     if ((rc = P.eq_device(&r[13], m - 13, dehi))) return rc;
-    if ((rc = zc_round1_device(ctx, st, a, b, z, m, dt, dehi, dout))) return rc;
-    uint4 h[128];
+    if ((rc = zc_round1_device(ctx, st, a, b, m, dt, dehi, dout))) return rc;
+    // P^C(lambda_j) = sum_s phi8(M[j][s]) C_s with C_s = sum_x eq(x) bit_s(c[x]);
+    // elements are row pairs (x = 2e + h), e = il + 16 ih.
+    if ((rc = P.eq_device(&r[11], m - 11, dhic))) return rc;
+    if ((rc = bitfold_device(ctx, st, z, 4, 1ull << (m - 11), dhic, nullptr, dq))) return rc;
+    uint4 h[64];
     if ((rc = P.d2h(h, dout, sizeof h))) return rc;
+    for (int j = 0; j < 64; ++j) r1ab[j] = fv(h[j]);
+    std::vector<F128> q(2048);
+    if ((rc = P.d2h(q.data(), dq, 2048 * 16))) return rc;
+    std::vector<F128> lo = eq_table(&r[7], 4);
+    F128 cs[64];
+    const F128 w0 = ONE + r[6], w1 = r[6];
+    for (int sb = 0; sb < 64; ++sb) {
+      F128 s0, s1;
+      for (int il = 0; il < 16; ++il) {
+        s0 += lo[il] * q[il * 128 + sb];
+        s1 += lo[il] * q[il * 128 + 64 + sb];
+      }
+      cs[sb] = w0 * s0 + w1 * s1;
+    }
     for (int j = 0; j < 64; ++j) {
-      r1ab[j] = fv(h[j]);
-      r1c[j] = fv(h[64 + j]);
+      F128 acc;
+      for (int sb = 0; sb < 64; ++sb)
+        if (EXT_M[j][sb]) acc += phi8(EXT_M[j][sb]) * cs[sb];
+      r1c[j] = acc;
     }
   }
   t.observe_slice(r1ab, 64);
@@ -536,7 +561,8 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     if ((rc = P.h2d(deq, eqi.data(), eqi.size() * 16))) return rc;
     if ((rc = lc_comb_device(ctx, st, dap, dar, dbp, dbr, deq, alpha, beta, r1cs::Z_CONST, dcomb))) return rc;
     if ((rc = P.eq_device(&rhos[8], m - 14, deqo))) return rc;
-    if ((rc = lc_zfold_device(ctx, st, z, deqo, 1ull << (m - 14), dzv))) return rc;
+    // z_vec[128 c + r] = sum_blk eq_outer[blk] bit_r(16-byte chunk c of block blk)
+    if ((rc = bitfold_device(ctx, st, z, 7, 1ull << (m - 14), deqo, nullptr, dzv))) return rc;
     std::vector<F128> comb(1 << 14), zv(1 << 14);
     if ((rc = P.d2h(comb.data(), dcomb, comb.size() * 16)) || (rc = P.d2h(zv.data(), dzv, zv.size() * 16))) return rc;
     size_t len = 1 << 14;
@@ -585,17 +611,26 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   t.label("flock-pcs-open-batch-v0");
   const size_t N = (size_t)1 << L;
   void *e0 = P.dev("rs.e0", N * 16), *e1 = P.dev("rs.e1", N * 16), *cb = P.dev("lig.cb", N * 16);
-  void *d256 = P.dev("rs.s", 256 * 16), *dtab = P.dev("rs.tab", 2 * 32 * 16 * 16);
-  if (!e0 || !e1 || !cb || !d256 || !dtab) return FLOCK_E_OOM;
+  void *dtab = P.dev("rs.tab", 2 * 32 * 16 * 16);
+  if (!e0 || !e1 || !cb || !dtab) return FLOCK_E_OOM;
   F128 shv[2][128], sclaim[2], eqr[2][128];
-  if ((rc = P.eq_device(&xf[0][1], L, e0)) || (rc = P.eq_device(&xf[1][1], L, e1))) return rc;
-  if ((rc = rs_fold1b_device(ctx, st, z, e0, e1, N, d256))) return rc;
   {
-    uint4 h[256];
-    if ((rc = P.d2h(h, d256, sizeof h))) return rc;
-    for (int i = 0; i < 128; ++i) {
-      shv[0][i] = fv(h[i]);
-      shv[1][i] = fv(h[128 + i]);
+    // s_hat_v[k][r] = sum_i bit_r(z[i]) eq_k(i), eq_k(i) = lo_k(i mod 16) hi_k(i / 16)
+    void *h0 = P.dev("rs.hi0", (N / 16) * 16), *h1 = P.dev("rs.hi1", (N / 16) * 16);
+    void *dq = P.dev("rs.q", 2 * 2048 * 16);
+    if (!h0 || !h1 || !dq) return FLOCK_E_OOM;
+    if ((rc = P.eq_device(&xf[0][5], L - 4, h0)) || (rc = P.eq_device(&xf[1][5], L - 4, h1))) return rc;
+    if ((rc = bitfold_device(ctx, st, z, 4, N / 16, h0, h1, dq))) return rc;
+    if ((rc = P.eq_device(&xf[0][1], L, e0)) || (rc = P.eq_device(&xf[1][1], L, e1))) return rc;
+    std::vector<F128> q(2 * 2048);
+    if ((rc = P.d2h(q.data(), dq, q.size() * 16))) return rc;
+    for (int k = 0; k < 2; ++k) {
+      std::vector<F128> lo = eq_table(&xf[k][1], 4);
+      for (int rb = 0; rb < 128; ++rb) {
+        F128 acc;
+        for (int il = 0; il < 16; ++il) acc += lo[il] * q[k * 2048 + il * 128 + rb];
+        shv[k][rb] = acc;
+      }
     }
   }
   for (int k = 0; k < 2; ++k) {

@@ -177,44 +177,6 @@ int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, 
 }
 
 // ---------------------------------------------------------------- lincheck
-// z_vec[i] = sum_blk eq_outer[blk] * bit_i(z block blk), i < 2^14
-// (partial_fold_packed_z, lincheck.rs:660-845).  Workgroup = 4 words of the
-// 256-word block x a range of blocks; results XORed into out with atomics.
-__global__ __launch_bounds__(256) void lc_zfold_kernel(const uint64_t *__restrict__ z, const f128 *__restrict__ eq,
-                                                       uint64_t n_blocks, uint64_t blocks_per_wg,
-                                                       uint32_t *__restrict__ out) {
-  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const uint64_t b0 = (uint64_t)blockIdx.y * blocks_per_wg;
-  uint64_t b1 = b0 + blocks_per_wg;
-  if (b1 > n_blocks) b1 = n_blocks;
-  f128 acc = f128_zero();
-  for (uint64_t blk = b0; blk < b1; ++blk) {
-    const uint64_t word = z[blk * 256 + w];
-    const uint32_t mask = 0u - (uint32_t)((word >> lane) & 1);
-    const f128 e = eq[blk];
-    acc.x ^= e.x & mask;
-    acc.y ^= e.y & mask;
-    acc.z ^= e.z & mask;
-    acc.w ^= e.w & mask;
-  }
-  uint32_t *o = out + 4 * (64 * w + lane);
-  atomicXor(o + 0, acc.x);
-  atomicXor(o + 1, acc.y);
-  atomicXor(o + 2, acc.z);
-  atomicXor(o + 3, acc.w);
-}
-
-int lc_zfold_device(Ctx *ctx, hipStream_t st, const void *z, const void *eq_outer, uint64_t n_blocks, void *out) {
-  ScopedPhase ph(ctx, "lc_zfold", st);
-  FLOCK_HIP_CHECK(ctx, hipMemsetAsync(out, 0, 16 << 14, st));
-  uint64_t ranges = n_blocks < 64 ? 1 : 64;
-  uint64_t per = (n_blocks + ranges - 1) / ranges;
-  hipLaunchKernelGGL(lc_zfold_kernel, dim3(64, (unsigned)ranges), dim3(256), 0, st, (const uint64_t *)z,
-                     (const f128 *)eq_outer, n_blocks, per, (uint32_t *)out);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return FLOCK_OK;
-}
-
 // comb[c] = alpha (A_0^T eq)[c] + (B_0^T eq)[c] (+ beta at the pinned column)
 // (fold_alpha_batched, lincheck.rs:1158-1240; const pin, :1530-1545).
 __global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict__ ap, const uint32_t *__restrict__ ar,
@@ -244,42 +206,6 @@ int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t 
 }
 
 // ---------------------------------------------------------------- ring switch
-// s_hat_v[k][r] = sum_i bit_r(z[i]) eq_k[i] for two claims (fold_1b_rows,
-// ring_switch.rs:1568-1618).  Thread = bit r; workgroup = element range.
-__global__ __launch_bounds__(128) void rs_fold1b_kernel(const f128 *__restrict__ z, const f128 *__restrict__ e0,
-                                                        const f128 *__restrict__ e1, uint64_t n, uint64_t per_wg,
-                                                        f128 *__restrict__ partial) {
-  const uint32_t r = threadIdx.x;
-  const uint64_t i0 = (uint64_t)blockIdx.x * per_wg;
-  uint64_t i1 = i0 + per_wg;
-  if (i1 > n) i1 = n;
-  f128 s0 = f128_zero(), s1 = f128_zero();
-  const uint32_t word = r >> 5, sh = r & 31;
-  for (uint64_t i = i0; i < i1; ++i) {
-    const f128 zi = z[i];
-    const uint32_t w = word == 0 ? zi.x : word == 1 ? zi.y : word == 2 ? zi.z : zi.w;
-    const uint32_t mask = 0u - ((w >> sh) & 1u);
-    const f128 a = e0[i], b = e1[i];
-    s0.x ^= a.x & mask; s0.y ^= a.y & mask; s0.z ^= a.z & mask; s0.w ^= a.w & mask;
-    s1.x ^= b.x & mask; s1.y ^= b.y & mask; s1.z ^= b.z & mask; s1.w ^= b.w & mask;
-  }
-  partial[(uint64_t)blockIdx.x * 256 + r] = s0;
-  partial[(uint64_t)blockIdx.x * 256 + 128 + r] = s1;
-}
-
-int rs_fold1b_device(Ctx *ctx, hipStream_t st, const void *z, const void *e0, const void *e1, uint64_t n, void *out256) {
-  ScopedPhase ph(ctx, "rs_fold1b", st);
-  uint64_t wgs = n < 4096 ? 1 : 4096;
-  uint64_t per = (n + wgs - 1) / wgs;
-  wgs = (n + per - 1) / per;
-  f128 *part = (f128 *)ctx->get_scratch("rs.fold1b.partial", wgs * 256 * sizeof(f128));
-  if (!part) return FLOCK_E_OOM;
-  hipLaunchKernelGGL(rs_fold1b_kernel, dim3((unsigned)wgs), dim3(128), 0, st, (const f128 *)z, (const f128 *)e0,
-                     (const f128 *)e1, n, per, part);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, wgs, 256, out256);
-}
-
 // b[i] = sum_k sum_bits bit_b(eq_k[i]) w_k[b] with w_k = gamma_k eq(r''_k)
 // (fold_b128_elems, ring_switch.rs:1720-1760, combined as in pcs.rs:1025-1100)
 // through nibble tables T[k][p][v] (2 x 32 x 16 F128, host-built).
