@@ -112,11 +112,31 @@ __device__ __forceinline__ f128 f128_shl4(f128 a) {
   return r;
 }
 
-// General product a * b with both operands varying per lane.  Each lane owns
-// a 16-entry 4-bit window table of a in LDS (entry e at tab[e * 64 + lane],
-// lane = 0..63 within the wavefront, so a wave's reads of one entry index
-// are contiguous), then runs Horner over the 32 nibbles of b.  No cross-lane
-// traffic: a lane only reads back what it wrote, so no barrier is needed.
+// Reduce a 256-bit carry-less product (d0 = bits 0..31 ... d7) mod P.
+__device__ __forceinline__ f128 f128_reduce256(uint32_t l0, uint32_t l1, uint32_t l2, uint32_t l3, uint32_t h0,
+                                               uint32_t h1, uint32_t h2, uint32_t h3) {
+  // H x^128 = H (x^7 + x^2 + x + 1); bits pushed past 127 fold once more.
+  f128 r;
+  r.x = l0 ^ h0 ^ (h0 << 1) ^ (h0 << 2) ^ (h0 << 7);
+  r.y = l1 ^ h1 ^ __builtin_amdgcn_alignbit(h1, h0, 31) ^ __builtin_amdgcn_alignbit(h1, h0, 30) ^
+        __builtin_amdgcn_alignbit(h1, h0, 25);
+  r.z = l2 ^ h2 ^ __builtin_amdgcn_alignbit(h2, h1, 31) ^ __builtin_amdgcn_alignbit(h2, h1, 30) ^
+        __builtin_amdgcn_alignbit(h2, h1, 25);
+  r.w = l3 ^ h3 ^ __builtin_amdgcn_alignbit(h3, h2, 31) ^ __builtin_amdgcn_alignbit(h3, h2, 30) ^
+        __builtin_amdgcn_alignbit(h3, h2, 25);
+  const uint32_t o = (h3 >> 31) ^ (h3 >> 30) ^ (h3 >> 25);
+  r.x ^= fold87(o);
+  return r;
+}
+
+// General product a * b with both operands varying per lane (comb method).
+// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS (entry e at
+// tab[e * 64 + lane], so a wave's reads of one entry index are contiguous);
+// then for nibble position j = 7..0 of the four 32-bit words b_i of b, the
+// entries T[nibble_j(b_i)] are XORed into a 256-bit accumulator at dword
+// offset i, and the accumulator shifts by 4 between positions: 32 lookups,
+// 7 wide shifts and one final reduction.  No cross-lane traffic: a lane only
+// reads back what it wrote, so no barrier is needed.
 __device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab, uint32_t lane) {
   f128 t1 = a, t2 = f128_mulx(a), t4 = f128_mulx(t2), t8 = f128_mulx(t4);
   f128 t3 = f128_xor(t1, t2), t5 = f128_xor(t4, t1), t6 = f128_xor(t4, t2), t7 = f128_xor(t4, t3);
@@ -137,20 +157,32 @@ __device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab,
   T[832] = f128_xor(t8, t5);
   T[896] = f128_xor(t8, t6);
   T[960] = f128_xor(t8, t7);
-  f128 acc = f128_zero();
-  uint32_t w0 = b.w, w1 = b.z, w2 = b.y, w3 = b.x;
-#pragma unroll 1
-  for (int q = 0; q < 4; ++q) {
+  uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0, d6 = 0, d7 = 0;
 #pragma unroll
-    for (int k = 7; k >= 0; --k) {
-      acc = f128_shl4(acc);
-      f128_xor_into(acc, T[((w0 >> (4 * k)) & 15u) << 6]);
+  for (int j = 7; j >= 0; --j) {
+    const f128 e0 = T[((b.x >> (4 * j)) & 15u) << 6];
+    const f128 e1 = T[((b.y >> (4 * j)) & 15u) << 6];
+    const f128 e2 = T[((b.z >> (4 * j)) & 15u) << 6];
+    const f128 e3 = T[((b.w >> (4 * j)) & 15u) << 6];
+    d0 ^= e0.x;
+    d1 ^= e0.y ^ e1.x;
+    d2 ^= e0.z ^ e1.y ^ e2.x;
+    d3 ^= e0.w ^ e1.z ^ e2.y ^ e3.x;
+    d4 ^= e1.w ^ e2.z ^ e3.y;
+    d5 ^= e2.w ^ e3.z;
+    d6 ^= e3.w;
+    if (j) {
+      d7 = __builtin_amdgcn_alignbit(d7, d6, 28);
+      d6 = __builtin_amdgcn_alignbit(d6, d5, 28);
+      d5 = __builtin_amdgcn_alignbit(d5, d4, 28);
+      d4 = __builtin_amdgcn_alignbit(d4, d3, 28);
+      d3 = __builtin_amdgcn_alignbit(d3, d2, 28);
+      d2 = __builtin_amdgcn_alignbit(d2, d1, 28);
+      d1 = __builtin_amdgcn_alignbit(d1, d0, 28);
+      d0 <<= 4;
     }
-    w0 = w1;
-    w1 = w2;
-    w2 = w3;
   }
-  return acc;
+  return f128_reduce256(d0, d1, d2, d3, d4, d5, d6, d7);
 }
 
 // LDS bytes a wavefront needs for f128_mul scratch.

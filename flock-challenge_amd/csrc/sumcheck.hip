@@ -178,7 +178,20 @@ int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, 
 
 // ---------------------------------------------------------------- lincheck
 // comb[c] = alpha (A_0^T eq)[c] + (B_0^T eq)[c] (+ beta at the pinned column)
-// (fold_alpha_batched, lincheck.rs:1158-1240; const pin, :1530-1545).
+// (fold_alpha_batched, lincheck.rs:1158-1240; const pin, :1530-1545).  One
+// wavefront per column: lanes stride the column's nonzeros, then a butterfly
+// XOR across the wavefront.
+__device__ __forceinline__ f128 wave_xor(f128 v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    v.x ^= __shfl_xor(v.x, d);
+    v.y ^= __shfl_xor(v.y, d);
+    v.z ^= __shfl_xor(v.z, d);
+    v.w ^= __shfl_xor(v.w, d);
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict__ ap, const uint32_t *__restrict__ ar,
                                                       const uint32_t *__restrict__ bp, const uint32_t *__restrict__ br,
                                                       const f128 *__restrict__ eq, f128 alpha, f128 beta, uint32_t pin,
@@ -186,20 +199,25 @@ __global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict
   __shared__ f128 tab[512];
   if (threadIdx.x < 64) f128_build_bytetab(alpha, tab, threadIdx.x);
   __syncthreads();
-  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
   f128 ta = f128_zero(), tb = f128_zero();
-  for (uint32_t k = ap[c]; k < ap[c + 1]; ++k) f128_xor_into(ta, eq[ar[k]]);
-  for (uint32_t k = bp[c]; k < bp[c + 1]; ++k) f128_xor_into(tb, eq[br[k]]);
-  f128 v = f128_xor(f128_mul_bytetab(ta, tab), tb);
-  if (c == pin) f128_xor_into(v, beta);
-  comb[c] = v;
+  for (uint32_t k = ap[c] + lane; k < ap[c + 1]; k += 64) f128_xor_into(ta, eq[ar[k]]);
+  for (uint32_t k = bp[c] + lane; k < bp[c + 1]; k += 64) f128_xor_into(tb, eq[br[k]]);
+  ta = wave_xor(ta);
+  tb = wave_xor(tb);
+  if (lane == 0) {
+    f128 v = f128_xor(f128_mul_bytetab(ta, tab), tb);
+    if (c == pin) f128_xor_into(v, beta);
+    comb[c] = v;
+  }
 }
 
 int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
                    const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb) {
   ScopedPhase ph(ctx, "lc_comb", st);
   auto cv = [](F128 v) { return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32)); };
-  hipLaunchKernelGGL(lc_comb_kernel, dim3((1 << 14) / 256), dim3(256), 0, st, ap, ar, bp, br, (const f128 *)eq,
+  hipLaunchKernelGGL(lc_comb_kernel, dim3((1 << 14) / 4), dim3(256), 0, st, ap, ar, bp, br, (const f128 *)eq,
                      cv(alpha), cv(beta), pin, (f128 *)comb);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;

@@ -19,12 +19,44 @@ namespace flock {
 
 namespace {
 
-// Three word-aligned bit streams advanced in lockstep.
+// Three word-aligned bit streams advanced in lockstep.  Finished words go to
+// a 16-word LDS ring per stream (one column per lane); every 16 words the
+// wavefront writes the ring out so that 16 lanes cover one compression's
+// 128 contiguous bytes (four compressions per store instruction).
+constexpr int RING = 16;
+constexpr int RING_STRIDE = 65;  // words per ring row: 64 lanes + 1 against bank conflicts
+
 struct Writer {
-  uint64_t *z, *a, *b;
+  uint64_t *z, *a, *b;       // global streams, compression 0 of this wavefront
+  uint64_t *sz, *sa, *sb;    // LDS rings
+  uint32_t lane;
+  uint64_t n_valid;          // compressions of this wavefront that exist
   uint64_t pz = 0, pa = 0, pb = 0;
   uint32_t used = 0, word = 0;
 
+  __device__ __forceinline__ void flush(uint32_t first_word) {
+    __syncthreads();
+    const uint32_t w = lane & 15u;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const uint32_t c = 4u * q + (lane >> 4);
+      if (c < n_valid) {
+        const uint64_t g = (uint64_t)c * 256 + first_word + w;
+        z[g] = sz[w * RING_STRIDE + c];
+        a[g] = sa[w * RING_STRIDE + c];
+        b[g] = sb[w * RING_STRIDE + c];
+      }
+    }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void emit(uint64_t vz, uint64_t va, uint64_t vb) {
+    const uint32_t slot = word % RING;
+    sz[slot * RING_STRIDE + lane] = vz;
+    sa[slot * RING_STRIDE + lane] = va;
+    sb[slot * RING_STRIDE + lane] = vb;
+    if (slot == RING - 1) flush(word - (RING - 1));
+    ++word;
+  }
   __device__ __forceinline__ void push(uint32_t vz, uint32_t va, uint32_t vb, uint32_t w) {
     const uint64_t mask = (w == 32) ? 0xFFFFFFFFull : ((1ull << w) - 1ull);
     uint64_t xz = (uint64_t)vz & mask, xa = (uint64_t)va & mask, xb = (uint64_t)vb & mask;
@@ -33,10 +65,7 @@ struct Writer {
     pb |= xb << used;
     used += w;
     if (used >= 64) {
-      z[word] = pz;
-      a[word] = pa;
-      b[word] = pb;
-      ++word;
+      emit(pz, pa, pb);
       used -= 64;
       const uint32_t sh = w - used;  // bits of this value already written
       pz = used ? xz >> sh : 0;
@@ -74,8 +103,10 @@ constexpr uint8_t G_LANES_D[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13}, {2, 6, 10, 14
 __global__ __launch_bounds__(64) void witness_kernel(const flock_compression *__restrict__ blocks, uint64_t n_real,
                                                      uint64_t n_total, uint64_t *__restrict__ z,
                                                      uint64_t *__restrict__ a, uint64_t *__restrict__ b) {
-  const uint64_t blk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (blk >= n_total) return;
+  __shared__ uint64_t ring[3][RING * RING_STRIDE];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t blk0 = (uint64_t)blockIdx.x * 64;
+  const uint64_t blk = blk0 + lane;
   uint32_t cv[8], m[16], tlo = 0, thi = 0, blen = 0, flags = 0;
   if (blk < n_real) {
     const flock_compression &c = blocks[blk];
@@ -94,15 +125,25 @@ __global__ __launch_bounds__(64) void witness_kernel(const flock_compression *__
 #pragma unroll
     for (int i = 0; i < 16; ++i) m[i] = 0;
   }
-  const uint64_t base = blk * 256;
+  // out_lo (slot words 4..7) first, so every word is produced in order.
+  uint32_t out[16];
+  b3::compress_full(cv, m, ((uint64_t)thi << 32) | tlo, blen, flags, out);
   Writer w;
-  w.z = z + base;
-  w.a = a + base;
-  w.b = b + base;
+  w.z = z + blk0 * 256;
+  w.a = a + blk0 * 256;
+  w.b = b + blk0 * 256;
+  w.sz = ring[0];
+  w.sa = ring[1];
+  w.sb = ring[2];
+  w.lane = lane;
+  w.n_valid = n_total - blk0 < 64 ? n_total - blk0 : 64;
 #pragma unroll
   for (int i = 0; i < 8; ++i) w.lin(cv[i]);
-  // words 4..7 (out_lo) are written after the rounds
-  w.word = 8;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint64_t lo = (uint64_t)out[2 * q] | ((uint64_t)out[2 * q + 1] << 32);
+    w.emit(lo, lo, ~0ull);
+  }
   w.push(1, 1, 1, 1);
 #pragma unroll
   for (int i = 0; i < 16; ++i) w.lin(m[i]);
@@ -139,26 +180,14 @@ __global__ __launch_bounds__(64) void witness_kernel(const flock_compression *__
   }
 #pragma unroll
   for (int i = 0; i < 8; ++i) w.lin(s[i + 8] ^ cv[i]);  // out_hi
-  // flush the partial word and zero the padding words
+  // the partial word, then zero padding up to 256 words
   if (w.used) {
-    w.z[w.word] = w.pz;
-    w.a[w.word] = w.pa;
-    w.b[w.word] = w.pb;
-    ++w.word;
+    w.emit(w.pz, w.pa, w.pb);
+    w.used = 0;
   }
-  for (uint32_t k = w.word; k < 256; ++k) {
-    w.z[k] = 0;
-    w.a[k] = 0;
-    w.b[k] = 0;
-  }
-  // out_lo = s[i] ^ s[i + 8] at bits 256..511 (words 4..7)
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint64_t lo = (uint64_t)(s[2 * q] ^ s[2 * q + 8]) | ((uint64_t)(s[2 * q + 1] ^ s[2 * q + 9]) << 32);
-    w.z[4 + q] = lo;
-    w.a[4 + q] = lo;
-    w.b[4 + q] = ~0ull;
-  }
+  for (int k = 0; k < 256; ++k)
+    if (k >= (int)w.word) w.emit(0, 0, 0);
 }
 
 int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_real, int n_log, void *d_z,
