@@ -21,6 +21,15 @@ __device__ __forceinline__ void f128_xor_into(f128 &a, f128 b) {
   a.x ^= b.x; a.y ^= b.y; a.z ^= b.z; a.w ^= b.w;
 }
 
+// a ^ b ^ c in one v_bitop3_b32 per dword (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+__device__ __forceinline__ f128 f128_xor3(f128 a, f128 b, f128 c) {
+  return make_uint4(xor3_u32(a.x, b.x, c.x), xor3_u32(a.y, b.y, c.y), xor3_u32(a.z, b.z, c.z),
+                    xor3_u32(a.w, b.w, c.w));
+}
+
 // a * x mod P
 __device__ __forceinline__ f128 f128_mulx(f128 v) {
   uint32_t c = v.w >> 31;
