@@ -115,8 +115,25 @@ void Ctx::resolve_phases() {
   phase_events.clear();
 }
 
+uint8_t *Ctx::pinned_alloc(size_t n) {
+  if (!pinned) {
+    const size_t want = 64ull << 20;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+    pinned = (uint8_t *)p;
+    pinned_bytes = want;
+    pinned_off = 0;
+  }
+  const size_t at = (pinned_off + 255) & ~(size_t)255;
+  if (at + n > pinned_bytes) return nullptr;
+  pinned_off = at + n;
+  return pinned + at;
+}
+
 Ctx::~Ctx() {
   hipSetDevice(device);
+  if (pinned) hipHostFree(pinned);
+  if (landing) hipHostFree(landing);
   for (auto &kv : twiddles_dev) hipFree(kv.second);
   for (auto &kv : scratch)
     if (kv.second.ptr) hipFree(kv.second.ptr);

@@ -44,6 +44,15 @@ struct Ctx {
   bool csc_uploaded = false;
   // Bytes of the last proof made on this context.
   std::vector<uint8_t> last_proof;
+  // Pinned host staging: an upload ring (asynchronous host-to-device copies
+  // read from it; reset only after the stream has drained) and a landing
+  // area for small device-to-host reads.
+  uint8_t *pinned = nullptr;
+  size_t pinned_bytes = 0, pinned_off = 0;
+  uint8_t *landing = nullptr;
+  size_t landing_bytes = 0;
+  uint8_t *pinned_alloc(size_t n);  // nullptr when the ring is full (or unavailable)
+  void pinned_reset() { pinned_off = 0; }
 
   void *get_scratch(const std::string &name, size_t bytes);
   const std::vector<F128> &host_twiddles(int dim);

@@ -29,7 +29,10 @@ int xor_reduce_device(Ctx *ctx, hipStream_t st, const void *in, uint32_t ny, uin
 int fold_at_z_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_w64, uint64_t rows, void *A,
                      void *B);
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out);
-int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2);
+// (G(1), G(inf)) of one multilinear zerocheck round; writes the next round's
+// eq table into E_next (half / 2 entries) unless it is null.
+int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2,
+                    void *E_next);
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3);
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);

@@ -247,14 +247,35 @@ struct Prover {
 
   void *dev(const char *name, size_t bytes) { return ctx->get_scratch(name, bytes); }
 
+  // Device-to-host read; waits for the stream (the host needs the bytes).
+  // Small reads land in pinned memory first.
   int d2h(void *h, const void *d, size_t n) {
+    if (n <= ctx->landing_bytes) {
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, d, n, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      memcpy(h, ctx->landing, n);
+      return FLOCK_OK;
+    }
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, st));
     FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
     return FLOCK_OK;
   }
+  // Host-to-device upload without waiting: the bytes are staged in the
+  // pinned ring, which is only recycled after the stream has drained.
   int h2d(void *d, const void *h, size_t n) {
-    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st));
-    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    uint8_t *p = ctx->pinned_alloc(n);
+    if (!p) {
+      FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      ctx->pinned_reset();
+      p = ctx->pinned_alloc(n);
+    }
+    if (!p) {  // larger than the ring: plain synchronous copy
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st));
+      FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      return FLOCK_OK;
+    }
+    memcpy(p, h, n);
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, st));
     return FLOCK_OK;
   }
 
@@ -392,6 +413,16 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     return FLOCK_E_SHAPE;
   }
   Prover P{ctx, st, m, m - 7, cfg, {}};
+  // The stream is idle between proves, so the upload ring can be recycled.
+  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  ctx->pinned_reset();
+  if (!ctx->landing) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, 1 << 20, hipHostMallocDefault) == hipSuccess) {
+      ctx->landing = (uint8_t *)p;
+      ctx->landing_bytes = 1 << 20;
+    }
+  }
   ScopedPhase ph_all(ctx, "prove", st);
   int rc = 0;
   const int L = m - 7, n = m - 6;
@@ -491,15 +522,18 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     const size_t rows = words;
     void *A = P.dev("zc.A", rows * 16), *Bv = P.dev("zc.B", rows * 16);
     void *A2 = P.dev("zc.A2", rows * 8), *B2 = P.dev("zc.B2", rows * 8);
-    void *E = P.dev("zc.E", rows * 8), *dw = P.dev("zc.w", 64 * 16), *dmsg = P.dev("zc.msg", 32);
-    if (!A || !Bv || !A2 || !B2 || !E || !dw || !dmsg) return FLOCK_E_OOM;
+    void *E = P.dev("zc.E", rows * 8), *E2 = P.dev("zc.E2", rows * 4), *dw = P.dev("zc.w", 64 * 16);
+    void *dmsg = P.dev("zc.msg", 32);
+    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dw || !dmsg) return FLOCK_E_OOM;
     if ((rc = P.h2d(dw, ws, sizeof ws))) return rc;
     if ((rc = fold_at_z_device(ctx, st, a, b, dw, rows, A, Bv))) return rc;
     size_t len = rows;
+    if ((rc = P.eq_device(&r[7], n - 1, E))) return rc;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
-      if ((rc = P.eq_device(&r[6 + i + 1], n - i - 1, E))) return rc;
-      if ((rc = zc_round_device(ctx, st, A, Bv, E, half, dmsg))) return rc;
+      // round i's eq table is eq(r[7 + i ..]); the kernel folds it for round i + 1
+      if ((rc = zc_round_device(ctx, st, A, Bv, E, half, dmsg, i + 1 < n ? E2 : nullptr))) return rc;
+      std::swap(E, E2);
       uint4 h[2];
       if ((rc = P.d2h(h, dmsg, 32))) return rc;
       F128 g1 = fv(h[0]), gi = fv(h[1]);
