@@ -107,17 +107,14 @@ const ZcTables &zc_tables() {
         }
         M[j][s] = f8_mul(num, f8_inv(den));
       }
-    for (int p = 0; p < 16; ++p)
-      for (int v = 0; v < 16; ++v)
-        for (int jq = 0; jq < 16; ++jq) {
-          uint32_t word = 0;
-          for (int k = 0; k < 4; ++k) {
-            uint8_t acc = 0;
-            for (int t = 0; t < 4; ++t)
-              if ((v >> t) & 1) acc ^= M[4 * jq + k][4 * p + t];
-            word |= (uint32_t)acc << (8 * k);
-          }
-          T.ext[(p * 16 + v) * 16 + jq] = word;
+    uint8_t *ext8 = (uint8_t *)T.ext8;
+    for (int v = 0; v < 256; ++v)
+      for (int p = 0; p < 8; ++p)
+        for (int lam = 0; lam < 64; ++lam) {
+          uint8_t acc = 0;
+          for (int t = 0; t < 8; ++t)
+            if ((v >> t) & 1) acc ^= M[lam][8 * p + t];
+          ext8[(v * 8 + p) * 64 + lam] = acc;
         }
     uint8_t x = 1;
     for (int i = 0; i < 255; ++i) {
@@ -475,7 +472,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     ScopedPhase ph(ctx, "zc_round1_all", st);
     void *dt = P.dev("zc.tables", sizeof(ZcTables));
     void *dehi = P.dev("zc.ehi", ((size_t)1 << (m - 13)) * 16);
-    void *dout = P.dev("zc.r1out", 64 * 16);
+    void *dout = P.dev("zc.r1q", 64 * 128 * 16);
     void *dhic = P.dev("zc.hic", ((size_t)1 << (m - 11)) * 16), *dq = P.dev("zc.cq", 2048 * 16);
     if (!dt || !dehi || !dout || !dhic || !dq) return FLOCK_E_OOM;
     if ((rc = P.h2d(dt, &zc_tables(), sizeof(ZcTables)))) return rc;
@@ -485,9 +482,22 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // elements are row pairs (x = 2e + h), e = il + 16 ih.
     if ((rc = P.eq_device(&r[11], m - 11, dhic))) return rc;
     if ((rc = bitfold_device(ctx, st, z, 4, 1ull << (m - 11), dhic, nullptr, dq))) return rc;
-    uint4 h[64];
-    if ((rc = P.d2h(h, dout, sizeof h))) return rc;
-    for (int j = 0; j < 64; ++j) r1ab[j] = fv(h[j]);
+    {
+      // P^AB(lambda_j) = sum_{x_mid, t} E4(x_mid) phi8(2^t) Q[4 x_mid + j / 16][8 (j % 16) + t]
+      std::vector<F128> qab(64 * 128);
+      if ((rc = P.d2h(qab.data(), dout, qab.size() * 16))) return rc;
+      const ZcTables &T = zc_tables();
+      for (int j = 0; j < 64; ++j) {
+        F128 acc;
+        for (int xm = 0; xm < 16; ++xm)
+          for (int t = 0; t < 8; ++t) {
+            F128 w;
+            memcpy(&w, T.pt[xm][t], 16);
+            acc += w * qab[(4 * xm + j / 16) * 128 + 8 * (j % 16) + t];
+          }
+        r1ab[j] = acc;
+      }
+    }
     std::vector<F128> q(2048);
     if ((rc = P.d2h(q.data(), dq, 2048 * 16))) return rc;
     std::vector<F128> lo = eq_table(&r[7], 4);

@@ -6,7 +6,10 @@
 namespace flock {
 
 struct ZcTables {
-  uint32_t ext[16 * 16 * 16];  // [nibble pos p][nibble v][jq] -> 4 packed Lambda bytes
+  // S -> Lambda extension by byte: the 64 Lambda bytes of the contribution
+  // of byte value v at byte position p of a row sit at byte offset
+  // (v * 8 + p) * 64 (16-byte chunk q4 = lambda / 16 at (v * 8 + p) * 4 + q4).
+  alignas(16) uint32_t ext8[256 * 8 * 16];
   uint16_t log8[256];          // discrete log base 3; log8[0] = 1024 (zero sentinel)
   uint8_t exp8[3 * 1024 + 4];  // exp8[i] = 3^(i mod 255) for i < 1024, else 0
   uint16_t log_eps[8];         // log of eps(x_lo)

@@ -7,19 +7,21 @@
 // beta_1..beta_4, r_outer...] the first three eq coordinates are images of
 // GF(2^8) constants, so the eq weight of the low three row bits, eps(x_lo),
 // lives in GF(2^8) and
-//   Y_j = sum_{x_lo} eps(x_lo) * ahat_j * bhat_j        (all in GF(2^8))
-// is formed without touching GF(2^128).  The next four coordinates are fixed
-// constants E4(x_mid); per outer index x_hi the workgroup scales the 16 x 8
-// table E4(x_mid) * phi8(2^t) by the outer weight Ehi(x_hi) and folds Y_j in
-// with two nibble lookups.  P^AB(lambda_j) = sum over everything.  This
-// matches the reference round1_naive (zerocheck/univariate_skip.rs:84-138)
-// exactly: every step is a regrouping of the same exact field sum.  P^C is
-// linear in the bits of c and is computed from a bit-column fold (bitfold.hip).
+//   Y_j(x_mid, x_hi) = sum_{x_lo} eps(x_lo) * ahat_j * bhat_j    (in GF(2^8))
+// is formed without touching GF(2^128).  zc_y_kernel writes these bytes;
+// then P^AB(lambda_j) = sum_{x_mid} E4(x_mid) sum_t phi8(2^t)
+// sum_{x_hi} Ehi(x_hi) bit_t(Y_j(x_mid, x_hi)), whose inner sums are a
+// bit-column fold of Y with the weights Ehi (bitfold.hip).  This matches the
+// reference round1_naive (zerocheck/univariate_skip.rs:84-138) exactly: every
+// step is a regrouping of the same exact field sum.  P^C is linear in the
+// bits of c and is computed from a bit-column fold of c.
 //
-// Thread layout: q4 = t & 3 picks lambda 16 q4 .. 16 q4 + 15 (one uint4 of
-// extension bytes per nibble lookup), xs = (t >> 2) & 3 picks rows
-// x_lo = 2 xs, 2 xs + 1, x_mid = t >> 4.  The four xs lanes of a (x_mid, q4)
-// XOR their partial Y, then each keeps four lambdas for the F128 accumulation.
+// Extension by byte tables: ahat over all 64 lambdas is the XOR of 8 table
+// rows, one per byte of the row.  Thread (x_mid, q4) handles lambdas
+// 16 q4 .. 16 q4 + 15 (one 16-byte LDS read per byte).  Entry (v, p, q4) is
+// at chunk (v * 8 + p) * 4 + q4, so its bank group (mod 16) is
+// 4 (p mod 4) + q4; a lane visits byte positions p = step ^ (x_mid mod 4),
+// which makes the 16 lanes served together hit 16 distinct bank groups.
 #include <hip/hip_runtime.h>
 
 #include "../../include/flock_gpu.h"
@@ -41,106 +43,58 @@ __device__ __forceinline__ uint32_t f8_prod4(uint32_t ea, uint32_t eb, uint32_t 
   return y;
 }
 
-__device__ __forceinline__ uint4 ext16(uint64_t row, const uint4 *__restrict__ s_ext, uint32_t q4) {
+__device__ __forceinline__ uint4 ext8(uint64_t row, const uint4 *__restrict__ s_ext, uint32_t q4, uint32_t sw) {
+  const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
   uint4 e = make_uint4(0, 0, 0, 0);
 #pragma unroll
-  for (int p = 0; p < 16; ++p) {
-    const uint4 v = s_ext[(p * 16 + (uint32_t)((row >> (4 * p)) & 15)) * 4 + q4];
-    e.x ^= v.x;
-    e.y ^= v.y;
-    e.z ^= v.z;
-    e.w ^= v.w;
+  for (uint32_t step = 0; step < 8; ++step) {
+    const uint32_t p = step ^ sw;
+    const uint32_t w = step < 4 ? lo : hi;
+    const uint32_t v = __builtin_amdgcn_ubfe(w, 8 * (p & 3), 8);
+    const uint4 t = s_ext[(v * 8 + p) * 4 + q4];
+    e.x ^= t.x;
+    e.y ^= t.y;
+    e.z ^= t.z;
+    e.w ^= t.w;
   }
   return e;
 }
 
-__global__ __launch_bounds__(256) void zc_round1_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
-                                                        const ZcTables *__restrict__ T, const f128 *__restrict__ ehi,
-                                                        uint64_t n_hi, uint64_t hi_per_wg, f128 *__restrict__ partial) {
-  __shared__ uint4 s_ext[16 * 16 * 4];
+constexpr int ZCY_THREADS = 1024;
+
+__global__ __launch_bounds__(ZCY_THREADS) void zc_y_kernel(const uint64_t *__restrict__ a,
+                                                           const uint64_t *__restrict__ b,
+                                                           const ZcTables *__restrict__ T, uint64_t n_blocks,
+                                                           uint4 *__restrict__ Y) {
+  __shared__ uint4 s_ext[256 * 8 * 4];
   __shared__ uint16_t s_log[256];
   __shared__ uint8_t s_exp[3 * 1024 + 4];
-  __shared__ f128 s_tab[512];     // byte table of Ehi(x_hi)
-  __shared__ f128 s_pt[16][8];    // E4 * phi8(2^t), unscaled
-  __shared__ f128 s_pts[16][8];   // scaled by Ehi(x_hi)
-  __shared__ f128 s_ptn[16][2][16];
   __shared__ uint32_t s_leps[8];
   const uint32_t tid = threadIdx.x;
   {
-    const uint4 *src = (const uint4 *)T->ext;
-    for (uint32_t i = tid; i < 16 * 16 * 4; i += 256) s_ext[i] = src[i];
+    const uint4 *src = (const uint4 *)T->ext8;
+    for (uint32_t i = tid; i < 256 * 8 * 4; i += ZCY_THREADS) s_ext[i] = src[i];
   }
-  s_log[tid] = T->log8[tid];
-  for (uint32_t i = tid; i < 3 * 1024 + 4; i += 256) s_exp[i] = T->exp8[i];
-  if (tid < 128) s_pt[tid >> 3][tid & 7] = *(const f128 *)T->pt[tid >> 3][tid & 7];
+  if (tid < 256) s_log[tid] = T->log8[tid];
+  for (uint32_t i = tid; i < 3 * 1024 + 4; i += ZCY_THREADS) s_exp[i] = T->exp8[i];
   if (tid < 8) s_leps[tid] = T->log_eps[tid];
-  const uint32_t q4 = tid & 3, xs = (tid >> 2) & 3, x_mid = tid >> 4;
-  f128 acc[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) acc[k] = f128_zero();
   __syncthreads();
-
-  const uint64_t hi0 = (uint64_t)blockIdx.x * hi_per_wg;
-  uint64_t hi1 = hi0 + hi_per_wg;
-  if (hi1 > n_hi) hi1 = n_hi;
-  for (uint64_t xh = hi0; xh < hi1; ++xh) {
-    // (1) two rows -> partial GF(2^8) Y for 16 lambdas
-    const uint64_t row0 = (xh << 7) | ((uint64_t)x_mid << 3) | (2 * xs);
+  const uint32_t q4 = tid & 3, xm = (tid >> 2) & 15, xo = tid >> 6, sw = xm & 3;
+  for (uint64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const uint64_t xh = blk * 16 + xo;
+    const uint64_t row0 = (xh << 7) | ((uint64_t)xm << 3);
     uint4 y = make_uint4(0, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const uint4 ea = ext16(a[row0 + r], s_ext, q4);
-      const uint4 eb = ext16(b[row0 + r], s_ext, q4);
-      const uint32_t le = s_leps[2 * xs + r];
+#pragma unroll 1
+    for (uint32_t xl = 0; xl < 8; ++xl) {
+      const uint4 ea = ext8(a[row0 + xl], s_ext, q4, sw);
+      const uint4 eb = ext8(b[row0 + xl], s_ext, q4, sw);
+      const uint32_t le = s_leps[xl];
       y.x ^= f8_prod4(ea.x, eb.x, le, s_log, s_exp);
       y.y ^= f8_prod4(ea.y, eb.y, le, s_log, s_exp);
       y.z ^= f8_prod4(ea.z, eb.z, le, s_log, s_exp);
       y.w ^= f8_prod4(ea.w, eb.w, le, s_log, s_exp);
     }
-    // (2) combine the four xs lanes (lanes differing in bits 2 and 3)
-#pragma unroll
-    for (int d = 4; d <= 8; d <<= 1) {
-      y.x ^= __shfl_xor(y.x, d);
-      y.y ^= __shfl_xor(y.y, d);
-      y.z ^= __shfl_xor(y.z, d);
-      y.w ^= __shfl_xor(y.w, d);
-    }
-    const uint32_t yw = xs == 0 ? y.x : xs == 1 ? y.y : xs == 2 ? y.z : y.w;
-    // (3) scale E4 * phi8(2^t) by Ehi(x_hi) and expand to nibble tables
-    __syncthreads();
-    if (tid < 64) f128_build_bytetab(ehi[xh], s_tab, tid);
-    __syncthreads();
-    if (tid < 128) s_pts[tid >> 3][tid & 7] = f128_mul_bytetab(s_pt[tid >> 3][tid & 7], s_tab);
-    __syncthreads();
-    for (uint32_t e = tid; e < 512; e += 256) {
-      const uint32_t xm = e >> 5, h = (e >> 4) & 1, v = e & 15;
-      f128 s = f128_zero();
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-        if ((v >> t) & 1) f128_xor_into(s, s_pts[xm][4 * h + t]);
-      s_ptn[xm][h][v] = s;
-    }
-    __syncthreads();
-    // (4) accumulate lambda = 16 q4 + 4 xs + k
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t yb = (yw >> (8 * k)) & 255u;
-      f128_xor_into(acc[k], s_ptn[x_mid][0][yb & 15]);
-      f128_xor_into(acc[k], s_ptn[x_mid][1][yb >> 4]);
-    }
-  }
-  // reduce over the 16 x_mid values; reuse s_ext (16 KiB = 1024 f128) as scratch
-  __syncthreads();
-  f128 *red = (f128 *)s_ext;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) red[k * 256 + tid] = acc[k];
-  __syncthreads();
-  if (tid < 64) {
-    // output lambda j = tid: q4 = j >> 4, xs = (j >> 2) & 3, k = j & 3
-    const uint32_t j = tid, jq4 = j >> 4, jxs = (j >> 2) & 3, k = j & 3;
-    f128 s = f128_zero();
-    for (uint32_t xm = 0; xm < 16; ++xm) f128_xor_into(s, red[k * 256 + (xm << 4) + (jxs << 2) + jq4]);
-    partial[(uint64_t)blockIdx.x * 64 + j] = s;
+    Y[((xh << 4) + xm) * 4 + q4] = y;
   }
 }
 
@@ -166,19 +120,22 @@ int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, ui
 }
 
 int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int m, const void *d_tables,
-                     const void *d_ehi, void *d_out64) {
+                     const void *d_ehi, void *d_q) {
   ScopedPhase ph(ctx, "zc_round1", st);
-  const uint64_t n_hi = 1ull << (m - 6 - 7);
-  uint64_t wgs = 2048;
-  if (wgs > n_hi) wgs = n_hi;
-  const uint64_t per = (n_hi + wgs - 1) / wgs;
-  wgs = (n_hi + per - 1) / per;
-  f128 *part = (f128 *)ctx->get_scratch("zc.round1.partial", wgs * 64 * sizeof(f128));
-  if (!part) return FLOCK_E_OOM;
-  hipLaunchKernelGGL(zc_round1_kernel, dim3((unsigned)wgs), dim3(256), 0, st, (const uint64_t *)a,
-                     (const uint64_t *)b, (const ZcTables *)d_tables, (const f128 *)d_ehi, n_hi, per, part);
+  const uint64_t n_hi = 1ull << (m - 13);
+  if (n_hi < 16) {
+    ctx->last_error = "zerocheck: m too small";
+    return FLOCK_E_SHAPE;
+  }
+  const uint64_t n_blocks = n_hi / 16;
+  uint4 *Y = (uint4 *)ctx->get_scratch("zc.round1.y", (size_t)n_hi * 16 * 64);
+  if (!Y) return FLOCK_E_OOM;
+  const unsigned grid = (unsigned)(n_blocks < 1024 ? n_blocks : 1024);
+  hipLaunchKernelGGL(zc_y_kernel, dim3(grid), dim3(ZCY_THREADS), 0, st, (const uint64_t *)a, (const uint64_t *)b,
+                     (const ZcTables *)d_tables, n_blocks, Y);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_reduce_device(ctx, st, part, 1, (uint32_t)wgs, 64, d_out64);
+  // Q[4 x_mid + q4][8 j + t] = sum_{x_hi} Ehi(x_hi) bit_t(Y_{16 q4 + j}(x_mid, x_hi))
+  return bitfold_device(ctx, st, Y, 6, n_hi, d_ehi, nullptr, d_q);
 }
 
 size_t zc_tables_bytes() { return sizeof(ZcTables); }
