@@ -52,19 +52,31 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-# Algorithmic HBM bytes per launch of the prover's heavy kernels at m = 32
-# (W = 2^29 bytes, one packed bit-vector; 2^25 F128 message; 2^26 F128
-# codeword).  Each persistent input read once, each output written once.
+# Algorithmic HBM bytes per launch of the prover's heavy kernels (W = 2^(m-3)
+# bytes, one packed bit-vector = 2^(m-7) F128; the rate-1/2 codeword is 2W).
+# Each persistent input read once, each output written once.
 def algorithmic_bytes(name, m):
     W = 1 << (m - 3)
-    cw = 2 * (W << 1)  # codeword: 2^(m-7) F128 x 2 (rate 1/2) x 16 B = 4W
+    cw = 2 * W
     return {
-        "ntt_pass_kernel": 2 * cw,          # read + write the codeword
-        "zc_round1": 3 * W,                 # read a, b, c (= z)
+        "ntt_pass_shared_kernel": 2 * cw,   # read + write the codeword
+        "ntt_pass_kernel": 2 * cw,
         "zc_fold_at_z": 2 * W + 2 * 4 * W,  # read a, b; write A, B (2^(m-6) F128 = 4W each)
         "witness": 3 * W + (1 << (m - 14)) * 112,  # write z, a, b; read the compressions
         "merkle_leaf_kernel": cw + 32 * (cw // 1024),
     }.get(name)
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "prove_m32_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["traffic_bytes"]
+    except Exception:
+        return None
 
 
 def bench_prove(args):
@@ -84,6 +96,14 @@ def bench_prove(args):
     warm_seeds = [canary_seed] + [WORKER_SEED] * max(0, args.warmup - 1)
     timed_seeds = list(range(1, args.steps + 1))
     bufs = {s: flockgpu.generate_compressions_device(ctx, log2, s) for s in set(warm_seeds + timed_seeds)}
+    if args.calibrate_copy:
+        # A plain 1 GiB device copy, the reference dispatch for scaling the
+        # FETCH_SIZE / WRITE_SIZE counters of a profiled run.
+        src, dst = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+        ctx.copy(dst, src, 1 << 30)
+        ctx.copy(dst, src, 1 << 30)
+        src.free()
+        dst.free()
     ctx.synchronize()
     for i, s in enumerate(warm_seeds):
         p = setup.prove_fast_device(bufs[s].ptr, n)
@@ -140,7 +160,7 @@ def bench_prove(args):
                    "proof_bytes": len(proofs[0]), "verified": verified,
                    "device_ms_per_proof": dev_ms, "phases_ms_per_proof": phases},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(kname), "kernel": kname,
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": abytes},
     }
     if not args.no_cpu_baseline:
@@ -234,6 +254,8 @@ def main():
     ap.add_argument("--workload", default="prove", choices=["prove", "commit"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true", help="skip the verifier check (profiling runs only)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter calibration for profiled runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))

@@ -60,6 +60,14 @@ extern "C" int flock_memcpy_d2h(flock_ctx *ctx, void *dst, const void *d_src, si
   return FLOCK_OK;
 }
 
+extern "C" int flock_memcpy_d2d(flock_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
+  if (!ctx || (!d_src && bytes) || (!d_dst && bytes)) return FLOCK_E_ARG;
+  hipSetDevice(ctx->c.device);
+  FLOCK_HIP_CHECK(&ctx->c, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  return FLOCK_OK;
+}
+
 extern "C" int flock_synchronize(flock_ctx *ctx) {
   if (!ctx) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);

@@ -75,6 +75,7 @@ def lib():
         L.flock_device_free.argtypes = [P, P]
         L.flock_memcpy_h2d.argtypes = [P, P, P, ctypes.c_size_t]
         L.flock_memcpy_d2h.argtypes = [P, P, P, ctypes.c_size_t]
+        L.flock_memcpy_d2d.argtypes = [P, P, P, ctypes.c_size_t]
         L.flock_synchronize.argtypes = [P]
         L.flock_timing_enable.argtypes = [P, ctypes.c_int]
         L.flock_timing_report.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
@@ -96,7 +97,7 @@ def exported_symbols():
     """Names the header declares, for the ABI smoke test."""
     return ["flock_ctx_create", "flock_ctx_destroy", "flock_last_error", "flock_commit", "flock_commit_device",
             "flock_timing_enable", "flock_timing_report", "flock_device_count", "flock_device_alloc",
-            "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_synchronize", "flock_setup_create",
+            "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_memcpy_d2d", "flock_synchronize", "flock_setup_create",
             "flock_setup_destroy", "flock_setup_info", "flock_prove_blake3", "flock_prove_blake3_device",
             "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof"]
 
@@ -184,6 +185,10 @@ class Context:
 
     def synchronize(self):
         self._check(lib().flock_synchronize(self.handle))
+
+    def copy(self, dst, src, nbytes):
+        """Device-to-device copy between two DeviceBuffers (the runtime's copy kernel)."""
+        self._check(lib().flock_memcpy_d2d(self.handle, ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes))
 
     # ---------------------------------------------------------------- commit
     def commit(self, params, z_packed):

@@ -81,6 +81,7 @@ int flock_device_alloc(flock_ctx *ctx, size_t bytes, void **d_ptr);
 int flock_device_free(flock_ctx *ctx, void *d_ptr);
 int flock_memcpy_h2d(flock_ctx *ctx, void *d_dst, const void *src, size_t bytes);
 int flock_memcpy_d2h(flock_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int flock_memcpy_d2d(flock_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 int flock_synchronize(flock_ctx *ctx);
 
 /* ------------------------------------------------------------------ prover */
