@@ -7,6 +7,17 @@
  *
  * Every function cites the reference file:line it restates (paths relative
  * to the Layr-Labs/flock-challenge checkout).
+ *
+ * Pins (tests/test_oracle_*.py): GHASH known answers; the PHI_8_TABLE literal
+ * values and the phi8 homomorphism; published BLAKE3 digests and an
+ * independent Python BLAKE3 / GF(2^128); SplitMix64 outputs; the novel-basis
+ * NTT by brute force on tiny domains; the GF(2^8) S -> Lambda extension
+ * against a restatement of the reference's F8 LCH NTT; R1CS satisfaction by
+ * honest witnesses and rejection of mutated ones; prove -> verify round
+ * trips with tamper rejection; the reference's canary proof length (437,551
+ * bytes at m = 32).
+ * Parity unpinned (no reference datum reachable here): the statement digest
+ * value, Merkle roots, and proof bytes beyond the canary length.
  */
 #ifndef FLOCK_ORACLE_H
 #define FLOCK_ORACLE_H
