@@ -81,6 +81,11 @@ extern "C" int flock_ctx_create(int device, flock_ctx **out) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return FLOCK_E_DEVICE;
   if (hipSetDevice(device) != hipSuccess) return FLOCK_E_DEVICE;
+  // The prover waits on the stream after every round message (the host holds
+  // the Fiat-Shamir transcript); spinning instead of sleeping shortens each
+  // wait.  Only possible before the device is first used; otherwise ignored.
+  (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+  (void)hipGetLastError();
   flock_ctx *c = new (std::nothrow) flock_ctx();
   if (!c) return FLOCK_E_OOM;
   c->c.device = device;

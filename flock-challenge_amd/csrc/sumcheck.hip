@@ -13,6 +13,7 @@
 #include "../../include/flock_gpu.h"
 #include "blake3_dev.h"
 #include "ctx.h"
+#include "fulltab.h"
 #include "gf128_dev.h"
 #include "prove_kernels.h"
 
@@ -34,24 +35,40 @@ __device__ __forceinline__ void wg_xor_reduce(f128 v, f128 *red, f128 *out) {
 
 // ---------------------------------------------------------------- eq tables
 // out[i] = lo[i & (2^k - 1)] * hi[i >> k].  Each workgroup covers a range
-// inside one hi index, so hi is a workgroup-uniform multiplier.
-__global__ __launch_bounds__(256) void eq_expand_kernel(const f128 *__restrict__ lo, int k, const f128 *__restrict__ hi,
-                                                        uint64_t per_wg, f128 *__restrict__ out) {
-  __shared__ f128 tab[512];
+// inside one hi index, so hi is a workgroup-uniform multiplier: its full byte
+// tables (fulltab.h) are built once per workgroup.
+constexpr int FT_THREADS = 1024;
+
+__global__ __launch_bounds__(FT_THREADS) void eq_expand_kernel(const f128 *__restrict__ lo, int k,
+                                                               const f128 *__restrict__ hi, uint64_t per_wg,
+                                                               f128 *__restrict__ out) {
+  __shared__ f128 tab[4096];
+  __shared__ f128 basis[128];
+  const uint32_t tid = threadIdx.x, rot = tid & 15;
   const uint64_t lo_n = 1ull << k;
   const uint64_t chunks = lo_n / per_wg;
   const uint64_t h = blockIdx.x / chunks, c0 = (blockIdx.x % chunks) * per_wg;
-  if (threadIdx.x < 64) f128_build_bytetab(hi[h], tab, threadIdx.x);
+  fulltab_basis(hi[h], basis, tid);
   __syncthreads();
-  for (uint64_t i = c0 + threadIdx.x; i < c0 + per_wg; i += 256) out[(h << k) + i] = f128_mul_bytetab(lo[i], tab);
+  build_fulltab<FT_THREADS>(basis, tab, tid);
+  __syncthreads();
+  const FulltabLane L(rot);
+  uint64_t i = c0 + tid;
+  for (; i + FT_THREADS < c0 + per_wg; i += 2 * FT_THREADS) {
+    f128 p0, p1;
+    f128_mul2_fulltab(lo[i], lo[i + FT_THREADS], (const char *)tab, L, rot, p0, p1);
+    out[(h << k) + i] = p0;
+    out[(h << k) + i + FT_THREADS] = p1;
+  }
+  if (i < c0 + per_wg) out[(h << k) + i] = f128_mul_fulltab(lo[i], (const char *)tab, L, rot);
 }
 
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out) {
   uint64_t lo_n = 1ull << k;
-  uint64_t per = lo_n < 4096 ? lo_n : 4096;
+  uint64_t per = lo_n < 8192 ? lo_n : 8192;
   uint64_t wgs = (lo_n / per) << n_hi_log;
-  hipLaunchKernelGGL(eq_expand_kernel, dim3((unsigned)wgs), dim3(256), 0, st, (const f128 *)lo, k, (const f128 *)hi,
-                     per, (f128 *)out);
+  hipLaunchKernelGGL(eq_expand_kernel, dim3((unsigned)wgs), dim3(FT_THREADS), 0, st, (const f128 *)lo, k,
+                     (const f128 *)hi, per, (f128 *)out);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
@@ -110,19 +127,29 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
 }
 
 // v'[x] = v[2x] + r (v[2x] + v[2x+1]) for two vectors (fold_in_place_pair,
-// multilinear.rs:3209-3230; partial_eval_lsb, pcs/ligerito.rs:1673).
-__global__ __launch_bounds__(256) void fold_pair_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
-                                                        uint64_t half, f128 r, f128 *__restrict__ A2,
-                                                        f128 *__restrict__ B2) {
-  __shared__ f128 tab[512];
-  if (threadIdx.x < 64) f128_build_bytetab(r, tab, threadIdx.x);
+// multilinear.rs:3209-3230; partial_eval_lsb, pcs/ligerito.rs:1673).  The
+// product by r uses the workgroup's full byte tables of r (fulltab.h).
+__global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
+                                                               uint64_t half, f128 r, f128 *__restrict__ A2,
+                                                               f128 *__restrict__ B2) {
+  __shared__ f128 tab[4096];
+  __shared__ f128 basis[128];
+  const uint32_t tid = threadIdx.x, rot = tid & 15;
+  fulltab_basis(r, basis, tid);
   __syncthreads();
-  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < half; x += (uint64_t)gridDim.x * 256) {
+  build_fulltab<FT_THREADS>(basis, tab, tid);
+  __syncthreads();
+  const FulltabLane L(rot);
+  for (uint64_t x = (uint64_t)blockIdx.x * FT_THREADS + tid; x < half; x += (uint64_t)gridDim.x * FT_THREADS) {
     const f128 a0 = A[2 * x], a1 = A[2 * x + 1];
-    A2[x] = f128_xor(a0, f128_mul_bytetab(f128_xor(a0, a1), tab));
     if (B) {
       const f128 b0 = B[2 * x], b1 = B[2 * x + 1];
-      B2[x] = f128_xor(b0, f128_mul_bytetab(f128_xor(b0, b1), tab));
+      f128 pa, pb;
+      f128_mul2_fulltab(f128_xor(a0, a1), f128_xor(b0, b1), (const char *)tab, L, rot, pa, pb);
+      A2[x] = f128_xor(a0, pa);
+      B2[x] = f128_xor(b0, pb);
+    } else {
+      A2[x] = f128_xor(a0, f128_mul_fulltab(f128_xor(a0, a1), (const char *)tab, L, rot));
     }
   }
 }
@@ -130,8 +157,10 @@ __global__ __launch_bounds__(256) void fold_pair_kernel(const f128 *__restrict__
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2) {
   ScopedPhase ph(ctx, "fold_pair", st);
   f128 rr = make_uint4((uint32_t)r.lo, (uint32_t)(r.lo >> 32), (uint32_t)r.hi, (uint32_t)(r.hi >> 32));
-  hipLaunchKernelGGL(fold_pair_kernel, dim3(grid_for(half, 256, 8192)), dim3(256), 0, st, (const f128 *)A,
-                     (const f128 *)B, half, rr, (f128 *)A2, (f128 *)B2);
+  // enough elements per workgroup to amortize its table build
+  unsigned g = grid_for(half, FT_THREADS * 8, 512);
+  hipLaunchKernelGGL(fold_pair_kernel, dim3(g), dim3(FT_THREADS), 0, st, (const f128 *)A, (const f128 *)B, half, rr,
+                     (f128 *)A2, (f128 *)B2);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
