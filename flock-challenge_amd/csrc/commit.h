@@ -10,8 +10,11 @@ namespace flock {
 
 // Forward interleaved additive NTT over 2^k positions x num_lanes lanes, from
 // `start_layer`, fusing up to `max_fuse` layers per pass.
+// Forward interleaved NTT from start_layer.  When d_msg is not null the
+// first pass reads the message (positions masked to 2^(k - start_layer))
+// instead of the replicated codeword, if that pass is a shared-table pass.
 int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k, int num_lanes, int start_layer,
-                                   int max_fuse);
+                                   int max_fuse, const void *d_msg = nullptr);
 // Flat BLAKE3 Merkle tree of n_leaves leaves of leaf_bytes each.
 int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t leaf_bytes, uint64_t n_leaves,
                        void *d_tree);

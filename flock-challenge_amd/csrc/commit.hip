@@ -81,7 +81,9 @@ constexpr int SHARED_TWIDDLES = 15;  // 2^F - 1 for F = 4
 
 template <int F, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__restrict__ data,
-                                                                     const f128 *__restrict__ tw, int k, int l) {
+                                                                     const f128 *__restrict__ tw, int k, int l,
+                                                                     const f128 *__restrict__ src,
+                                                                     uint64_t src_pos_mask) {
   constexpr uint32_t THREADS = 64 * WAVES;
   static_assert((1 << F) - 1 <= SHARED_TWIDDLES, "twiddle basis area");
   __shared__ f128 lds[2 * 4096 + SHARED_TWIDDLES * 128];
@@ -104,9 +106,11 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__res
     const uint32_t c = n + 1 - (1u << j);
     basis[e] = f128_mul_xpow(tw[(1ull << (l + j)) - 1ull + (B << j) + c], 8 * kb + i);
   }
+  // The first pass of a commit reads the message itself: position p of the
+  // replicated codeword holds message position p & src_pos_mask.
   f128 v[1 << F];
 #pragma unroll
-  for (int e = 0; e < (1 << F); ++e) v[e] = data[((base | ((uint64_t)e << s)) << 6) + lane];
+  for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_pos_mask) << 6) + lane];
   __syncthreads();  // basis visible
   int buf = 0;
 #pragma unroll
@@ -212,13 +216,19 @@ int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len
   return FLOCK_OK;
 }
 
-static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k, int l, int F) {
+static bool ntt_pass_is_shared(int k, int l, int F) { return F == 4 && k - l - F >= 3; }
+
+// One pass of F layers from layer l.  src (may be null) is a message that the
+// pass reads instead of data, replicated by masking positions with
+// src_pos_mask (shared passes only).
+static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k, int l, int F,
+                    const f128 *src = nullptr, uint64_t src_pos_mask = ~0ull) {
   constexpr int SW = 8;  // wavefronts sharing one set of tables
-  if (F == 4 && k - l - F >= 3) {
+  if (ntt_pass_is_shared(k, l, F)) {
     ScopedPhase ph(ctx, "ntt_pass_shared_kernel", st);
     const uint64_t groups = 1ull << (k - F);
     hipLaunchKernelGGL((ntt_pass_shared_kernel<4, SW>), dim3((unsigned)(groups / SW)), dim3(64 * SW), 0, st, data,
-                       tw, k, l);
+                       tw, k, l, src ? src : data, src ? src_pos_mask : ~0ull);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
@@ -238,7 +248,7 @@ static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k,
 }
 
 int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k, int num_lanes, int start_layer,
-                                   int max_fuse) {
+                                   int max_fuse, const void *d_msg) {
   if (num_lanes != 64) {
     ctx->last_error = "ntt: only 64 interleaved lanes are implemented";
     return FLOCK_E_SHAPE;
@@ -246,10 +256,15 @@ int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k
   const F128 *tw = ctx->device_twiddles(k);
   if (!tw) return FLOCK_E_DEVICE;
   int l = start_layer;
+  bool first = true;
   while (l < k) {
     int F = std::min(max_fuse, k - l);
-    int rc = ntt_pass(ctx, st, (f128 *)d_data, (const f128 *)tw, k, l, F);
+    int rc = (first && d_msg)
+                 ? ntt_pass(ctx, st, (f128 *)d_data, (const f128 *)tw, k, l, F, (const f128 *)d_msg,
+                            (1ull << (k - start_layer)) - 1)
+                 : ntt_pass(ctx, st, (f128 *)d_data, (const f128 *)tw, k, l, F);
     if (rc) return rc;
+    first = false;
     l += F;
   }
   return FLOCK_OK;
@@ -297,10 +312,17 @@ int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_
   const uint64_t msg_len = 1ull << log_msg;
   const uint64_t cw_len = msg_len << p->log_inv_rate;
   ScopedPhase ph(ctx, "commit", st);
-  hipLaunchKernelGGL(replicate_kernel, dim3(2048), dim3(256), 0, st, (const uint4 *)d_z, (uint4 *)d_codeword,
-                     msg_len, cw_len);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  int rc = ntt_forward_interleaved_device(ctx, st, d_codeword, k, 1 << p->log_batch_size, (int)p->log_inv_rate, 4);
+  // The first layers equal replication; a shared first pass reads the
+  // message directly, otherwise replicate explicitly.
+  const int rate = (int)p->log_inv_rate;
+  const bool fused = rate < k && ntt_pass_is_shared(k, rate, std::min(4, k - rate));
+  if (!fused) {
+    hipLaunchKernelGGL(replicate_kernel, dim3(2048), dim3(256), 0, st, (const uint4 *)d_z, (uint4 *)d_codeword,
+                       msg_len, cw_len);
+    FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  }
+  int rc = ntt_forward_interleaved_device(ctx, st, d_codeword, k, 1 << p->log_batch_size, rate, 4,
+                                          fused ? d_z : nullptr);
   if (rc) return rc;
   rc = merkle_tree_device(ctx, st, d_codeword, 16u << p->log_batch_size, 1ull << k, d_tree);
   if (rc) return rc;

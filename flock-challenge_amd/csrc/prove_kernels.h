@@ -28,13 +28,19 @@ int bitfold_device(Ctx *ctx, hipStream_t st, const void *elems, int il_log, uint
                    const void *hi1, void *out);
 // out[y][w] = xor_x in[y][x][w] (ny stacks of nx rows of width F128).
 int xor_reduce_device(Ctx *ctx, hipStream_t st, const void *in, uint32_t ny, uint32_t nx, uint32_t width, void *out);
-int fold_at_z_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_w64, uint64_t rows, void *A,
-                     void *B);
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out);
 // (G(1), G(inf)) of one multilinear zerocheck round; writes the next round's
 // eq table into E_next (half / 2 entries) unless it is null.
 int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2,
                     void *E_next);
+// Zerocheck multilinear round 0 and its fold, reading the packed rows a, b and
+// evaluating A = L(row) through tables of L: 2048-entry byte tables d_lw8
+// (entry v * 8 + p) for the round, 256-entry nibble tables d_lw / d_lwr
+// (= rho L, entry v * 16 + p) for the fold.
+int zc_round0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *E,
+                     uint64_t half, void *out2, void *E_next);
+int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *d_lwr,
+                    uint64_t half, void *A2, void *B2);
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3);
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);

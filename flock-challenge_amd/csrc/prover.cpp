@@ -530,19 +530,50 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   F128 final_a, final_b;
   {
     const size_t rows = words;
-    void *A = P.dev("zc.A", rows * 16), *Bv = P.dev("zc.B", rows * 16);
-    void *A2 = P.dev("zc.A2", rows * 8), *B2 = P.dev("zc.B2", rows * 8);
-    void *E = P.dev("zc.E", rows * 8), *E2 = P.dev("zc.E2", rows * 4), *dw = P.dev("zc.w", 64 * 16);
-    void *dmsg = P.dev("zc.msg", 32);
-    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dw || !dmsg) return FLOCK_E_OOM;
-    if ((rc = P.h2d(dw, ws, sizeof ws))) return rc;
-    if ((rc = fold_at_z_device(ctx, st, a, b, dw, rows, A, Bv))) return rc;
+    // After round 1, A[x] = L(a[x]) with L(row) = sum_s bit_s(row) L_s(z); round
+    // 0 and its fold evaluate L on the rows, so only the half-size folded
+    // tables are materialized.
+    void *A = P.dev("zc.A", rows * 8), *Bv = P.dev("zc.B", rows * 8);
+    void *A2 = P.dev("zc.A2", rows * 4), *B2 = P.dev("zc.B2", rows * 4);
+    void *E = P.dev("zc.E", rows * 8), *E2 = P.dev("zc.E2", rows * 4);
+    void *dlw = P.dev("zc.lw", 256 * 16), *dlwr = P.dev("zc.lwr", 256 * 16), *dmsg = P.dev("zc.msg", 32);
+    void *dlw8 = P.dev("zc.lw8", 2048 * 16);
+    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw || !dlwr || !dlw8 || !dmsg) return FLOCK_E_OOM;
+    // nibble tables: lw[v * 16 + p] = sum_{t < 4, bit t of v} L_{4p + t}(z)
+    auto nibble_tables = [](const F128 *w, F128 *out) {
+      for (int v = 0; v < 16; ++v)
+        for (int p = 0; p < 16; ++p) {
+          F128 acc;
+          for (int t = 0; t < 4; ++t)
+            if ((v >> t) & 1) acc += w[4 * p + t];
+          out[v * 16 + p] = acc;
+        }
+    };
+    F128 lw[256];
+    nibble_tables(ws, lw);
+    if ((rc = P.h2d(dlw, lw, sizeof lw))) return rc;
+    {
+      // byte tables: lw8[v * 8 + p] = sum_{t < 8, bit t of v} L_{8p + t}(z)
+      std::vector<F128> lw8(2048);
+      for (int v = 0; v < 256; ++v)
+        for (int p = 0; p < 8; ++p) {
+          F128 acc;
+          for (int t = 0; t < 8; ++t)
+            if ((v >> t) & 1) acc += ws[8 * p + t];
+          lw8[v * 8 + p] = acc;
+        }
+      if ((rc = P.h2d(dlw8, lw8.data(), lw8.size() * 16))) return rc;
+    }
     size_t len = rows;
     if ((rc = P.eq_device(&r[7], n - 1, E))) return rc;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
       // round i's eq table is eq(r[7 + i ..]); the kernel folds it for round i + 1
-      if ((rc = zc_round_device(ctx, st, A, Bv, E, half, dmsg, i + 1 < n ? E2 : nullptr))) return rc;
+      if (i == 0)
+        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, dmsg, i + 1 < n ? E2 : nullptr);
+      else
+        rc = zc_round_device(ctx, st, A, Bv, E, half, dmsg, i + 1 < n ? E2 : nullptr);
+      if (rc) return rc;
       std::swap(E, E2);
       uint4 h[2];
       if ((rc = P.d2h(h, dmsg, 32))) return rc;
@@ -552,9 +583,17 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       t.observe(gi);
       F128 rho = t.sample();
       rhos.push_back(rho);
-      if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2))) return rc;
-      std::swap(A, A2);
-      std::swap(Bv, B2);
+      if (i == 0) {
+        F128 wr[64], lwr[256];
+        for (int q = 0; q < 64; ++q) wr[q] = rho * ws[q];
+        nibble_tables(wr, lwr);
+        if ((rc = P.h2d(dlwr, lwr, sizeof lwr))) return rc;
+        if ((rc = zc_fold0_device(ctx, st, a, b, dlw, dlwr, half, A, Bv))) return rc;
+      } else {
+        if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2))) return rc;
+        std::swap(A, A2);
+        std::swap(Bv, B2);
+      }
       len = half;
     }
     uint4 h[2];

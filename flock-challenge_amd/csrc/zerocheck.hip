@@ -140,42 +140,4 @@ int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int
 
 size_t zc_tables_bytes() { return sizeof(ZcTables); }
 
-// ---------------------------------------------------------------- fold at z
-// A[x] = sum_s bit_s(a[x]) L_s(z) (zerocheck/multilinear.rs:598-625) through
-// eight byte tables of the 64 Lagrange weights, for a and b together.
-__global__ __launch_bounds__(256) void fold_at_z_kernel(const uint64_t *__restrict__ a,
-                                                        const uint64_t *__restrict__ b,
-                                                        const f128 *__restrict__ w, uint64_t rows,
-                                                        f128 *__restrict__ A, f128 *__restrict__ B) {
-  __shared__ f128 tab[8][256];
-  for (uint32_t e = threadIdx.x; e < 2048; e += 256) {
-    const uint32_t p = e >> 8, v = e & 255;
-    f128 s = f128_zero();
-    for (int t = 0; t < 8; ++t)
-      if ((v >> t) & 1) f128_xor_into(s, w[8 * p + t]);
-    tab[p][v] = s;
-  }
-  __syncthreads();
-  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < rows; x += (uint64_t)gridDim.x * 256) {
-    const uint64_t ra = a[x], rb = b[x];
-    f128 sa = f128_zero(), sb = f128_zero();
-#pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      f128_xor_into(sa, tab[p][(ra >> (8 * p)) & 255]);
-      f128_xor_into(sb, tab[p][(rb >> (8 * p)) & 255]);
-    }
-    A[x] = sa;
-    B[x] = sb;
-  }
-}
-
-int fold_at_z_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_w64, uint64_t rows, void *A,
-                     void *B) {
-  ScopedPhase ph(ctx, "zc_fold_at_z", st);
-  hipLaunchKernelGGL(fold_at_z_kernel, dim3(4096), dim3(256), 0, st, (const uint64_t *)a, (const uint64_t *)b,
-                     (const f128 *)d_w64, rows, (f128 *)A, (f128 *)B);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return FLOCK_OK;
-}
-
 }  // namespace flock
