@@ -61,7 +61,6 @@ def algorithmic_bytes(name, m):
     return {
         "ntt_pass_shared_kernel": 2 * cw,   # read + write the codeword
         "ntt_pass_kernel": 2 * cw,
-        "zc_fold_at_z": 2 * W + 2 * 4 * W,  # read a, b; write A, B (2^(m-6) F128 = 4W each)
         "witness": 3 * W + (1 << (m - 14)) * 112,  # write z, a, b; read the compressions
         "merkle_leaf_kernel": cw + 32 * (cw // 1024),
     }.get(name)
@@ -110,8 +109,20 @@ def bench_prove(args):
         if i == 0:
             assert len(p) == canary["reference_expected_len"], len(p)
             assert O.blake3(p).hex() == canary["proof_blake3"], "canary proof differs from the oracle's"
+    # Per-phase breakdown (every phase timed: more events, so kept out of the
+    # timed region), which also names the dominant single kernel.
     ctx.synchronize()
     ctx.timing(True)
+    setup.prove_fast_device(bufs[warm_seeds[0]].ptr, n)
+    ctx.synchronize()
+    brk = ctx.timing_report()
+    ctx.timing(False)
+    cands = [(v[1], k) for k, v in brk.items() if algorithmic_bytes(k, m)]
+    _, kname = max(cands)
+    # Timed region: HIP events only around each proof and each launch of the
+    # dominant kernel, on the stream they run on.
+    ctx.synchronize()
+    ctx.timing(True, only=["prove", kname])
     proofs, lat = [], []
     wall0 = time.perf_counter()
     for s in timed_seeds:
@@ -133,10 +144,7 @@ def bench_prove(args):
 
     ms_per_step = 1000.0 * wall / args.steps
     dev_ms = rep["prove"][1] / rep["prove"][0]
-    phases = {k: round(v[1] / args.steps, 4) for k, v in sorted(rep.items(), key=lambda kv: -kv[1][1])}
-    # Dominant kernel with its own per-launch HIP-event timing.
-    cands = [(v[1], k) for k, v in rep.items() if algorithmic_bytes(k, m)]
-    _, kname = max(cands)
+    phases = {k: round(v[1], 4) for k, v in sorted(brk.items(), key=lambda kv: -kv[1][1])}
     cnt, tot = rep[kname]
     avg_ms = tot / cnt
     abytes = algorithmic_bytes(kname, m)
@@ -158,7 +166,8 @@ def bench_prove(args):
                        "min": 1000 * min(lat), "max": 1000 * max(lat)},
         "config": {"workload": "prove_fast 2^18 compressions (m=32, Fast, rate 1/2, blake3 FS + Merkle)",
                    "proof_bytes": len(proofs[0]), "verified": verified,
-                   "device_ms_per_proof": dev_ms, "phases_ms_per_proof": phases},
+                   "device_ms_per_proof": dev_ms,
+                   "phases_ms_one_proof": phases},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(kname), "kernel": kname,
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": abytes},

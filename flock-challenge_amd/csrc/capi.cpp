@@ -276,6 +276,23 @@ extern "C" int flock_timing_enable(flock_ctx *ctx, int on) {
   return FLOCK_OK;
 }
 
+extern "C" int flock_timing_select(flock_ctx *ctx, const char *names) {
+  if (!ctx) return FLOCK_E_ARG;
+  ctx->c.timing_only.clear();
+  if (names) {
+    std::string all(names), cur;
+    for (char ch : all + ",") {
+      if (ch == ',') {
+        if (!cur.empty()) ctx->c.timing_only.insert(cur);
+        cur.clear();
+      } else {
+        cur += ch;
+      }
+    }
+  }
+  return FLOCK_OK;
+}
+
 extern "C" int flock_timing_report(flock_ctx *ctx, char *buf, size_t cap) {
   if (!ctx || !buf || cap == 0) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);

@@ -84,11 +84,22 @@ void *Ctx::get_scratch(const std::string &name, size_t bytes) {
   return b.ptr;
 }
 
+hipEvent_t Ctx::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
 void Ctx::phase_begin(const std::string &name, hipStream_t st) {
   PhaseEvent e;
   e.name = name;
-  hipEventCreate(&e.begin);
-  hipEventCreate(&e.end);
+  e.begin = take_event();
+  e.end = take_event();
   hipEventRecord(e.begin, st);
   open_phases.push_back(phase_events.size());
   phase_events.push_back(e);
@@ -109,8 +120,8 @@ void Ctx::resolve_phases() {
       t.first += ms;
       t.second += 1;
     }
-    hipEventDestroy(e.begin);
-    hipEventDestroy(e.end);
+    event_pool.push_back(e.begin);
+    event_pool.push_back(e.end);
   }
   phase_events.clear();
 }
@@ -132,6 +143,11 @@ uint8_t *Ctx::pinned_alloc(size_t n) {
 
 Ctx::~Ctx() {
   hipSetDevice(device);
+  for (auto &e : phase_events) {
+    hipEventDestroy(e.begin);
+    hipEventDestroy(e.end);
+  }
+  for (hipEvent_t e : event_pool) hipEventDestroy(e);
   if (pinned) hipHostFree(pinned);
   if (landing) hipHostFree(landing);
   for (auto &kv : twiddles_dev) hipFree(kv.second);

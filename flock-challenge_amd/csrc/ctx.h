@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -27,6 +29,11 @@ struct PhaseEvent {
 struct Ctx {
   int device = 0;
   bool timing = false;
+  // When non-empty, only these phases are timed (fewer events on the stream).
+  std::set<std::string> timing_only;
+  bool timed(const char *name) const { return timing && (timing_only.empty() || timing_only.count(name)); }
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
   std::vector<PhaseEvent> phase_events;
   std::vector<size_t> open_phases;
   std::map<std::string, std::pair<double, long>> phase_totals;  // ms, count
@@ -64,7 +71,7 @@ struct ScopedPhase {
   Ctx *ctx;
   hipStream_t st;
   bool open;
-  ScopedPhase(Ctx *c, const char *name, hipStream_t s) : ctx(c), st(s), open(c->timing) {
+  ScopedPhase(Ctx *c, const char *name, hipStream_t s) : ctx(c), st(s), open(c->timed(name)) {
     if (open) ctx->phase_begin(name, st);
   }
   void end() {
@@ -72,6 +79,24 @@ struct ScopedPhase {
     open = false;
   }
   ~ScopedPhase() { end(); }
+};
+
+// Host wall time of a section, reported as "host:<name>" when timing is on.
+struct HostPhase {
+  Ctx *ctx;
+  const char *name;
+  std::chrono::steady_clock::time_point t0;
+  bool on;
+  HostPhase(Ctx *c, const char *n) : ctx(c), name(n), on(c->timing && c->timing_only.empty()) {
+    if (on) t0 = std::chrono::steady_clock::now();
+  }
+  ~HostPhase() {
+    if (!on) return;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    auto &t = ctx->phase_totals[std::string("host:") + name];
+    t.first += ms;
+    t.second += 1;
+  }
 };
 
 // Error plumbing: every HIP failure is turned into a return code plus text.

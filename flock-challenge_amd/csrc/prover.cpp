@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -48,17 +49,37 @@ std::vector<F128> eq_table(const F128 *r, int n) {
 
 // Lagrange weights at z over nodes phi8(first .. first + count) within the
 // node set phi8(set_first .. set_first + set_count)
-// (zerocheck/multilinear.rs:370-389, 509-563).
+// (zerocheck/multilinear.rs:370-389, 509-563).  The node set is one of the
+// two 64-point sets S = {0..63}, Lambda = {64..127} (count == set_count), so
+// w_i = prod_{j != i} (z + s_j) * inv_den_i with the denominators cached per
+// set and the numerators from prefix / suffix products.
 void lagrange(F128 z, int first, int count, int set_first, int set_count, F128 *w) {
-  for (int i = 0; i < count; ++i) {
-    F128 si = phi8((uint8_t)(first + i)), num = ONE, den = ONE;
-    for (int j = set_first; j < set_first + set_count; ++j) {
-      if (j == first + i) continue;
-      F128 sj = phi8((uint8_t)j);
-      num = num * (z + sj);
-      den = den * (si + sj);
+  static std::mutex mu;
+  static std::map<int, std::vector<F128>> inv_den;  // keyed by set_first
+  std::vector<F128> idn;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = inv_den.find(set_first * 1000 + set_count);
+    if (it == inv_den.end()) {
+      std::vector<F128> v(set_count);
+      for (int i = 0; i < set_count; ++i) {
+        F128 si = phi8((uint8_t)(set_first + i)), den = ONE;
+        for (int j = 0; j < set_count; ++j)
+          if (j != i) den = den * (si + phi8((uint8_t)(set_first + j)));
+        v[i] = inv(den);
+      }
+      it = inv_den.emplace(set_first * 1000 + set_count, v).first;
     }
-    w[i] = num * inv(den);
+    idn = it->second;
+  }
+  std::vector<F128> pre(set_count + 1), suf(set_count + 1);
+  pre[0] = ONE;
+  for (int j = 0; j < set_count; ++j) pre[j + 1] = pre[j] * (z + phi8((uint8_t)(set_first + j)));
+  suf[set_count] = ONE;
+  for (int j = set_count - 1; j >= 0; --j) suf[j] = suf[j + 1] * (z + phi8((uint8_t)(set_first + j)));
+  for (int i = 0; i < count; ++i) {
+    const int li = first + i - set_first;
+    w[i] = pre[li] * suf[li + 1] * idn[li];
   }
 }
 
@@ -247,6 +268,7 @@ struct Prover {
   // Device-to-host read; waits for the stream (the host needs the bytes).
   // Small reads land in pinned memory first.
   int d2h(void *h, const void *d, size_t n) {
+    HostPhase hp(ctx, "wait_d2h");
     if (n <= ctx->landing_bytes) {
       FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, d, n, hipMemcpyDeviceToHost, st));
       FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
@@ -279,7 +301,12 @@ struct Prover {
   // eq table over n variables into device buffer `out` (LSB-first).
   int eq_device(const F128 *r, int n, void *out) {
     int k = n < 13 ? n : 13;
-    std::vector<F128> lo = eq_table(r, k), hi = eq_table(r + k, n - k);
+    std::vector<F128> lo, hi;
+    {
+      HostPhase hp(ctx, "eq_tables");
+      lo = eq_table(r, k);
+      hi = eq_table(r + k, n - k);
+    }
     void *dlo = dev("eq.lo", lo.size() * 16), *dhi = dev("eq.hi", hi.size() * 16);
     if (!dlo || !dhi) return FLOCK_E_OOM;
     int rc = h2d(dlo, lo.data(), lo.size() * 16);
@@ -421,6 +448,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     }
   }
   ScopedPhase ph_all(ctx, "prove", st);
+  HostPhase hp_all(ctx, "prove_wall");
   int rc = 0;
   const int L = m - 7, n = m - 6;
   const size_t words = (size_t)1 << (m - 6);  // uint64 words per bit-vector
@@ -500,6 +528,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     }
     std::vector<F128> q(2048);
     if ((rc = P.d2h(q.data(), dq, 2048 * 16))) return rc;
+    HostPhase hp(ctx, "round1_combine");
     std::vector<F128> lo = eq_table(&r[7], 4);
     F128 cs[64];
     const F128 w0 = ONE + r[6], w1 = r[6];
@@ -522,9 +551,12 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   t.observe_slice(r1c, 64);
   const F128 zz = t.sample();
   F128 wl[64], ws[64];
-  lagrange(zz, 64, 64, 64, 64, wl);
+  {
+    HostPhase hp(ctx, "lagrange_z");
+    lagrange(zz, 64, 64, 64, 64, wl);
+    lagrange(zz, 0, 64, 0, 64, ws);
+  }
   const F128 final_c = dot(wl, r1c, 64);
-  lagrange(zz, 0, 64, 0, 64, ws);
   std::vector<F128> rhos;
   std::vector<std::pair<F128, F128>> zc_rounds;
   F128 final_a, final_b;
@@ -635,6 +667,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // quirky eq table (lincheck.rs:1127-1142): index i_skip + 64 i_rest
     std::vector<F128> eqi(1 << 14);
     {
+      HostPhase hp(ctx, "lincheck_eq");
       F128 lam[64];
       lagrange(zz, 0, 64, 0, 64, lam);
       std::vector<F128> er = eq_table(rhos.data(), 8);
@@ -648,6 +681,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     if ((rc = bitfold_device(ctx, st, z, 7, 1ull << (m - 14), deqo, nullptr, dzv))) return rc;
     std::vector<F128> comb(1 << 14), zv(1 << 14);
     if ((rc = P.d2h(comb.data(), dcomb, comb.size() * 16)) || (rc = P.d2h(zv.data(), dzv, zv.size() * 16))) return rc;
+    HostPhase hp(ctx, "lincheck_rounds");
     size_t len = 1 << 14;
     F128 rr[8];
     for (int rnd = 0; rnd < 8; ++rnd) {

@@ -79,6 +79,7 @@ def lib():
         L.flock_synchronize.argtypes = [P]
         L.flock_timing_enable.argtypes = [P, ctypes.c_int]
         L.flock_timing_report.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+        L.flock_timing_select.argtypes = [P, ctypes.c_char_p]
         SZ, U64, U32 = ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint32
         PSZ = ctypes.POINTER(ctypes.c_size_t)
         L.flock_setup_create.argtypes = [P, U64, U32, U32, ctypes.POINTER(P)]
@@ -96,7 +97,7 @@ def lib():
 def exported_symbols():
     """Names the header declares, for the ABI smoke test."""
     return ["flock_ctx_create", "flock_ctx_destroy", "flock_last_error", "flock_commit", "flock_commit_device",
-            "flock_timing_enable", "flock_timing_report", "flock_device_count", "flock_device_alloc",
+            "flock_timing_enable", "flock_timing_select", "flock_timing_report", "flock_device_count", "flock_device_alloc",
             "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_memcpy_d2d", "flock_synchronize", "flock_setup_create",
             "flock_setup_destroy", "flock_setup_info", "flock_prove_blake3", "flock_prove_blake3_device",
             "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof"]
@@ -207,7 +208,10 @@ class Context:
         return bytes(root)
 
     # ---------------------------------------------------------------- timing
-    def timing(self, on=True):
+    def timing(self, on=True, only=None):
+        """Enable phase timing; `only` = iterable of phase names to time (None: all)."""
+        names = ",".join(only).encode() if only else None
+        self._check(lib().flock_timing_select(self.handle, names))
         self._check(lib().flock_timing_enable(self.handle, 1 if on else 0))
 
     def timing_report(self):

@@ -135,6 +135,9 @@ int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len);
  * (crates/flock-prover/src/prover.rs:1267-1284).  Enabling resets the totals.
  * The report is text, one line per phase: "<name> <count> <total_ms>". */
 int flock_timing_enable(flock_ctx *ctx, int on);
+/* Restrict timing to the listed phases (comma-separated names; NULL or "" =
+ * every phase, plus host-side sections reported as "host:<name>"). */
+int flock_timing_select(flock_ctx *ctx, const char *names);
 int flock_timing_report(flock_ctx *ctx, char *buf, size_t cap);
 
 #ifdef __cplusplus
