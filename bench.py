@@ -8,6 +8,10 @@ Prints ONE JSON line.  Workloads:
           profile, rate 1/2, BLAKE3 transcript and Merkle hash).  Inputs are
           generate_compressions(18, seed) expanded into HBM before the timed
           region; one step = one proof, seed -> proof bundle bytes on the host.
+          The same line carries "batched" (64 instances of 2^10..2^18
+          compressions submitted at once) and "served" (open-loop Poisson
+          arrivals, 2^12..2^18 compressions, p50/p99 latency) results from
+          the proof server, unless --no-serve.
   commit  isolated PCS commit: 2^22 F128 (seeded SplitMix64) -> rate-1/4
           interleaved additive NTT to 2^24 F128 -> BLAKE3 Merkle over 2^18
           1 KiB leaves.  One step = one commit, inputs resident in HBM.
@@ -190,6 +194,111 @@ def bench_prove(args):
     return out
 
 
+def splitmix64(seed):
+    """SplitMix64 stream (benchmark-tools/common/src/lib.rs) as a generator."""
+    state = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+BATCH_SEED = 0xBA7C000000000040
+SERVE_SEED = 0x5E7E000000000041
+
+
+def run_requests(server, reqs):
+    """Submit (t_offset_s, log2, seed) requests at their offsets; collect all.
+    Returns {request_id: (log2, seed, proof, latency_s)} and the wall span."""
+    import flockgpu  # noqa: F401
+    out, meta = {}, {}
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(reqs) or len(out) < len(reqs):
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i][0] <= now:
+            _, log2, seed = reqs[i]
+            meta[server.submit_seeded(log2, seed)] = (log2, seed)
+            i += 1
+        wait_ms = 0 if i < len(reqs) else 1000
+        if i < len(reqs):
+            wait_ms = max(0, int(1000 * (reqs[i][0] - (time.perf_counter() - t0))))
+        r = server.poll(min(wait_ms, 5))
+        if r is not None:
+            rid, proof, lat, _ = r
+            out[rid] = meta[rid] + (proof, lat)
+    return out, time.perf_counter() - t0
+
+
+def summarize_served(out, span):
+    lats = sorted(v[3] for v in out.values())
+    comps = sum(1 << v[0] for v in out.values())
+    return {"requests": len(out), "compressions": comps, "wall_s": span,
+            "compressions_per_s": comps / span, "p50_ms": 1000 * percentile(lats, 50),
+            "p99_ms": 1000 * percentile(lats, 99), "max_ms": 1000 * lats[-1]}
+
+
+def check_sample(out, n_check):
+    """Oracle verifier acceptance for the n_check smallest results (untimed)."""
+    import oracle_lib as O
+    import proof_format
+    for v in out.values():
+        assert len(v[2]) < PROOF_BOUND and len(proof_format.fields(v[2])) > 0
+    ok = 0
+    for log2, seed, proof, _ in sorted(out.values())[:n_check]:
+        blocks = O.generate_compressions(log2, seed)
+        assert O.verify(proof, blocks.size, blocks) == 0, f"proof of 2^{log2} (seed {seed:#x}) rejected"
+        ok += 1
+    return ok
+
+
+def bench_batched(args):
+    """64 instances, 2^10..2^18 compressions (log2 = 10 + draw mod 9, sub-seed
+    = next draw of one SplitMix64 stream), all submitted at once to the proof
+    server; whole-batch compressions/s."""
+    import flockgpu
+    g = splitmix64(BATCH_SEED)
+    reqs = []
+    for _ in range(64):
+        log2 = 10 + next(g) % 9
+        reqs.append((0.0, log2, next(g)))
+    server = flockgpu.ProofServer(0, n_workers=args.workers)
+    try:
+        run_requests(server, [(0.0, 10, 1), (0.0, 18, 2)])  # warm each worker's buffers
+        out, span = run_requests(server, reqs)
+    finally:
+        server.close()
+    res = summarize_served(out, span)
+    res["workers"] = args.workers
+    res["verified_sample"] = check_sample(out, 2) if not args.no_verify else 0
+    return res
+
+
+def bench_served(args):
+    """Open-loop Poisson arrivals at args.rate requests/s, sizes 2^12..2^18
+    (log2 uniform), seeded inputs; p50/p99 latency submit -> bytes ready."""
+    import flockgpu
+    import random
+    rng = random.Random(SERVE_SEED)
+    g = splitmix64(SERVE_SEED)
+    t, reqs = 0.0, []
+    for _ in range(args.requests):
+        t += rng.expovariate(args.rate)
+        reqs.append((t, 12 + next(g) % 7, next(g)))
+    server = flockgpu.ProofServer(0, n_workers=args.workers)
+    try:
+        run_requests(server, [(0.0, 12, 1), (0.0, 18, 2), (0.0, 18, 3)])  # warm the workers
+        out, span = run_requests(server, reqs)
+    finally:
+        server.close()
+    res = summarize_served(out, span)
+    res.update({"offered_rate_per_s": args.rate, "workers": args.workers,
+                "verified_sample": check_sample(out, 3) if not args.no_verify else 0})
+    return res
+
+
 def bench_commit(args):
     import numpy as np
 
@@ -263,13 +372,23 @@ def main():
     ap.add_argument("--workload", default="prove", choices=["prove", "commit"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true", help="skip the verifier check (profiling runs only)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--workers", type=int, default=3, help="concurrent proofs in the batched/served runs")
+    ap.add_argument("--rate", type=float, default=100.0, help="served: offered load, requests/s")
+    ap.add_argument("--requests", type=int, default=200, help="served: number of requests")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration for profiled runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         return 2
-    out = bench_prove(args) if args.workload == "prove" else bench_commit(args)
+    if args.workload == "commit":
+        out = bench_commit(args)
+    else:
+        out = bench_prove(args)
+        if not args.no_serve:
+            out["batched"] = bench_batched(args)
+            out["served"] = bench_served(args)
     print(json.dumps(out))
     return 0
 

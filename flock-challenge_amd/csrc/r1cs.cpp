@@ -8,6 +8,8 @@
 
 #include <string.h>
 
+#include <array>
+#include <map>
 #include <mutex>
 
 #include "host_blake3.h"
@@ -194,13 +196,17 @@ static void absorb_csr(hb3::Hasher &h, const Csr &c) {
 }
 
 void statement_digest(int m, uint8_t out[32]) {
+  // One digest per m, kept for the process (several sizes may be proved
+  // concurrently by a server).
   static std::mutex mu;
-  static int cached_m = -1;
-  static uint8_t cached[32];
-  std::lock_guard<std::mutex> lk(mu);
-  if (cached_m == m) {
-    memcpy(out, cached, 32);
-    return;
+  static std::map<int, std::array<uint8_t, 32>> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(m);
+    if (it != cache.end()) {
+      memcpy(out, it->second.data(), 32);
+      return;
+    }
   }
   const Matrices &mx = matrices();
   hb3::Hasher h;
@@ -219,8 +225,10 @@ void statement_digest(int m, uint8_t out[32]) {
     h.update(row, 16);
   }
   h.finalize_xof(out, 32);
-  memcpy(cached, out, 32);
-  cached_m = m;
+  std::array<uint8_t, 32> d;
+  memcpy(d.data(), out, 32);
+  std::lock_guard<std::mutex> lk(mu);
+  cache[m] = d;
 }
 
 }  // namespace r1cs

@@ -1,0 +1,297 @@
+// Proof server: continuous batching of independent BLAKE3 proofs on one GPU.
+//
+// Requests (compressions, or a (log2, seed) pair expanded on the device by
+// generate_compressions, benchmark-tools/common/src/lib.rs:20-33) enter a FIFO
+// queue.  A pool of worker threads, each with its own context (device
+// buffers, HIP stream), admits the request at the head of the queue when a
+// worker is free and the HBM footprint of all admitted work stays within the
+// budget.  Proofs in different streams overlap on the device: one proof's
+// kernels run while another waits on its Fiat-Shamir round trip, and small
+// late-round kernels of one proof share the chip with large kernels of
+// another.  Each proof is computed by the same code as a solo prove, on
+// private buffers, so its bytes do not depend on what else is running.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/flock_gpu.h"
+#include "ctx.h"
+#include "ligerito_configs.h"
+#include "prover.h"
+#include "r1cs.h"
+
+namespace flock {
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int blocks_log2(uint64_t n) {
+  int l = 0;
+  while ((1ull << l) < (n < 8 ? 8 : n)) ++l;
+  return l;
+}
+
+}  // namespace
+
+// Device bytes a proof of 2^log2 compressions keeps in its context: about 18
+// packed bit-vectors of 2^(m-3) bytes (witness, codeword, zerocheck and
+// ring-switch tables, Ligerito folds) plus fixed tables.
+uint64_t prove_footprint_bytes(int n_log) {
+  const int m = r1cs::K_LOG + n_log;
+  return 18ull * (1ull << (m - 3)) + (256ull << 20);
+}
+
+struct Request {
+  uint64_t id = 0;
+  uint64_t n = 0;
+  int log2 = 0;
+  bool seeded = false;
+  uint64_t seed = 0;
+  std::vector<flock_compression> blocks;
+  std::vector<uint8_t> domain;
+  double t_submit = 0;
+};
+
+struct Result {
+  uint64_t id = 0;
+  int rc = 0;
+  std::string error;
+  std::vector<uint8_t> proof;
+  double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+class Server {
+ public:
+  Server(int device, int n_workers, uint64_t budget) : device_(device), budget_(budget) {
+    for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this, i] { worker_main(i); });
+    reserved_.assign(n_workers, 0);
+  }
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : workers_) t.join();
+  }
+
+  int submit(Request &&r, uint64_t *id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (prove_footprint_bytes(r.log2) > budget_) return FLOCK_E_OOM;
+    r.id = next_id_++;
+    r.t_submit = now_s();
+    *id = r.id;
+    queue_.push_back(std::move(r));
+    cv_.notify_all();
+    return FLOCK_OK;
+  }
+
+  // 1 when a result was returned, 0 on timeout.
+  int poll(int wait_ms, Result &out) {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (wait_ms != 0) {
+      auto pred = [this] { return !done_.empty(); };
+      if (wait_ms < 0)
+        done_cv_.wait(lk, pred);
+      else
+        done_cv_.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
+    }
+    if (done_.empty()) return 0;
+    out = std::move(done_.front());
+    done_.pop_front();
+    return 1;
+  }
+
+  size_t pending() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return queue_.size() + running_;
+  }
+
+ private:
+  // Can worker w take the head request without exceeding the HBM budget?
+  bool admissible(int w) const {
+    const uint64_t need = prove_footprint_bytes(queue_.front().log2);
+    uint64_t total = 0;
+    for (size_t i = 0; i < reserved_.size(); ++i) total += (int)i == w ? std::max(reserved_[i], need) : reserved_[i];
+    return total <= budget_;
+  }
+
+  void worker_main(int w) {
+    Ctx ctx;
+    ctx.device = device_;
+    hipSetDevice(device_);
+    const bool stream_ok = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking) == hipSuccess;
+    for (;;) {
+      Request req;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || (!queue_.empty() && admissible(w)); });
+        if (stop_) return;
+        req = std::move(queue_.front());
+        queue_.pop_front();
+        reserved_[w] = std::max(reserved_[w], prove_footprint_bytes(req.log2));
+        ++running_;
+      }
+      Result res;
+      res.id = req.id;
+      res.t_submit = req.t_submit;
+      res.t_start = now_s();
+      res.rc = stream_ok ? run(ctx, req, res.proof) : FLOCK_E_DEVICE;
+      if (res.rc) res.error = ctx.last_error;
+      res.t_done = now_s();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        --running_;
+        done_.push_back(std::move(res));
+      }
+      done_cv_.notify_all();
+      cv_.notify_all();
+    }
+  }
+
+  int run(Ctx &ctx, const Request &req, std::vector<uint8_t> &proof) {
+    hipSetDevice(device_);
+    const size_t bytes = req.n * sizeof(flock_compression);
+    void *d = ctx.get_scratch("server.blocks", bytes);
+    if (!d) return FLOCK_E_OOM;
+    int rc = FLOCK_OK;
+    if (req.seeded) {
+      rc = generate_compressions_device(&ctx, ctx.stream, (uint32_t)req.log2, req.seed, (flock_compression *)d);
+    } else {
+      FLOCK_HIP_CHECK(&ctx, hipMemcpyAsync(d, req.blocks.data(), bytes, hipMemcpyHostToDevice, ctx.stream));
+    }
+    if (rc) return rc;
+    return prove_blake3_device(&ctx, ctx.stream, (const flock_compression *)d, req.n, req.domain.data(),
+                               req.domain.size(), proof);
+  }
+
+  int device_;
+  uint64_t budget_;
+  std::vector<std::thread> workers_;
+  std::vector<uint64_t> reserved_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<Request> queue_;
+  std::deque<Result> done_;
+  size_t running_ = 0;
+  uint64_t next_id_ = 1;
+  bool stop_ = false;
+};
+
+}  // namespace flock
+
+// ------------------------------------------------------------------ C ABI
+struct flock_server {
+  flock::Server *s;
+  std::string last_error;
+};
+
+static bool lig_config_exists(int n_log) {
+  const int m = flock::r1cs::K_LOG + n_log;
+  for (int i = 0; i < FLOCK_LIG_N_CONFIGS; ++i)
+    if (FLOCK_LIG_CONFIGS[i].m == m && FLOCK_LIG_CONFIGS[i].profile == 0) return true;
+  return false;
+}
+
+extern "C" int flock_server_create(int device, int n_workers, uint64_t hbm_budget_bytes, flock_server **out) {
+  if (!out || n_workers < 1 || n_workers > 16) return FLOCK_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return FLOCK_E_DEVICE;
+  if (hbm_budget_bytes == 0) {
+    size_t free_b = 0, total_b = 0;
+    hipSetDevice(device);
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return FLOCK_E_DEVICE;
+    hbm_budget_bytes = free_b / 10 * 8;
+  }
+  flock::r1cs::matrices();  // build once before the workers start
+  flock_server *s = new (std::nothrow) flock_server();
+  if (!s) return FLOCK_E_OOM;
+  s->s = new (std::nothrow) flock::Server(device, n_workers, hbm_budget_bytes);
+  if (!s->s) {
+    delete s;
+    return FLOCK_E_OOM;
+  }
+  *out = s;
+  return FLOCK_OK;
+}
+
+extern "C" void flock_server_destroy(flock_server *s) {
+  if (!s) return;
+  delete s->s;
+  delete s;
+}
+
+extern "C" const char *flock_server_last_error(const flock_server *s) {
+  return s ? s->last_error.c_str() : "null server";
+}
+
+static int submit_common(flock_server *s, flock::Request &&r, uint64_t *id) {
+  if (!lig_config_exists(r.log2)) {
+    s->last_error = "no Ligerito configuration for this size (2^8 .. 2^21 compressions)";
+    return FLOCK_E_SHAPE;
+  }
+  int rc = s->s->submit(std::move(r), id);
+  if (rc) s->last_error = "request exceeds the HBM budget";
+  return rc;
+}
+
+extern "C" int flock_server_submit(flock_server *s, const flock_compression *blocks, uint64_t n_blocks,
+                                   const uint8_t *domain, size_t domain_len, uint64_t *request_id) {
+  if (!s || !blocks || !request_id || n_blocks == 0 || (!domain && domain_len)) return FLOCK_E_ARG;
+  flock::Request r;
+  r.n = n_blocks;
+  r.log2 = flock::blocks_log2(n_blocks);
+  r.blocks.assign(blocks, blocks + n_blocks);
+  r.domain.assign(domain, domain + domain_len);
+  return submit_common(s, std::move(r), request_id);
+}
+
+extern "C" int flock_server_submit_seeded(flock_server *s, uint32_t log2, uint64_t seed, const uint8_t *domain,
+                                          size_t domain_len, uint64_t *request_id) {
+  if (!s || !request_id || log2 > 21 || (!domain && domain_len)) return FLOCK_E_ARG;
+  flock::Request r;
+  r.n = 1ull << log2;
+  r.log2 = flock::blocks_log2(r.n);
+  r.seeded = true;
+  r.seed = seed;
+  r.domain.assign(domain, domain + domain_len);
+  return submit_common(s, std::move(r), request_id);
+}
+
+extern "C" int flock_server_poll(flock_server *s, int wait_ms, uint64_t *request_id, uint8_t *out, size_t cap,
+                                 size_t *out_len, double *latency_s, double *queue_s) {
+  if (!s || !request_id) return FLOCK_E_ARG;
+  flock::Result r;
+  if (!s->s->poll(wait_ms, r)) return 0;
+  *request_id = r.id;
+  if (latency_s) *latency_s = r.t_done - r.t_submit;
+  if (queue_s) *queue_s = r.t_start - r.t_submit;
+  if (out_len) *out_len = r.proof.size();
+  if (r.rc) {
+    s->last_error = r.error;
+    return r.rc;
+  }
+  if (!out || cap < r.proof.size()) {
+    s->last_error = "output buffer too small; the result is dropped";
+    return FLOCK_E_CAP;
+  }
+  memcpy(out, r.proof.data(), r.proof.size());
+  return 1;
+}
+
+extern "C" int flock_server_pending(flock_server *s, uint64_t *n) {
+  if (!s || !n) return FLOCK_E_ARG;
+  *n = s->s->pending();
+  return FLOCK_OK;
+}

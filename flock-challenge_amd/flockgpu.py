@@ -90,6 +90,16 @@ def lib():
         L.flock_prove_blake3_seeded.argtypes = [P, U32, U64, P, SZ, P, SZ, PSZ]
         L.flock_generate_compressions_device.argtypes = [P, U32, U64, P]
         L.flock_last_proof.argtypes = [P, P, SZ, PSZ]
+        PD = ctypes.POINTER(ctypes.c_double)
+        PU64 = ctypes.POINTER(ctypes.c_uint64)
+        L.flock_server_create.argtypes = [ctypes.c_int, ctypes.c_int, U64, ctypes.POINTER(P)]
+        L.flock_server_destroy.argtypes = [P]
+        L.flock_server_last_error.restype = ctypes.c_char_p
+        L.flock_server_last_error.argtypes = [P]
+        L.flock_server_submit.argtypes = [P, P, U64, P, SZ, PU64]
+        L.flock_server_submit_seeded.argtypes = [P, U32, U64, P, SZ, PU64]
+        L.flock_server_poll.argtypes = [P, ctypes.c_int, PU64, P, SZ, PSZ, PD, PD]
+        L.flock_server_pending.argtypes = [P, PU64]
         _lib = L
     return _lib
 
@@ -100,7 +110,9 @@ def exported_symbols():
             "flock_timing_enable", "flock_timing_select", "flock_timing_report", "flock_device_count", "flock_device_alloc",
             "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_memcpy_d2d", "flock_synchronize", "flock_setup_create",
             "flock_setup_destroy", "flock_setup_info", "flock_prove_blake3", "flock_prove_blake3_device",
-            "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof"]
+            "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof",
+            "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
+            "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending"]
 
 
 def device_count():
@@ -295,3 +307,66 @@ def generate_compressions_device(ctx, log2, seed):
     buf = ctx.alloc((1 << log2) * COMPRESSION_DTYPE.itemsize)
     ctx._check(lib().flock_generate_compressions_device(ctx.handle, log2, seed, ctypes.c_void_p(buf.ptr)))
     return buf
+
+
+class ProofServer:
+    """Continuous batching of independent proofs on one GPU (server.cpp):
+    `n_workers` proofs in flight, each on its own stream and buffers,
+    admitted by HBM footprint."""
+
+    def __init__(self, device=0, n_workers=3, hbm_budget_bytes=0):
+        self.handle = ctypes.c_void_p()
+        rc = lib().flock_server_create(device, n_workers, hbm_budget_bytes, ctypes.byref(self.handle))
+        if rc != FLOCK_OK:
+            raise FlockError(rc, "flock_server_create failed")
+        self._buf = np.empty(PROOF_CAP, np.uint8)
+
+    def _err(self, rc):
+        raise FlockError(rc, lib().flock_server_last_error(self.handle).decode())
+
+    def submit(self, blocks, domain=BENCH_DOMAIN):
+        b = np.ascontiguousarray(blocks, dtype=COMPRESSION_DTYPE)
+        d = bytes(domain)
+        rid = ctypes.c_uint64()
+        rc = lib().flock_server_submit(self.handle, b.ctypes.data, b.size, d, len(d), ctypes.byref(rid))
+        if rc != FLOCK_OK:
+            self._err(rc)
+        return rid.value
+
+    def submit_seeded(self, log2, seed, domain=BENCH_DOMAIN):
+        d = bytes(domain)
+        rid = ctypes.c_uint64()
+        rc = lib().flock_server_submit_seeded(self.handle, log2, seed, d, len(d), ctypes.byref(rid))
+        if rc != FLOCK_OK:
+            self._err(rc)
+        return rid.value
+
+    def poll(self, wait_ms=-1):
+        """(request_id, proof_bytes, latency_s, queue_s) or None on timeout."""
+        rid = ctypes.c_uint64()
+        n = ctypes.c_size_t()
+        lat = ctypes.c_double()
+        qs = ctypes.c_double()
+        rc = lib().flock_server_poll(self.handle, wait_ms, ctypes.byref(rid), self._buf.ctypes.data, self._buf.nbytes,
+                                     ctypes.byref(n), ctypes.byref(lat), ctypes.byref(qs))
+        if rc == 0:
+            return None
+        if rc < 0:
+            self._err(rc)
+        return rid.value, self._buf[:n.value].tobytes(), lat.value, qs.value
+
+    def pending(self):
+        n = ctypes.c_uint64()
+        lib().flock_server_pending(self.handle, ctypes.byref(n))
+        return n.value
+
+    def close(self):
+        if self.handle:
+            lib().flock_server_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

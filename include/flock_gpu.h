@@ -130,6 +130,34 @@ int flock_generate_compressions_device(flock_ctx *ctx, uint32_t log2, uint64_t s
 /* Copy the bytes of the context's last proof (kept after FLOCK_E_CAP). */
 int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len);
 
+/* ------------------------------------------------------------------ serving */
+
+/* A proof server on GPU `device`: continuous batching of independent proofs.
+ * No counterpart in the reference (its worker proves one seed per process,
+ * worker/src/main.rs:36-45); each proof is the same bundle bytes as
+ * flock_prove_blake3 for the same input, whatever else is in flight.
+ * n_workers proofs run concurrently, each on its own HIP stream and buffers;
+ * a request is admitted (FIFO) when a worker is free and the HBM footprint of
+ * the admitted work stays within hbm_budget_bytes (0: 80% of free memory). */
+typedef struct flock_server flock_server;
+int flock_server_create(int device, int n_workers, uint64_t hbm_budget_bytes, flock_server **out);
+void flock_server_destroy(flock_server *server);
+const char *flock_server_last_error(const flock_server *server);
+/* Enqueue a proof of host compressions (copied) or of
+ * generate_compressions(log2, seed); *request_id identifies the result. */
+int flock_server_submit(flock_server *server, const flock_compression *blocks, uint64_t n_blocks,
+                        const uint8_t *domain, size_t domain_len, uint64_t *request_id);
+int flock_server_submit_seeded(flock_server *server, uint32_t log2, uint64_t seed, const uint8_t *domain,
+                               size_t domain_len, uint64_t *request_id);
+/* Take one finished result, waiting up to wait_ms (-1: forever, 0: no wait).
+ * Returns 1 with the proof bytes copied out, 0 if none finished in time, or a
+ * negative error code (the request failed, or cap < *out_len).  latency_s is
+ * submit -> bytes ready, queue_s submit -> start. */
+int flock_server_poll(flock_server *server, int wait_ms, uint64_t *request_id, uint8_t *out, size_t cap,
+                      size_t *out_len, double *latency_s, double *queue_s);
+/* Requests queued or running. */
+int flock_server_pending(flock_server *server, uint64_t *n);
+
 /* Device phase timing (HIP events on the launching stream), the counterpart
  * of the reference's FLOCK_PHASE_TIMING / prove_fast_timed
  * (crates/flock-prover/src/prover.rs:1267-1284).  Enabling resets the totals.
