@@ -264,9 +264,9 @@ def bench_batched(args):
     for _ in range(64):
         log2 = 10 + next(g) % 9
         reqs.append((0.0, log2, next(g)))
-    server = flockgpu.ProofServer(0, n_workers=args.workers)
+    server = flockgpu.ProofServer(0, n_workers=args.workers, reserve_log2=18)
     try:
-        run_requests(server, [(0.0, 10, 1), (0.0, 18, 2)])  # warm each worker's buffers
+        run_requests(server, [(0.0, 10, 1 + w) for w in range(args.workers)])  # workers up and warm
         out, span = run_requests(server, reqs)
     finally:
         server.close()
@@ -287,9 +287,9 @@ def bench_served(args):
     for _ in range(args.requests):
         t += rng.expovariate(args.rate)
         reqs.append((t, 12 + next(g) % 7, next(g)))
-    server = flockgpu.ProofServer(0, n_workers=args.workers)
+    server = flockgpu.ProofServer(0, n_workers=args.workers, reserve_log2=18)
     try:
-        run_requests(server, [(0.0, 12, 1), (0.0, 18, 2), (0.0, 18, 3)])  # warm the workers
+        run_requests(server, [(0.0, 12, 1 + w) for w in range(args.workers)])  # workers up and warm
         out, span = run_requests(server, reqs)
     finally:
         server.close()

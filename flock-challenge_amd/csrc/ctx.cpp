@@ -70,7 +70,7 @@ void *Ctx::get_scratch(const std::string &name, size_t bytes) {
   auto &b = scratch[name];
   if (b.bytes >= bytes && b.ptr) return b.ptr;
   if (b.ptr) {
-    hipDeviceSynchronize();  // queued kernels may still use the old buffer
+    hipStreamSynchronize(stream);  // this context's queued kernels may still use the old buffer
     hipFree(b.ptr);
   }
   b.ptr = nullptr;

@@ -11,6 +11,8 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "host_blake3.h"
 
@@ -229,6 +231,16 @@ void statement_digest(int m, uint8_t out[32]) {
   memcpy(d.data(), out, 32);
   std::lock_guard<std::mutex> lk(mu);
   cache[m] = d;
+}
+
+void precompute_digests(int m_lo, int m_hi) {
+  std::vector<std::thread> ts;
+  for (int m = m_lo; m <= m_hi; ++m)
+    ts.emplace_back([m] {
+      uint8_t d[32];
+      statement_digest(m, d);
+    });
+  for (auto &t : ts) t.join();
 }
 
 }  // namespace r1cs

@@ -31,6 +31,9 @@ struct Matrices {
 const Matrices &matrices();
 // BLAKE3 digest of the instance for 2^(m - 14) blocks.
 void statement_digest(int m, uint8_t out[32]);
+// Digests for every m in [m_lo, m_hi], computed in parallel threads (setup
+// work for a server that proves many sizes).
+void precompute_digests(int m_lo, int m_hi);
 // Message word read by G g of round r as operand 0 (mx) or 1 (my).
 int msg_index(int r, int g, int which);
 

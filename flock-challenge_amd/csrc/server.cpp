@@ -72,9 +72,10 @@ struct Result {
 
 class Server {
  public:
-  Server(int device, int n_workers, uint64_t budget) : device_(device), budget_(budget) {
-    for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this, i] { worker_main(i); });
+  Server(int device, int n_workers, uint64_t budget, int reserve_log2)
+      : device_(device), budget_(budget), reserve_log2_(reserve_log2) {
     reserved_.assign(n_workers, 0);
+    for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this, i] { worker_main(i); });
   }
   ~Server() {
     {
@@ -131,6 +132,20 @@ class Server {
     ctx.device = device_;
     hipSetDevice(device_);
     const bool stream_ok = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking) == hipSuccess;
+    if (stream_ok && reserve_log2_ > 0) {
+      // Size every device buffer once for the largest request (one proof of
+      // generated compressions), so serving never reallocates.
+      Request warm;
+      warm.log2 = reserve_log2_;
+      warm.n = 1ull << reserve_log2_;
+      warm.seeded = true;
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        reserved_[w] = prove_footprint_bytes(warm.log2);
+      }
+      std::vector<uint8_t> scratch_proof;
+      run(ctx, warm, scratch_proof);
+    }
     for (;;) {
       Request req;
       {
@@ -177,6 +192,7 @@ class Server {
 
   int device_;
   uint64_t budget_;
+  int reserve_log2_;
   std::vector<std::thread> workers_;
   std::vector<uint64_t> reserved_;
   std::mutex mu_;
@@ -203,8 +219,9 @@ static bool lig_config_exists(int n_log) {
   return false;
 }
 
-extern "C" int flock_server_create(int device, int n_workers, uint64_t hbm_budget_bytes, flock_server **out) {
-  if (!out || n_workers < 1 || n_workers > 16) return FLOCK_E_ARG;
+extern "C" int flock_server_create(int device, int n_workers, uint64_t hbm_budget_bytes, uint32_t reserve_log2,
+                                   flock_server **out) {
+  if (!out || n_workers < 1 || n_workers > 16 || reserve_log2 > 21) return FLOCK_E_ARG;
   *out = nullptr;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return FLOCK_E_DEVICE;
@@ -215,9 +232,17 @@ extern "C" int flock_server_create(int device, int n_workers, uint64_t hbm_budge
     hbm_budget_bytes = free_b / 10 * 8;
   }
   flock::r1cs::matrices();  // build once before the workers start
+  // statement digests of every supported size (setup work, as in the reference)
+  flock::r1cs::precompute_digests(flock::r1cs::K_LOG + 8, flock::r1cs::K_LOG + 21);
   flock_server *s = new (std::nothrow) flock_server();
   if (!s) return FLOCK_E_OOM;
-  s->s = new (std::nothrow) flock::Server(device, n_workers, hbm_budget_bytes);
+  if (reserve_log2 && (uint64_t)n_workers * flock::prove_footprint_bytes(flock::blocks_log2(1ull << reserve_log2)) >
+                          hbm_budget_bytes) {
+    delete s;
+    return FLOCK_E_OOM;
+  }
+  s->s = new (std::nothrow) flock::Server(device, n_workers, hbm_budget_bytes,
+                                          reserve_log2 ? flock::blocks_log2(1ull << reserve_log2) : 0);
   if (!s->s) {
     delete s;
     return FLOCK_E_OOM;

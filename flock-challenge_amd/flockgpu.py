@@ -92,7 +92,7 @@ def lib():
         L.flock_last_proof.argtypes = [P, P, SZ, PSZ]
         PD = ctypes.POINTER(ctypes.c_double)
         PU64 = ctypes.POINTER(ctypes.c_uint64)
-        L.flock_server_create.argtypes = [ctypes.c_int, ctypes.c_int, U64, ctypes.POINTER(P)]
+        L.flock_server_create.argtypes = [ctypes.c_int, ctypes.c_int, U64, U32, ctypes.POINTER(P)]
         L.flock_server_destroy.argtypes = [P]
         L.flock_server_last_error.restype = ctypes.c_char_p
         L.flock_server_last_error.argtypes = [P]
@@ -314,9 +314,9 @@ class ProofServer:
     `n_workers` proofs in flight, each on its own stream and buffers,
     admitted by HBM footprint."""
 
-    def __init__(self, device=0, n_workers=3, hbm_budget_bytes=0):
+    def __init__(self, device=0, n_workers=3, hbm_budget_bytes=0, reserve_log2=0):
         self.handle = ctypes.c_void_p()
-        rc = lib().flock_server_create(device, n_workers, hbm_budget_bytes, ctypes.byref(self.handle))
+        rc = lib().flock_server_create(device, n_workers, hbm_budget_bytes, reserve_log2, ctypes.byref(self.handle))
         if rc != FLOCK_OK:
             raise FlockError(rc, "flock_server_create failed")
         self._buf = np.empty(PROOF_CAP, np.uint8)

@@ -138,9 +138,12 @@ int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len);
  * flock_prove_blake3 for the same input, whatever else is in flight.
  * n_workers proofs run concurrently, each on its own HIP stream and buffers;
  * a request is admitted (FIFO) when a worker is free and the HBM footprint of
- * the admitted work stays within hbm_budget_bytes (0: 80% of free memory). */
+ * the admitted work stays within hbm_budget_bytes (0: 80% of free memory).
+ * reserve_log2 > 0 sizes each worker's buffers for 2^reserve_log2
+ * compressions at start (one throwaway proof), so serving never reallocates. */
 typedef struct flock_server flock_server;
-int flock_server_create(int device, int n_workers, uint64_t hbm_budget_bytes, flock_server **out);
+int flock_server_create(int device, int n_workers, uint64_t hbm_budget_bytes, uint32_t reserve_log2,
+                        flock_server **out);
 void flock_server_destroy(flock_server *server);
 const char *flock_server_last_error(const flock_server *server);
 /* Enqueue a proof of host compressions (copied) or of
