@@ -188,24 +188,54 @@ __global__ __launch_bounds__(256) void merkle_leaf_kernel(const uint4 *__restric
   out[2 * leaf + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
-// Parent level: out[i] = compress(IV, in[2i] || in[2i+1], 0, 64, PARENT)[0..8]
-// (merkle.rs:207-210).  Each workgroup folds up to `levels` levels through LDS.
-__global__ __launch_bounds__(256) void merkle_parent_kernel(const uint4 *__restrict__ in, uint64_t n_out,
-                                                            uint4 *__restrict__ out) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_out) return;
-  uint32_t m[16];
+// Up to 9 parent levels per launch: workgroup w takes input nodes
+// [w * 2^L, (w + 1) * 2^L) of one level and writes the L levels above them
+// (every level is kept: multiproofs read them), staging CVs in LDS.
+__global__ __launch_bounds__(256) void merkle_subtree_kernel(uint4 *__restrict__ tree, uint64_t in_off,
+                                                             uint64_t n_in, int L) {
+  __shared__ uint32_t cvs[256][8];
+  const uint32_t t = threadIdx.x;
+  const uint64_t w = blockIdx.x;
+  uint64_t off = in_off, n = n_in;  // current level's offset and node count
+  uint32_t width = 1u << L;         // nodes of the current level in this workgroup
+  for (int lev = 0; lev < L; ++lev) {
+    const uint32_t outs = width >> 1;
+    const uint64_t out_off = off + n;
+    const bool active = t < outs;
+    uint32_t cv[8];
+    if (active) {
+      uint32_t m[16];
+      if (lev == 0) {
+        const uint4 *src = tree + 2 * (off + w * width + 2 * t);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    uint4 w = in[4 * i + q];
-    m[4 * q] = w.x; m[4 * q + 1] = w.y; m[4 * q + 2] = w.z; m[4 * q + 3] = w.w;
+        for (int q = 0; q < 4; ++q) {
+          const uint4 v = src[q];
+          m[4 * q] = v.x; m[4 * q + 1] = v.y; m[4 * q + 2] = v.z; m[4 * q + 3] = v.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          m[i] = cvs[2 * t][i];
+          m[8 + i] = cvs[2 * t + 1][i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) cv[i] = b3::IV[i];
+      b3::compress_cv(cv, m, 0, 64, b3::PARENT);
+      const uint64_t node = out_off + w * outs + t;
+      tree[2 * node] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+      tree[2 * node + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    }
+    __syncthreads();  // every read of this level's inputs from cvs is done
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) cvs[t][i] = cv[i];
+    }
+    __syncthreads();  // the next level's inputs are in cvs
+    off = out_off;
+    n >>= 1;
+    width = outs;
   }
-  uint32_t cv[8];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) cv[t] = b3::IV[t];
-  b3::compress_cv(cv, m, 0, 64, b3::PARENT);
-  out[2 * i] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
-  out[2 * i + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
 int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len_f128, void *out,
@@ -223,12 +253,19 @@ static bool ntt_pass_is_shared(int k, int l, int F) { return F == 4 && k - l - F
 // src_pos_mask (shared passes only).
 static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k, int l, int F,
                     const f128 *src = nullptr, uint64_t src_pos_mask = ~0ull) {
-  constexpr int SW = 8;  // wavefronts sharing one set of tables
+  // 16 wavefronts share one set of tables when the block has that many
+  // groups (s >= 4), else 8.
   if (ntt_pass_is_shared(k, l, F)) {
     ScopedPhase ph(ctx, "ntt_pass_shared_kernel", st);
     const uint64_t groups = 1ull << (k - F);
-    hipLaunchKernelGGL((ntt_pass_shared_kernel<4, SW>), dim3((unsigned)(groups / SW)), dim3(64 * SW), 0, st, data,
-                       tw, k, l, src ? src : data, src ? src_pos_mask : ~0ull);
+    const f128 *in = src ? src : data;
+    const uint64_t mask = src ? src_pos_mask : ~0ull;
+    if (k - l - F >= 4)
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 16>), dim3((unsigned)(groups / 16)), dim3(1024), 0, st, data, tw,
+                         k, l, in, mask);
+    else
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 8>), dim3((unsigned)(groups / 8)), dim3(512), 0, st, data, tw, k,
+                         l, in, mask);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
@@ -286,12 +323,15 @@ int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t le
   ScopedPhase ph(ctx, "merkle_parents", st);
   uint64_t rs = 0, rl = n_leaves;
   while (rl > 1) {
-    uint64_t nl = rl >> 1;
-    hipLaunchKernelGGL(merkle_parent_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st,
-                       (const uint4 *)(tree + 2 * rs), nl, tree + 2 * (rs + rl));
+    int L = 0;
+    while (L < 9 && (2ull << L) <= rl) ++L;  // 2^L <= rl, at most 9 levels
+    const uint64_t wgs = rl >> L;
+    hipLaunchKernelGGL(merkle_subtree_kernel, dim3((unsigned)wgs), dim3(256), 0, st, tree, rs, rl, L);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
-    rs += rl;
-    rl = nl;
+    for (int i = 0; i < L; ++i) {
+      rs += rl;
+      rl >>= 1;
+    }
   }
   return FLOCK_OK;
 }

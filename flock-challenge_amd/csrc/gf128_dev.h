@@ -138,15 +138,8 @@ __device__ __forceinline__ f128 f128_reduce256(uint32_t l0, uint32_t l1, uint32_
   return r;
 }
 
-// General product a * b with both operands varying per lane (comb method).
-// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS (entry e at
-// tab[e * 64 + lane], so a wave's reads of one entry index are contiguous);
-// then for nibble position j = 7..0 of the four 32-bit words b_i of b, the
-// entries T[nibble_j(b_i)] are XORed into a 256-bit accumulator at dword
-// offset i, and the accumulator shifts by 4 between positions: 32 lookups,
-// 7 wide shifts and one final reduction.  No cross-lane traffic: a lane only
-// reads back what it wrote, so no barrier is needed.
-__device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab, uint32_t lane) {
+// The lane's table T[v] = v(x) * a (entry e at tab[e * 64 + lane]).
+__device__ __forceinline__ void f128_mul_build(f128 a, f128 *__restrict__ tab, uint32_t lane) {
   f128 t1 = a, t2 = f128_mulx(a), t4 = f128_mulx(t2), t8 = f128_mulx(t4);
   f128 t3 = f128_xor(t1, t2), t5 = f128_xor(t4, t1), t6 = f128_xor(t4, t2), t7 = f128_xor(t4, t3);
   f128 *T = tab + lane;
@@ -166,6 +159,11 @@ __device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab,
   T[832] = f128_xor(t8, t5);
   T[896] = f128_xor(t8, t6);
   T[960] = f128_xor(t8, t7);
+}
+
+// a * b with the lane's table of a already built (f128_mul_build).
+__device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ tab, uint32_t lane) {
+  const f128 *T = tab + lane;
   uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0, d6 = 0, d7 = 0;
 #pragma unroll
   for (int j = 7; j >= 0; --j) {
@@ -192,6 +190,19 @@ __device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab,
     }
   }
   return f128_reduce256(d0, d1, d2, d3, d4, d5, d6, d7);
+}
+
+// General product a * b with both operands varying per lane (comb method).
+// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS (entry e at
+// tab[e * 64 + lane], so a wave's reads of one entry index are contiguous);
+// then for nibble position j = 7..0 of the four 32-bit words b_i of b, the
+// entries T[nibble_j(b_i)] are XORed into a 256-bit accumulator at dword
+// offset i, and the accumulator shifts by 4 between positions: 32 lookups,
+// 7 wide shifts and one final reduction.  No cross-lane traffic: a lane only
+// reads back what it wrote, so no barrier is needed.
+__device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab, uint32_t lane) {
+  f128_mul_build(a, tab, lane);
+  return f128_mul_comb(b, tab, lane);
 }
 
 // LDS bytes a wavefront needs for f128_mul scratch.

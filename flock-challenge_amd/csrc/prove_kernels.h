@@ -47,6 +47,9 @@ int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, 
 int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
                    const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb);
 int rs_basis_device(Ctx *ctx, hipStream_t st, const void *e0, const void *e1, const void *tables, uint64_t n, void *out);
+// Whole forward (layers l0..k-1) or transposed (k-1..l0) NTT over 2^k
+// positions x 2^log_nl lanes, up to 4 layers per launch.
+int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l0, int log_nl, int transposed);
 int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed);
 int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
                     void *d_best);

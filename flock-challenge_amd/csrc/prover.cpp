@@ -298,21 +298,29 @@ struct Prover {
     return FLOCK_OK;
   }
 
-  // eq table over n variables into device buffer `out` (LSB-first).
+  // eq table over n variables into device buffer `out` (LSB-first):
+  // eq(r, x) = lo(x_0..) * mid(..) * hi(..) with host tables of at most 2^8,
+  // 2^8 and 2^(n-16) entries, expanded on the device in two steps.
   int eq_device(const F128 *r, int n, void *out) {
-    int k = n < 13 ? n : 13;
-    std::vector<F128> lo, hi;
+    const int a = std::min(n, 8), b = std::min(n - a, 8), c = n - a - b;
+    std::vector<F128> lo, mid, hi;
     {
       HostPhase hp(ctx, "eq_tables");
-      lo = eq_table(r, k);
-      hi = eq_table(r + k, n - k);
+      lo = eq_table(r, a);
+      mid = eq_table(r + a, b);
+      if (c) hi = eq_table(r + a + b, c);
     }
-    void *dlo = dev("eq.lo", lo.size() * 16), *dhi = dev("eq.hi", hi.size() * 16);
-    if (!dlo || !dhi) return FLOCK_E_OOM;
+    void *dlo = dev("eq.lo", lo.size() * 16), *dmid = dev("eq.mid", mid.size() * 16);
+    if (!dlo || !dmid) return FLOCK_E_OOM;
     int rc = h2d(dlo, lo.data(), lo.size() * 16);
-    if (!rc) rc = h2d(dhi, hi.data(), hi.size() * 16);
-    if (!rc) rc = eq_expand_device(ctx, st, dlo, k, dhi, n - k, out);
-    return rc;
+    if (!rc) rc = h2d(dmid, mid.data(), mid.size() * 16);
+    if (rc) return rc;
+    if (!c) return eq_expand_device(ctx, st, dlo, a, dmid, b, out);
+    void *dt = dev("eq.lomid", (16ull << (a + b))), *dhi = dev("eq.hi", hi.size() * 16);
+    if (!dt || !dhi) return FLOCK_E_OOM;
+    if ((rc = h2d(dhi, hi.data(), hi.size() * 16))) return rc;
+    if ((rc = eq_expand_device(ctx, st, dlo, a, dmid, b, dt))) return rc;
+    return eq_expand_device(ctx, st, dt, a + b, dhi, c, out);
   }
 
   int grind(Transcript &t, uint32_t bits, uint64_t *nonce_out) {
@@ -358,8 +366,7 @@ struct Prover {
     if (rc) return rc;
     const F128 *tw = ctx->device_twiddles(w.dim);
     if (!tw) return FLOCK_E_DEVICE;
-    for (int l = rate; l < w.dim; ++l)
-      if ((rc = ntt_layer_device(ctx, st, w.cw, tw, w.dim, l, k, 0))) return rc;
+    if ((rc = ntt_narrow_device(ctx, st, w.cw, tw, w.dim, rate, k, 0))) return rc;
     if ((rc = merkle_tree_device(ctx, st, w.cw, (uint32_t)(16 * w.lanes), w.block_len, w.tree))) return rc;
     return d2h(w.root, (const uint8_t *)w.tree + 32 * (2 * w.block_len - 2), 32);
   }
@@ -404,7 +411,7 @@ struct Prover {
     if (!rc) rc = scatter_device(ctx, st, sp, didx, dval, (uint32_t)q.size());
     const F128 *tw = ctx->device_twiddles(w.dim);
     if (!tw) return FLOCK_E_DEVICE;
-    for (int l = w.dim - 1; !rc && l >= w.rate; --l) rc = ntt_layer_device(ctx, st, sp, tw, w.dim, l, 0, 1);
+    if (!rc) rc = ntt_narrow_device(ctx, st, sp, tw, w.dim, w.rate, 0, 1);
     if (!rc) rc = fold_replicas_device(ctx, st, sp, (uint64_t)1 << n_cols, 1u << w.rate, basis);
     return rc;
   }

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "../../include/flock_gpu.h"
 #include "blake3_dev.h"
 #include "ctx.h"
@@ -99,8 +101,9 @@ __global__ __launch_bounds__(MUL_THREADS) void zc_round_kernel(const f128 *__res
     }
     const f128 p1 = f128_mul(a1, b1, tab, lane);
     const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, lane);
-    f128_xor_into(g1, f128_mul(e, p1, tab, lane));
-    f128_xor_into(gi, f128_mul(e, pi, tab, lane));
+    f128_mul_build(e, tab, lane);  // one table of E for both weighted sums
+    f128_xor_into(g1, f128_mul_comb(p1, tab, lane));
+    f128_xor_into(gi, f128_mul_comb(pi, tab, lane));
   }
   wg_xor_reduce(g1, red, &partial[2 * blockIdx.x]);
   wg_xor_reduce(gi, red, &partial[2 * blockIdx.x + 1]);
@@ -194,8 +197,9 @@ __global__ __launch_bounds__(ZR0_THREADS) void zc_round0_kernel(const uint64_t *
     }
     const f128 p1 = f128_mul(a1, b1, tab, lane);
     const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, lane);
-    f128_xor_into(g1, f128_mul(e, p1, tab, lane));
-    f128_xor_into(gi, f128_mul(e, pi, tab, lane));
+    f128_mul_build(e, tab, lane);  // one table of E for both weighted sums
+    f128_xor_into(g1, f128_mul_comb(p1, tab, lane));
+    f128_xor_into(gi, f128_mul_comb(pi, tab, lane));
   }
   __syncthreads();  // T is reused as the reduction buffer
   wg_xor_reduce(g1, T, &partial[2 * blockIdx.x]);
@@ -445,6 +449,99 @@ int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, i
   hipLaunchKernelGGL(ntt_layer_kernel, dim3(grid_for(total, MUL_THREADS, 8192)), dim3(MUL_THREADS), 0, st, (f128 *)d,
                      (const f128 *)tw, k, l, log_nl, transposed);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+// Several layers per launch for narrow codewords (2^log_nl interleaved lanes,
+// log_nl small): thread = (group, lane) holds the 2^F positions of its group
+// (the positions that exchange data in layers l .. l+F-1) in registers.
+// Forward applies layers l .. l+F-1; transposed applies their transposes in
+// reverse order (l+F-1 .. l), which is the next F steps of the transposed
+// transform.  Twiddle products use the per-lane general product.
+template <int F, bool TRANSPOSED>
+__global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__restrict__ d,
+                                                                       const f128 *__restrict__ tw, int k, int l,
+                                                                       int log_nl, uint64_t n_threads) {
+  __shared__ f128 mt[MUL_WAVES][1024];
+  const uint32_t lane = threadIdx.x & 63;
+  f128 *tab = mt[threadIdx.x >> 6];
+  const uint64_t t = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x;
+  if (t >= n_threads) return;
+  const uint64_t ln = t & ((1ull << log_nl) - 1), g = t >> log_nl;
+  const int s = k - l - F;
+  const uint64_t r = g & ((1ull << s) - 1), B = g >> s;
+  const uint64_t base = (B << (k - l)) | r;
+  f128 v[1 << F];
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) v[e] = d[((base | ((uint64_t)e << s)) << log_nl) + ln];
+  if (TRANSPOSED) {
+    // The induced-basis transform starts from a vector with a few hundred
+    // nonzeros: an all-zero group maps to zeros (the map is linear).
+    uint32_t any = 0;
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e) any |= v[e].x | v[e].y | v[e].z | v[e].w;
+    if (!any) return;
+  }
+#pragma unroll
+  for (int jj = 0; jj < F; ++jj) {
+    const int j = TRANSPOSED ? F - 1 - jj : jj;
+    const uint64_t tw_base = (1ull << (l + j)) - 1ull + (B << j);
+#pragma unroll
+    for (int c = 0; c < (1 << j); ++c) {
+      f128_mul_build(tw[tw_base + c], tab, lane);  // one table per twiddle
+#pragma unroll
+      for (int el = 0; el < (1 << (F - 1 - j)); ++el) {
+        const int e0 = (c << (F - j)) | el, e1 = e0 | (1 << (F - 1 - j));
+        if (!TRANSPOSED) {
+          const f128 u = f128_xor(v[e0], f128_mul_comb(v[e1], tab, lane));
+          v[e0] = u;
+          v[e1] = f128_xor(v[e1], u);
+        } else {
+          const f128 sum = f128_xor(v[e0], v[e1]);
+          v[e0] = sum;
+          v[e1] = f128_xor(f128_mul_comb(sum, tab, lane), v[e1]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) d[((base | ((uint64_t)e << s)) << log_nl) + ln] = v[e];
+}
+
+// Forward (from layer l0 up to k-1) or transposed (from k-1 down to l0)
+// transform over 2^k positions x 2^log_nl lanes, up to 4 layers per launch.
+int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l0, int log_nl, int transposed) {
+  ScopedPhase ph(ctx, "ntt_narrow", st);
+  int done = 0;
+  const int total = k - l0;
+  // Layers per launch: each thread runs F layers of serial products, so keep
+  // at least 2^18 threads in flight (fewer layers per launch when small).
+  const int f_par = std::max(1, std::min(4, k + log_nl - 18));
+  while (done < total) {
+    const int F = std::min(f_par, total - done);
+    // forward passes run l0, l0+4, ...; transposed ones run the same groups from the top
+    const int l = transposed ? k - done - F : l0 + done;
+    const uint64_t n_threads = 1ull << (k - F + log_nl);
+    const dim3 grid((unsigned)((n_threads + MUL_THREADS - 1) / MUL_THREADS)), block(MUL_THREADS);
+    f128 *dd = (f128 *)d;
+    const f128 *t = (const f128 *)tw;
+#define NARROW_CASE(FF)                                                                                         \
+  case FF:                                                                                                     \
+    if (transposed)                                                                                            \
+      hipLaunchKernelGGL((ntt_pass_narrow_kernel<FF, true>), grid, block, 0, st, dd, t, k, l, log_nl, n_threads); \
+    else                                                                                                       \
+      hipLaunchKernelGGL((ntt_pass_narrow_kernel<FF, false>), grid, block, 0, st, dd, t, k, l, log_nl, n_threads); \
+    break;
+    switch (F) {
+      NARROW_CASE(1)
+      NARROW_CASE(2)
+      NARROW_CASE(3)
+      NARROW_CASE(4)
+    }
+#undef NARROW_CASE
+    FLOCK_HIP_CHECK(ctx, hipGetLastError());
+    done += F;
+  }
   return FLOCK_OK;
 }
 
