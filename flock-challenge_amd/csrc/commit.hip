@@ -79,6 +79,49 @@ __global__ __launch_bounds__(64) void ntt_pass_kernel(f128 *__restrict__ data, c
 // buffer being rebuilt was last read before the previous barrier.
 constexpr int SHARED_TWIDDLES = 15;  // 2^F - 1 for F = 4
 
+// Twiddle n = 2^J - 1 + C of a shared-table pass (layer offset J, block C):
+// build its tables into buffer n & 1, then every butterfly of it.  Template
+// recursion keeps every register index a compile-time constant.
+template <int F, int THREADS, int J, int C>
+__device__ __forceinline__ void shared_pass_step(f128 (&v)[1 << F], f128 *tabs, const f128 *basis,
+                                                 const FulltabLane &FL, uint32_t tid) {
+  constexpr int N = (1 << J) - 1 + C;
+  constexpr bool HI = N & 1;
+  const char *tbase = (const char *)tabs;  // buffer 0 at +0, buffer 1 at +64 KiB
+  build_fulltab<THREADS>(basis + N * 128, tabs + (HI ? 4096 : 0), tid);
+  __syncthreads();
+  constexpr int nb = 1 << (F - 1 - J);
+  if constexpr (nb >= 2 && THREADS <= 512) {
+    // two products in flight per lane (enough registers at 8 waves)
+#pragma unroll
+    for (int el = 0; el < nb; el += 2) {
+      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const int f0 = e0 + 1, f1 = e1 + 1;
+      f128 p0, p1;
+      f128_mul2_fulltab<HI>(v[e1], v[f1], tbase, FL, p0, p1);
+      const f128 u0 = f128_xor(v[e0], p0), u1 = f128_xor(v[f0], p1);
+      v[e0] = u0;
+      f128_xor_into(v[e1], u0);
+      v[f0] = u1;
+      f128_xor_into(v[f1], u1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll
+    for (int el = 0; el < nb; ++el) {
+      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const f128 u0 = f128_xor(v[e0], f128_mul_fulltab<HI>(v[e1], tbase, FL));
+      v[e0] = u0;
+      f128_xor_into(v[e1], u0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if constexpr (C + 1 < (1 << J))
+    shared_pass_step<F, THREADS, J, C + 1>(v, tabs, basis, FL, tid);
+  else if constexpr (J + 1 < F)
+    shared_pass_step<F, THREADS, J + 1, 0>(v, tabs, basis, FL, tid);
+}
+
 template <int F, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__restrict__ data,
                                                                      const f128 *__restrict__ tw, int k, int l,
@@ -112,39 +155,7 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__res
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_pos_mask) << 6) + lane];
   __syncthreads();  // basis visible
-  int buf = 0;
-#pragma unroll
-  for (int j = 0; j < F; ++j) {
-#pragma unroll
-    for (int c = 0; c < (1 << j); ++c) {
-      f128 *tab = tabs + buf * 4096;
-      build_fulltab<THREADS>(basis + ((1 << j) - 1 + c) * 128, tab, tid);
-      __syncthreads();
-      const int nb = 1 << (F - 1 - j);
-      if (nb >= 2) {
-#pragma unroll
-        for (int el = 0; el < nb; el += 2) {
-          const int e0 = (c << (F - j)) | el, e1 = e0 | (1 << (F - 1 - j));
-          const int f0 = e0 + 1, f1 = e1 + 1;
-          f128 p0, p1;
-          f128_mul2_fulltab(v[e1], v[f1], (const char *)tab, FL, rot, p0, p1);
-          const f128 u0 = f128_xor(v[e0], p0), u1 = f128_xor(v[f0], p1);
-          v[e0] = u0;
-          f128_xor_into(v[e1], u0);
-          v[f0] = u1;
-          f128_xor_into(v[f1], u1);
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      } else {
-        const int e0 = c << 1, e1 = e0 | 1;
-        const f128 u0 = f128_xor(v[e0], f128_mul_fulltab(v[e1], (const char *)tab, FL, rot));
-        v[e0] = u0;
-        f128_xor_into(v[e1], u0);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      buf ^= 1;
-    }
-  }
+  shared_pass_step<F, THREADS, 0, 0>(v, tabs, basis, FL, tid);
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
 }
@@ -260,12 +271,10 @@ static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k,
     const uint64_t groups = 1ull << (k - F);
     const f128 *in = src ? src : data;
     const uint64_t mask = src ? src_pos_mask : ~0ull;
-    if (k - l - F >= 4)
-      hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 16>), dim3((unsigned)(groups / 16)), dim3(1024), 0, st, data, tw,
-                         k, l, in, mask);
-    else
-      hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 8>), dim3((unsigned)(groups / 8)), dim3(512), 0, st, data, tw, k,
-                         l, in, mask);
+    // 8 waves: 169 VGPRs keep all 16 elements and two products in flight
+    // without spills (16 waves would cap a lane at 128)
+    hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 8>), dim3((unsigned)(groups / 8)), dim3(512), 0, st, data, tw, k,
+                       l, in, mask);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }

@@ -13,95 +13,97 @@
 namespace flock {
 
 // Conflict-free lookups.  The 16 byte tables are interleaved: entry (b, k)
-// = T_k[b] sits at f128 index b * 16 + k, so its 16-byte bank group is k mod
-// 16 (k mod 8 with 32 banks).  At step q lane L reads table k = q ^ (L mod
-// 16), so the 16 (or 8) lanes the LDS serves together always touch distinct
-// bank groups whatever the data.  The lane first permutes the bytes of v the
-// same way (byte q of the result = byte q ^ rot of v).
-__device__ __forceinline__ f128 f128_xperm_bytes(f128 v, uint32_t r) {
-  uint32_t w0 = v.x, w1 = v.y, w2 = v.z, w3 = v.w;
-  if (r & 8) {
-    uint32_t t = w0; w0 = w2; w2 = t;
-    t = w1; w1 = w3; w3 = t;
-  }
-  if (r & 4) {
-    uint32_t t = w0; w0 = w1; w1 = t;
-    t = w2; w2 = w3; w3 = t;
-  }
-  // within a dword: bit 1 swaps the halves, bit 0 swaps bytes inside halves
-  const uint32_t sel = (r & 3) == 0 ? 0x03020100u : (r & 3) == 1 ? 0x02030001u : (r & 3) == 2 ? 0x01000302u
-                                                                                            : 0x00010203u;
-  f128 o;
-  o.x = __builtin_amdgcn_perm(0, w0, sel);
-  o.y = __builtin_amdgcn_perm(0, w1, sel);
-  o.z = __builtin_amdgcn_perm(0, w2, sel);
-  o.w = __builtin_amdgcn_perm(0, w3, sel);
-  return o;
-}
-
-// Per-lane byte offsets of table k = (4 q + i) ^ rot inside one b row:
-// off(q, i) = ((4 q) ^ (rot & 12)) * 16 + (i ^ (rot & 3)) * 16.
+// = T_k[b] sits at byte offset b * 256 + k * 16, so its 16-byte bank group is
+// k mod 16.  At step (q, i) lane L reads table k = (4 q + i) ^ (L mod 16), so
+// the 16 lanes the LDS serves together always touch distinct bank groups
+// whatever the data.  The lane first permutes the bytes of v the same way
+// (byte 4 q + i of x = byte (4 q + i) ^ rot of v).
+//
+// Every lookup address is one v_perm: byte 0 = k * 16 from a per-lane offset
+// register, byte 1 = the data byte, byte 2 = 0 or 1 (a second table 64 KiB
+// above the first, for double buffering).
 struct FulltabLane {
-  uint32_t lo[4];  // (i ^ (rot & 3)) * 16
-  uint32_t hi;     // (rot & 12) * 16
+  uint32_t sel_hi;  // per-lane v_perm selector: dword q ^ 2 or q (rot bit 3)
+  uint32_t sel_lo;  // dword q ^ 1 or q (rot bit 2), bytes i ^ (rot & 3)
+  uint32_t o01, o23;  // offsets of q = 0: bytes [off(i), off(i + 1), 1, 0]
   __device__ __forceinline__ explicit FulltabLane(uint32_t rot) {
+    sel_hi = (rot & 8u) ? 0x07060504u : 0x03020100u;
+    const uint32_t src = (rot & 4u) ? 4u : 0u, r3 = rot & 3u;
+    sel_lo = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) lo[i] = (i ^ (rot & 3u)) << 4;
-    hi = (rot & 12u) << 4;
+    for (uint32_t j = 0; j < 4; ++j) sel_lo |= (src + (j ^ r3)) << (8 * j);
+    const uint32_t r = rot & 15u;
+    o01 = ((0u ^ r) << 4) | (((1u ^ r) << 4) << 8) | (1u << 16);
+    o23 = ((2u ^ r) << 4) | (((3u ^ r) << 4) << 8) | (1u << 16);
+  }
+  // byte 4 q + i of the result = byte (4 q + i) ^ rot of v
+  __device__ __forceinline__ f128 xperm(f128 v) const {
+    const uint32_t a0 = __builtin_amdgcn_perm(v.z, v.x, sel_hi), a1 = __builtin_amdgcn_perm(v.w, v.y, sel_hi);
+    const uint32_t a2 = __builtin_amdgcn_perm(v.x, v.z, sel_hi), a3 = __builtin_amdgcn_perm(v.y, v.w, sel_hi);
+    f128 o;
+    o.x = __builtin_amdgcn_perm(a1, a0, sel_lo);
+    o.y = __builtin_amdgcn_perm(a0, a1, sel_lo);
+    o.z = __builtin_amdgcn_perm(a3, a2, sel_lo);
+    o.w = __builtin_amdgcn_perm(a2, a3, sel_lo);
+    return o;
   }
 };
 
-// v * t from the interleaved tables of t (byte addresses: tab + (b << 8) + k * 16).
-__device__ __forceinline__ f128 f128_mul_fulltab(f128 v, const char *__restrict__ tab, const FulltabLane &L,
-                                                 uint32_t rot) {
-  const f128 x = f128_xperm_bytes(v, rot);
+// v_perm selector of lookup i: byte 0 = offset byte (i & 1) of the pair
+// register, byte 1 = data byte i, byte 2 = the pair register's 1 (HI) or 0.
+__device__ __forceinline__ constexpr uint32_t fulltab_sel(int i, bool hi) {
+  return (4u + (uint32_t)(i & 1)) | ((uint32_t)i << 8) | ((hi ? 6u : 12u) << 16) | (12u << 24);
+}
+
+__device__ __forceinline__ uint32_t f128_word(const f128 &x, int q) {
+  return q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
+}
+
+// v * t from the interleaved tables of t at base (+ 64 KiB when HI).  The
+// four steps are unrolled with scheduling barriers between them, so one
+// step's four lookups are in flight at a time and nothing is hoisted.
+template <bool HI = false>
+__device__ __forceinline__ f128 f128_mul_fulltab(f128 v, const char *__restrict__ base, const FulltabLane &L) {
+  const f128 x = L.xperm(v);
   f128 acc = f128_zero();
-  uint32_t a = x.x, b = x.y, c = x.z, d = x.w;
-  // A real loop (not unrolled) keeps one word's four lookups in flight at a
-  // time; unrolled, the compiler hoists every lookup of the pass and spills.
-#pragma unroll 1
+#pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const uint32_t qoff = L.hi ^ (64u * q);
+    const uint32_t w = f128_word(x, q);
+    const uint32_t oa = L.o01 ^ (0x4040u * q), ob = L.o23 ^ (0x4040u * q);
     f128 t[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) t[i] = *(const f128 *)(tab + ((((a >> (8 * i)) & 255u) << 8) | qoff | L.lo[i]));
+    for (int i = 0; i < 4; ++i)
+      t[i] = *(const f128 *)(base + __builtin_amdgcn_perm(i < 2 ? oa : ob, w, fulltab_sel(i, HI)));
     acc = f128_xor3(acc, t[0], t[1]);
     acc = f128_xor3(acc, t[2], t[3]);
-    a = b;
-    b = c;
-    c = d;
+    __builtin_amdgcn_sched_barrier(0);
   }
   return acc;
 }
 
 // Two independent products by the same table, interleaved for latency.
-__device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *__restrict__ tab,
-                                                  const FulltabLane &L, uint32_t rot, f128 &p0, f128 &p1) {
-  const f128 x0 = f128_xperm_bytes(v0, rot), x1 = f128_xperm_bytes(v1, rot);
+template <bool HI = false>
+__device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *__restrict__ base,
+                                                  const FulltabLane &L, f128 &p0, f128 &p1) {
+  const f128 x0 = L.xperm(v0), x1 = L.xperm(v1);
   p0 = f128_zero();
   p1 = f128_zero();
-  uint32_t a0 = x0.x, b0 = x0.y, c0 = x0.z, d0 = x0.w;
-  uint32_t a1 = x1.x, b1 = x1.y, c1 = x1.z, d1 = x1.w;
-#pragma unroll 1
+#pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const uint32_t qoff = L.hi ^ (64u * q);
+    const uint32_t w0 = f128_word(x0, q), w1 = f128_word(x1, q);
+    const uint32_t oa = L.o01 ^ (0x4040u * q), ob = L.o23 ^ (0x4040u * q);
     f128 t0[4], t1[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t o = qoff | L.lo[i];
-      t0[i] = *(const f128 *)(tab + ((((a0 >> (8 * i)) & 255u) << 8) | o));
-      t1[i] = *(const f128 *)(tab + ((((a1 >> (8 * i)) & 255u) << 8) | o));
+      const uint32_t o = i < 2 ? oa : ob;
+      t0[i] = *(const f128 *)(base + __builtin_amdgcn_perm(o, w0, fulltab_sel(i, HI)));
+      t1[i] = *(const f128 *)(base + __builtin_amdgcn_perm(o, w1, fulltab_sel(i, HI)));
     }
     p0 = f128_xor3(p0, t0[0], t0[1]);
     p1 = f128_xor3(p1, t1[0], t1[1]);
     p0 = f128_xor3(p0, t0[2], t0[3]);
     p1 = f128_xor3(p1, t1[2], t1[3]);
-    a0 = b0;
-    b0 = c0;
-    c0 = d0;
-    a1 = b1;
-    b1 = c1;
-    c1 = d1;
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -125,15 +127,20 @@ __device__ __forceinline__ f128 f128_mul_xpow(f128 t, uint32_t i) {
 // Full byte tables of one twiddle from its 128 basis values, E = 4096 /
 // THREADS entries per thread.  basis[i * 16 + k] = t x^(8k + i); thread
 // (k = tid & 15, u = tid >> 4) writes entries b = E u .. E u + E - 1 of
-// table k, so the lanes of one store hit distinct bank groups.
+// table k (byte offset b * 256 + k * 16 from tab), so the lanes of one store
+// hit distinct bank groups.  The E entries differ in their low bits only and
+// are formed in registers in Gray-code order.
 template <int THREADS>
 __device__ __forceinline__ void build_fulltab(const f128 *__restrict__ basis, f128 *__restrict__ tab, uint32_t tid) {
   constexpr int E = 4096 / THREADS;
   constexpr int LE = E == 4 ? 2 : E == 8 ? 3 : E == 16 ? 4 : -1;
   static_assert(LE > 0, "4, 8 or 16 entries per thread");
   const uint32_t kk = tid & 15, u = tid >> 4;
+  f128 low[LE];
+#pragma unroll
+  for (int i = 0; i < LE; ++i) low[i] = basis[i * 16 + kk];
   f128 h = f128_zero();
-#pragma unroll 1
+#pragma unroll
   for (int i = LE; i < 8; ++i) {
     const f128 bi = basis[i * 16 + kk];
     const uint32_t m = 0u - ((u >> (i - LE)) & 1u);
@@ -145,11 +152,11 @@ __device__ __forceinline__ void build_fulltab(const f128 *__restrict__ basis, f1
   f128 *dst = tab + (E * u) * 16 + kk;  // entry (b, kk) at b * 16 + kk
   dst[0] = h;
 #pragma unroll
-  for (int lo = 1; lo < E; ++lo) {
-    // entry lo = entry (lo with its lowest set bit cleared) + that bit's basis
-    const int bit = __builtin_ctz(lo);
-    h = f128_xor(dst[(lo & (lo - 1)) * 16], basis[bit * 16 + kk]);
-    dst[lo * 16] = h;
+  for (int g = 1; g < E; ++g) {
+    // Gray code: entry g ^ (g >> 1) differs from the previous in one bit
+    const int bit = __builtin_ctz(g);
+    h = f128_xor(h, low[bit]);
+    dst[(g ^ (g >> 1)) * 16] = h;
   }
 }
 

@@ -58,11 +58,11 @@ __global__ __launch_bounds__(FT_THREADS) void eq_expand_kernel(const f128 *__res
   uint64_t i = c0 + tid;
   for (; i + FT_THREADS < c0 + per_wg; i += 2 * FT_THREADS) {
     f128 p0, p1;
-    f128_mul2_fulltab(lo[i], lo[i + FT_THREADS], (const char *)tab, L, rot, p0, p1);
+    f128_mul2_fulltab(lo[i], lo[i + FT_THREADS], (const char *)tab, L, p0, p1);
     out[(h << k) + i] = p0;
     out[(h << k) + i + FT_THREADS] = p1;
   }
-  if (i < c0 + per_wg) out[(h << k) + i] = f128_mul_fulltab(lo[i], (const char *)tab, L, rot);
+  if (i < c0 + per_wg) out[(h << k) + i] = f128_mul_fulltab(lo[i], (const char *)tab, L);
 }
 
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out) {
@@ -264,11 +264,11 @@ __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__res
     if (B) {
       const f128 b0 = B[2 * x], b1 = B[2 * x + 1];
       f128 pa, pb;
-      f128_mul2_fulltab(f128_xor(a0, a1), f128_xor(b0, b1), (const char *)tab, L, rot, pa, pb);
+      f128_mul2_fulltab(f128_xor(a0, a1), f128_xor(b0, b1), (const char *)tab, L, pa, pb);
       A2[x] = f128_xor(a0, pa);
       B2[x] = f128_xor(b0, pb);
     } else {
-      A2[x] = f128_xor(a0, f128_mul_fulltab(f128_xor(a0, a1), (const char *)tab, L, rot));
+      A2[x] = f128_xor(a0, f128_mul_fulltab(f128_xor(a0, a1), (const char *)tab, L));
     }
   }
 }
