@@ -64,7 +64,7 @@ def algorithmic_bytes(name, m):
     cw = 2 * W
     return {
         "ntt_pass_shared_kernel": 2 * cw,   # read + write the codeword
-        "ntt_pass_kernel": 2 * cw,
+        "ntt_tail_kernel": 2 * cw,
         "witness": 3 * W + (1 << (m - 14)) * 112,  # write z, a, b; read the compressions
         "merkle_leaf_kernel": cw + 32 * (cw // 1024),
     }.get(name)
@@ -328,7 +328,8 @@ def bench_commit(args):
 
     # Dominant kernel: one NTT pass reads and writes the whole codeword once.
     cw_bytes = params.codeword_len_f128() * 16
-    n_launch, ntt_ms = rep.get("ntt_pass_kernel", (0, 0.0))
+    ntt_kernel = max(("ntt_pass_shared_kernel", "ntt_tail_kernel"), key=lambda n: rep.get(n, (0, 0.0))[1])
+    n_launch, ntt_ms = rep.get(ntt_kernel, (0, 0.0))
     avg_ntt_ms = ntt_ms / max(1, n_launch)
     achieved = 2 * cw_bytes / (avg_ntt_ms * 1e-3) / 1e9
 
@@ -350,7 +351,7 @@ def bench_commit(args):
                    "wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ntt_pass_kernel", "avg_launch_ms": avg_ntt_ms,
+                     "kernel": ntt_kernel, "avg_launch_ms": avg_ntt_ms,
                      "algorithmic_bytes_per_launch": 2 * cw_bytes},
     }
     if not args.no_cpu_baseline:

@@ -13,8 +13,10 @@
 // keeps its 2^F elements in VGPRs, so every butterfly is lane-local and the
 // global loads/stores are 1 KiB-contiguous per position (fully coalesced).
 // The twiddle of a butterfly depends only on (layer, block), i.e. it is
-// wavefront-uniform: its product runs through a 2 x 256-entry byte table of
-// that twiddle built cooperatively in LDS (gf128_dev.h).
+// wavefront-uniform: its products run through lookup tables of that twiddle
+// in LDS, full byte tables shared by a workgroup (fulltab.h) where the
+// twiddle meets many products, per-wavefront nibble tables (nibtab.h) in the
+// last layers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,14 +28,57 @@
 #include "ctx.h"
 #include "fulltab.h"
 #include "gf128_dev.h"
+#include "nibtab.h"
 
 namespace flock {
 
+// Tail pass: F layers whose twiddles are shared by too few groups to pay for
+// full byte tables (the last layers of the transform, where a twiddle meets
+// 64 .. 512 products).  One wavefront per group; it builds the 8 KiB nibble
+// tables of its own twiddles, two at a time, in 16 KiB of LDS, so ten
+// wavefronts fit on a CU.  Step (J, C) handles twiddles C, C + 1 of layer
+// offset J; template recursion keeps every register index constant.
+template <int F, int J, int C>
+__device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__restrict__ tw, int l, uint64_t B,
+                                               char *lds, const NibtabLane &NL, uint32_t lane) {
+  constexpr int NT = (1 << J) - C >= 2 ? 2 : 1;
+  constexpr int nb = 1 << (F - 1 - J);
+  const uint64_t tb = (1ull << (l + J)) - 1ull + (B << J);
+  __syncthreads();  // the previous step's tables are read
+  build_nibtab<0>(tw[tb + C], lds, lane);
+  if constexpr (NT == 2) build_nibtab<NIBTAB_BYTES>(tw[tb + C + 1], lds, lane);
+  __syncthreads();
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int el = 0; el < nb; ++el) {
+    const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+    const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<0>(v[e1], lds, NL));
+    v[e0] = u0;
+    f128_xor_into(v[e1], u0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (NT == 2) {
+#pragma unroll
+    for (int el = 0; el < nb; ++el) {
+      const int e0 = ((C + 1) << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<NIBTAB_BYTES>(v[e1], lds, NL));
+      v[e0] = u0;
+      f128_xor_into(v[e1], u0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if constexpr (C + NT < (1 << J))
+    tail_pass_step<F, J, C + NT>(v, tw, l, B, lds, NL, lane);
+  else if constexpr (J + 1 < F)
+    tail_pass_step<F, J + 1, 0>(v, tw, l, B, lds, NL, lane);
+}
+
 template <int F>
-__global__ __launch_bounds__(64) void ntt_pass_kernel(f128 *__restrict__ data, const f128 *__restrict__ tw,
-                                                      int k, int l) {
-  __shared__ f128 tab[512];
+__global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *__restrict__ data, const f128 *__restrict__ tw, int k,
+                                                      int l) {
+  __shared__ __align__(16) char lds[2 * NIBTAB_BYTES];
   const uint32_t lane = threadIdx.x;
+  const NibtabLane NL(lane & 15u);
   const int s = k - l - F;
   const uint64_t g = blockIdx.x;
   const uint64_t r = g & ((1ull << s) - 1ull);
@@ -42,29 +87,7 @@ __global__ __launch_bounds__(64) void ntt_pass_kernel(f128 *__restrict__ data, c
   f128 v[1 << F];
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) v[e] = data[((base | ((uint64_t)e << s)) << 6) + lane];
-
-#pragma unroll
-  for (int j = 0; j < F; ++j) {
-    const uint64_t tw_base = (1ull << (l + j)) - 1ull + (B << j);
-#pragma unroll
-    for (int c = 0; c < (1 << j); ++c) {
-      const f128 t = tw[tw_base + c];
-      __syncthreads();
-      f128_build_bytetab(t, tab, lane);
-      __syncthreads();
-#pragma unroll
-      for (int el = 0; el < (1 << (F - 1 - j)); ++el) {
-        const int e0 = (c << (F - j)) | el;
-        const int e1 = e0 | (1 << (F - 1 - j));
-        f128 u = f128_xor(v[e0], f128_mul_bytetab(v[e1], tab));
-        v[e0] = u;
-        f128_xor_into(v[e1], u);
-        // Keep one product's 16 table loads in flight at a time; letting the
-        // scheduler hoist loads across butterflies blows the VGPR budget.
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-  }
+  tail_pass_step<F, 0, 0>(v, tw, l, B, lds, NL, lane);
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
 }
@@ -278,15 +301,14 @@ static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k,
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
-  ScopedPhase ph(ctx, "ntt_pass_kernel", st);
+  ScopedPhase ph(ctx, "ntt_tail_kernel", st);
   uint64_t groups = 1ull << (k - F);
   dim3 grid((unsigned)groups), block(64);
   switch (F) {
-    case 1: hipLaunchKernelGGL(ntt_pass_kernel<1>, grid, block, 0, st, data, tw, k, l); break;
-    case 2: hipLaunchKernelGGL(ntt_pass_kernel<2>, grid, block, 0, st, data, tw, k, l); break;
-    case 3: hipLaunchKernelGGL(ntt_pass_kernel<3>, grid, block, 0, st, data, tw, k, l); break;
-    case 4: hipLaunchKernelGGL(ntt_pass_kernel<4>, grid, block, 0, st, data, tw, k, l); break;
-    case 5: hipLaunchKernelGGL(ntt_pass_kernel<5>, grid, block, 0, st, data, tw, k, l); break;
+    case 1: hipLaunchKernelGGL(ntt_tail_kernel<1>, grid, block, 0, st, data, tw, k, l); break;
+    case 2: hipLaunchKernelGGL(ntt_tail_kernel<2>, grid, block, 0, st, data, tw, k, l); break;
+    case 3: hipLaunchKernelGGL(ntt_tail_kernel<3>, grid, block, 0, st, data, tw, k, l); break;
+    case 4: hipLaunchKernelGGL(ntt_tail_kernel<4>, grid, block, 0, st, data, tw, k, l); break;
     default: ctx->last_error = "ntt_pass: unsupported F"; return FLOCK_E_ARG;
   }
   FLOCK_HIP_CHECK(ctx, hipGetLastError());

@@ -107,21 +107,29 @@ __device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *
   }
 }
 
-// t * x^i mod P for 0 <= i < 128.
-__device__ __forceinline__ uint32_t f128_dword_or_zero(f128 t, int idx) {
-  return idx == 0 ? t.x : idx == 1 ? t.y : idx == 2 ? t.z : idx == 3 ? t.w : 0u;
-}
+// t * x^i mod P for 0 <= i < 128 (i may differ per lane): the dword shift
+// by i / 32 through lane masks and the bit shift by i % 32 through 64-bit
+// shifts, so no lane-dependent branch.
 __device__ __forceinline__ f128 f128_mul_xpow(f128 t, uint32_t i) {
-  const int q = (int)(i >> 5);
-  const uint32_t r = i & 31;
+  const uint32_t q = i >> 5, r = i & 31;
+  uint32_t m[4];
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) m[j] = 0u - (uint32_t)(q == j);
+  const uint32_t tw[4] = {t.x, t.y, t.z, t.w};
   uint32_t d[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    // dword k of (t << i) takes bits from dwords k - q and k - q - 1 of t
-    const uint32_t hi = f128_dword_or_zero(t, k - q), lo = f128_dword_or_zero(t, k - q - 1);
-    d[k] = r ? __builtin_amdgcn_alignbit(hi, lo, 32 - r) : hi;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (k - j >= 0 && k - j <= 3) acc |= tw[k - j] & m[j];
+    d[k] = acc;
   }
-  return f128_reduce256(d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
+  uint32_t e[8];
+  e[0] = d[0] << r;
+#pragma unroll
+  for (int k = 1; k < 8; ++k) e[k] = (uint32_t)(((((uint64_t)d[k]) << 32 | d[k - 1]) << r) >> 32);
+  return f128_reduce256(e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7]);
 }
 
 // Full byte tables of one twiddle from its 128 basis values, E = 4096 /
