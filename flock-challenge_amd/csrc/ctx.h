@@ -58,6 +58,12 @@ struct Ctx {
   size_t pinned_bytes = 0, pinned_off = 0;
   uint8_t *landing = nullptr;
   size_t landing_bytes = 0;
+  // Coherent pinned host memory mapped into the device: a kernel writes a
+  // small result (a sumcheck message) straight into it, and the host reads it
+  // after synchronizing the stream, with no copy kernel in between.
+  uint8_t *readback = nullptr;
+  void *readback_dev = nullptr;
+  static constexpr size_t READBACK_BYTES = 4096;
   uint8_t *pinned_alloc(size_t n);  // nullptr when the ring is full (or unavailable)
   void pinned_reset() { pinned_off = 0; }
 

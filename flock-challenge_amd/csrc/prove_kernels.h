@@ -53,6 +53,8 @@ int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, 
 int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed);
 int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
                     void *d_best);
+// out (device-visible) = *best, then *best = all-ones for the next window.
+int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out);
 int gather_device(Ctx *ctx, hipStream_t st, const void *src, const uint64_t *d_idx, uint32_t n, uint32_t unit_bytes,
                   void *out);
 int scatter_device(Ctx *ctx, hipStream_t st, void *dst, const uint64_t *d_idx, const void *d_val, uint32_t n);

@@ -263,6 +263,8 @@ struct Prover {
   const flock_lig_config *cfg = nullptr;
   std::vector<const char *> scratch_names;
 
+  bool pow_armed = false;  // the device's best nonce holds all-ones
+
   void *dev(const char *name, size_t bytes) { return ctx->get_scratch(name, bytes); }
 
   // Device-to-host read; waits for the stream (the host needs the bytes).
@@ -277,6 +279,14 @@ struct Prover {
     }
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, st));
     FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return FLOCK_OK;
+  }
+  // A small result a kernel wrote into the mapped readback area: wait for
+  // the stream, then read it.
+  int fetch(void *h, size_t n) {
+    HostPhase hp(ctx, "wait_d2h");
+    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    memcpy(h, ctx->readback, n);
     return FLOCK_OK;
   }
   // Host-to-device upload without waiting: the bytes are staged in the
@@ -331,13 +341,17 @@ struct Prover {
       t.state_digest(dg);
       unsigned long long *best = (unsigned long long *)dev("pow.best", 8);
       if (!best) return FLOCK_E_OOM;
+      if (!pow_armed) {  // every window's take re-arms it afterwards
+        unsigned long long none = ~0ull;
+        if (int rc = h2d(best, &none, 8)) return rc;
+        pow_armed = true;
+      }
       uint64_t start = 0, window = 1ull << std::min<uint32_t>(bits + 2, 26);
       for (;;) {
-        unsigned long long none = ~0ull;
-        int rc = h2d(best, &none, 8);
-        if (!rc) rc = pow_scan_device(ctx, st, dg, start, window, bits, best);
+        int rc = pow_scan_device(ctx, st, dg, start, window, bits, best);
+        if (!rc) rc = pow_take_device(ctx, st, best, ctx->readback_dev);
         unsigned long long got = 0;
-        if (!rc) rc = d2h(&got, best, 8);
+        if (!rc) rc = fetch(&got, 8);
         if (rc) return rc;
         if (got != ~0ull) {
           nonce = got;
@@ -417,11 +431,9 @@ struct Prover {
   }
 
   int msg(const void *f, const void *b, uint64_t len, int with_eval, F128 out[3]) {
-    uint4 *d = (uint4 *)dev("lig.msg.out", 48);
-    if (!d) return FLOCK_E_OOM;
-    int rc = lig_msg_device(ctx, st, f, b, len / 2, with_eval, d);
+    int rc = lig_msg_device(ctx, st, f, b, len / 2, with_eval, ctx->readback_dev);
     uint4 h[3];
-    if (!rc) rc = d2h(h, d, 48);
+    if (!rc) rc = fetch(h, 48);
     for (int i = 0; i < 3; ++i) out[i] = fv(h[i]);
     return rc;
   }
@@ -453,6 +465,17 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       ctx->landing = (uint8_t *)p;
       ctx->landing_bytes = 1 << 20;
     }
+  }
+  if (!ctx->readback) {
+    void *p = nullptr, *dp = nullptr;
+    FLOCK_HIP_CHECK(ctx, hipHostMalloc(&p, Ctx::READBACK_BYTES, hipHostMallocMapped | hipHostMallocCoherent));
+    if (hipHostGetDevicePointer(&dp, p, 0) != hipSuccess) {
+      hipHostFree(p);
+      ctx->last_error = "prove: cannot map the readback area into the device";
+      return FLOCK_E_DEVICE;
+    }
+    ctx->readback = (uint8_t *)p;
+    ctx->readback_dev = dp;
   }
   ScopedPhase ph_all(ctx, "prove", st);
   HostPhase hp_all(ctx, "prove_wall");
@@ -575,9 +598,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     void *A = P.dev("zc.A", rows * 8), *Bv = P.dev("zc.B", rows * 8);
     void *A2 = P.dev("zc.A2", rows * 4), *B2 = P.dev("zc.B2", rows * 4);
     void *E = P.dev("zc.E", rows * 8), *E2 = P.dev("zc.E2", rows * 4);
-    void *dlw = P.dev("zc.lw", 256 * 16), *dlwr = P.dev("zc.lwr", 256 * 16), *dmsg = P.dev("zc.msg", 32);
+    void *dlw = P.dev("zc.lw", 256 * 16), *dlwr = P.dev("zc.lwr", 256 * 16);
     void *dlw8 = P.dev("zc.lw8", 2048 * 16);
-    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw || !dlwr || !dlw8 || !dmsg) return FLOCK_E_OOM;
+    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw || !dlwr || !dlw8) return FLOCK_E_OOM;
     // nibble tables: lw[v * 16 + p] = sum_{t < 4, bit t of v} L_{4p + t}(z)
     auto nibble_tables = [](const F128 *w, F128 *out) {
       for (int v = 0; v < 16; ++v)
@@ -609,13 +632,13 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       const size_t half = len / 2;
       // round i's eq table is eq(r[7 + i ..]); the kernel folds it for round i + 1
       if (i == 0)
-        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, dmsg, i + 1 < n ? E2 : nullptr);
+        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
       else
-        rc = zc_round_device(ctx, st, A, Bv, E, half, dmsg, i + 1 < n ? E2 : nullptr);
+        rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
       if (rc) return rc;
       std::swap(E, E2);
       uint4 h[2];
-      if ((rc = P.d2h(h, dmsg, 32))) return rc;
+      if ((rc = P.fetch(h, 32))) return rc;
       F128 g1 = fv(h[0]), gi = fv(h[1]);
       zc_rounds.push_back({g1, gi});
       t.observe(g1);

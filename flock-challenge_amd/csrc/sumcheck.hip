@@ -576,6 +576,22 @@ int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t
   return FLOCK_OK;
 }
 
+// Hand the window's result to the host and re-arm the scan: out = best,
+// best = none (so the next window needs no upload).
+__global__ void pow_take_kernel(unsigned long long *__restrict__ best, unsigned long long *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = best[0];
+    best[0] = ~0ull;
+  }
+}
+
+int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out) {
+  hipLaunchKernelGGL(pow_take_kernel, dim3(1), dim3(64), 0, st, (unsigned long long *)d_best,
+                     (unsigned long long *)out);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
 // ---------------------------------------------------------------- gathers
 __global__ void gather_kernel(const uint4 *__restrict__ src, const uint64_t *__restrict__ idx, uint32_t n,
                               uint32_t unit_u4, uint4 *__restrict__ out) {
