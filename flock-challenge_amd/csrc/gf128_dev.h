@@ -138,39 +138,50 @@ __device__ __forceinline__ f128 f128_reduce256(uint32_t l0, uint32_t l1, uint32_
   return r;
 }
 
-// The lane's table T[v] = v(x) * a (entry e at tab[e * 64 + lane]).
+// The lane's table T[v] = v(x) * a, entry v of lane L at byte
+// (L >> 4) * 4096 + v * 256 + (L & 15) * 16 of the wavefront's 16 KiB area:
+// the 16 lanes the LDS serves together sit in 16 distinct bank groups, and a
+// lookup address is one v_perm of the lane's offset byte and the nibble.
+__device__ __forceinline__ uint32_t comb_lane_base(uint32_t lane) { return ((lane >> 4) << 12) | ((lane & 15u) << 4); }
+
 __device__ __forceinline__ void f128_mul_build(f128 a, f128 *__restrict__ tab, uint32_t lane) {
   f128 t1 = a, t2 = f128_mulx(a), t4 = f128_mulx(t2), t8 = f128_mulx(t4);
   f128 t3 = f128_xor(t1, t2), t5 = f128_xor(t4, t1), t6 = f128_xor(t4, t2), t7 = f128_xor(t4, t3);
-  f128 *T = tab + lane;
-  T[0] = f128_zero();
-  T[64] = t1;
-  T[128] = t2;
-  T[192] = t3;
-  T[256] = t4;
-  T[320] = t5;
-  T[384] = t6;
-  T[448] = t7;
-  T[512] = t8;
-  T[576] = f128_xor(t8, t1);
-  T[640] = f128_xor(t8, t2);
-  T[704] = f128_xor(t8, t3);
-  T[768] = f128_xor(t8, t4);
-  T[832] = f128_xor(t8, t5);
-  T[896] = f128_xor(t8, t6);
-  T[960] = f128_xor(t8, t7);
+  char *T = (char *)tab + comb_lane_base(lane & 63u);
+  const f128 v[16] = {f128_zero(),       t1,
+                      t2,                t3,
+                      t4,                t5,
+                      t6,                t7,
+                      t8,                f128_xor(t8, t1),
+                      f128_xor(t8, t2),  f128_xor(t8, t3),
+                      f128_xor(t8, t4),  f128_xor(t8, t5),
+                      f128_xor(t8, t6),  f128_xor(t8, t7)};
+#pragma unroll
+  for (int e = 0; e < 16; ++e) *(f128 *)(T + e * 256) = v[e];
 }
 
 // a * b with the lane's table of a already built (f128_mul_build).
 __device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ tab, uint32_t lane) {
-  const f128 *T = tab + lane;
+  const char *T = (const char *)tab;
+  const uint32_t lo = (lane & 15u) << 4;                  // byte 0 of every address
+  const uint32_t hb = ((lane >> 4) & 3u) * 0x10101010u;   // (L >> 4) << 4 in every byte
+  // even / odd nibbles of each word, one per byte, with the lane's 4 KiB block
+  const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
+  uint32_t ev[4], od[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ev[i] = (bw[i] & 0x0F0F0F0Fu) | hb;
+    od[i] = ((bw[i] >> 4) & 0x0F0F0F0Fu) | hb;
+  }
   uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0, d6 = 0, d7 = 0;
 #pragma unroll
   for (int j = 7; j >= 0; --j) {
-    const f128 e0 = T[((b.x >> (4 * j)) & 15u) << 6];
-    const f128 e1 = T[((b.y >> (4 * j)) & 15u) << 6];
-    const f128 e2 = T[((b.z >> (4 * j)) & 15u) << 6];
-    const f128 e3 = T[((b.w >> (4 * j)) & 15u) << 6];
+    // nibble j of word i: byte j / 2 of od (j odd) or ev (j even)
+    const uint32_t sel = 4u | ((uint32_t)(j >> 1) << 8) | 0x0C0C0000u;
+    const f128 e0 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[0] : ev[0], sel));
+    const f128 e1 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[1] : ev[1], sel));
+    const f128 e2 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[2] : ev[2], sel));
+    const f128 e3 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[3] : ev[3], sel));
     d0 ^= e0.x;
     d1 ^= e0.y ^ e1.x;
     d2 ^= e0.z ^ e1.y ^ e2.x;
@@ -193,9 +204,8 @@ __device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ t
 }
 
 // General product a * b with both operands varying per lane (comb method).
-// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS (entry e at
-// tab[e * 64 + lane], so a wave's reads of one entry index are contiguous);
-// then for nibble position j = 7..0 of the four 32-bit words b_i of b, the
+// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS (layout at
+// f128_mul_build); then for nibble position j = 7..0 of the four 32-bit words b_i of b, the
 // entries T[nibble_j(b_i)] are XORed into a 256-bit accumulator at dword
 // offset i, and the accumulator shifts by 4 between positions: 32 lookups,
 // 7 wide shifts and one final reduction.  No cross-lane traffic: a lane only

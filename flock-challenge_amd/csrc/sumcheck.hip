@@ -248,6 +248,8 @@ int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, cons
 // v'[x] = v[2x] + r (v[2x] + v[2x+1]) for two vectors (fold_in_place_pair,
 // multilinear.rs:3209-3230; partial_eval_lsb, pcs/ligerito.rs:1673).  The
 // product by r uses the workgroup's full byte tables of r (fulltab.h).
+constexpr int FOLD_UNROLL = 2;
+
 __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
                                                                uint64_t half, f128 r, f128 *__restrict__ A2,
                                                                f128 *__restrict__ B2) {
@@ -259,16 +261,35 @@ __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__res
   build_fulltab<FT_THREADS>(basis, tab, tid);
   __syncthreads();
   const FulltabLane L(rot);
-  for (uint64_t x = (uint64_t)blockIdx.x * FT_THREADS + tid; x < half; x += (uint64_t)gridDim.x * FT_THREADS) {
-    const f128 a0 = A[2 * x], a1 = A[2 * x + 1];
-    if (B) {
-      const f128 b0 = B[2 * x], b1 = B[2 * x + 1];
-      f128 pa, pb;
-      f128_mul2_fulltab(f128_xor(a0, a1), f128_xor(b0, b1), (const char *)tab, L, pa, pb);
-      A2[x] = f128_xor(a0, pa);
-      B2[x] = f128_xor(b0, pb);
-    } else {
-      A2[x] = f128_xor(a0, f128_mul_fulltab(f128_xor(a0, a1), (const char *)tab, L));
+  // FOLD_UNROLL elements per thread per pass, all loaded before any product,
+  // so a pass waits for memory once
+  const uint64_t stride = (uint64_t)gridDim.x * FT_THREADS;
+  for (uint64_t x0 = (uint64_t)blockIdx.x * FT_THREADS + tid; x0 < half; x0 += FOLD_UNROLL * stride) {
+    f128 a0[FOLD_UNROLL], a1[FOLD_UNROLL], b0[FOLD_UNROLL], b1[FOLD_UNROLL];
+#pragma unroll
+    for (int u = 0; u < FOLD_UNROLL; ++u) {
+      const uint64_t x = x0 + u * stride;
+      if (x < half) {
+        a0[u] = A[2 * x];
+        a1[u] = A[2 * x + 1];
+        if (B) {
+          b0[u] = B[2 * x];
+          b1[u] = B[2 * x + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < FOLD_UNROLL; ++u) {
+      const uint64_t x = x0 + u * stride;
+      if (x >= half) break;
+      if (B) {
+        f128 pa, pb;
+        f128_mul2_fulltab(f128_xor(a0[u], a1[u]), f128_xor(b0[u], b1[u]), (const char *)tab, L, pa, pb);
+        A2[x] = f128_xor(a0[u], pa);
+        B2[x] = f128_xor(b0[u], pb);
+      } else {
+        A2[x] = f128_xor(a0[u], f128_mul_fulltab(f128_xor(a0[u], a1[u]), (const char *)tab, L));
+      }
     }
   }
 }
@@ -276,8 +297,8 @@ __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__res
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2) {
   ScopedPhase ph(ctx, "fold_pair", st);
   f128 rr = make_uint4((uint32_t)r.lo, (uint32_t)(r.lo >> 32), (uint32_t)r.hi, (uint32_t)(r.hi >> 32));
-  // enough elements per workgroup to amortize its table build
-  unsigned g = grid_for(half, FT_THREADS * 8, 512);
+  // FOLD_UNROLL elements per thread (one memory wait) until 512 workgroups
+  unsigned g = grid_for(half, FT_THREADS * FOLD_UNROLL, 512);
   hipLaunchKernelGGL(fold_pair_kernel, dim3(g), dim3(FT_THREADS), 0, st, (const f128 *)A, (const f128 *)B, half, rr,
                      (f128 *)A2, (f128 *)B2);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
