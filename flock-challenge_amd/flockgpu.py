@@ -3,9 +3,11 @@ in include/flock_gpu.h (libflockgpu.so).
 
 Mirrors the names of crates/flock-prover/src/r1cs_hashes/blake3.rs
 (`Blake3Setup::new`, `prove_fast`, `verify`) and crates/flock-core/src/pcs/
-commit.rs (`PcsParams`, `commit`).  Every call runs the HIP kernels; there is
-no CPU fallback: if the library is missing or no GPU is present, the calls
-raise.
+commit.rs (`PcsParams`, `commit`).  Every proving call runs the HIP kernels; there
+is no CPU fallback: if the library is missing or no GPU is present, the calls
+raise.  `verify` (the reference verifier's checks) runs on the host, as the
+reference's does; the commitment root it compares against is recomputed on
+the GPU.
 """
 import ctypes
 import os
@@ -27,6 +29,11 @@ class FlockError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+
+
+class VerifyError(FlockError):
+    """A proof the verifier rejected (`VerifyError`, crates/flock-core/src/
+    verifier.rs:12-18); the message says which check failed."""
 
 
 class PcsParams(ctypes.Structure):
@@ -90,6 +97,11 @@ def lib():
         L.flock_prove_blake3_seeded.argtypes = [P, U32, U64, P, SZ, P, SZ, PSZ]
         L.flock_generate_compressions_device.argtypes = [P, U32, U64, P]
         L.flock_last_proof.argtypes = [P, P, SZ, PSZ]
+        L.flock_verify_blake3.argtypes = [P, SZ, U64, P, SZ, P, ctypes.c_char_p, SZ]
+        L.flock_commitment_root.argtypes = [P, P, U64, P]
+        L.flock_commitment_root_seeded.argtypes = [P, U32, U64, P]
+        L.flock_blake3.argtypes = [P, SZ, P]
+        L.flock_blake3.restype = None
         PD = ctypes.POINTER(ctypes.c_double)
         PU64 = ctypes.POINTER(ctypes.c_uint64)
         L.flock_server_create.argtypes = [ctypes.c_int, ctypes.c_int, U64, U32, ctypes.POINTER(P)]
@@ -111,6 +123,7 @@ def exported_symbols():
             "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_memcpy_d2d", "flock_synchronize", "flock_setup_create",
             "flock_setup_destroy", "flock_setup_info", "flock_prove_blake3", "flock_prove_blake3_device",
             "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof",
+            "flock_verify_blake3", "flock_commitment_root", "flock_commitment_root_seeded", "flock_blake3",
             "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
             "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending"]
 
@@ -299,6 +312,53 @@ class Blake3Setup:
         d = bytes(domain)
         return self._out(lambda o, c, n: lib().flock_prove_blake3_seeded(self.handle, log2, seed, d, len(d), o, c,
                                                                          n))
+
+
+    def commitment_root(self, blocks):
+        """Root of the commitment to the compressions' witness (the first
+        step of `prove_fast`), computed on the GPU."""
+        b = np.ascontiguousarray(blocks, dtype=COMPRESSION_DTYPE)
+        root = (ctypes.c_uint8 * 32)()
+        self.ctx._check(lib().flock_commitment_root(self.handle, b.ctypes.data, b.size, root))
+        return bytes(root)
+
+    def commitment_root_seeded(self, log2, seed):
+        root = (ctypes.c_uint8 * 32)()
+        self.ctx._check(lib().flock_commitment_root_seeded(self.handle, log2, seed, root))
+        return bytes(root)
+
+    def verify(self, proof, blocks=None, domain=BENCH_DOMAIN):
+        """`Blake3Setup::verify` (blake3.rs:3698) with the harness's bundle
+        checks; with `blocks`, the commitment must be the compressions' own.
+        Raises VerifyError on rejection."""
+        root = self.commitment_root(blocks) if blocks is not None else None
+        verify(proof, self.n_blocks, domain, root)
+
+    def verify_seeded(self, log2, seed, proof, domain=BENCH_DOMAIN):
+        """Verify a proof of generate_compressions(log2, seed)."""
+        verify(proof, 1 << log2, domain, self.commitment_root_seeded(log2, seed))
+
+
+def verify(proof, n_blocks, domain=BENCH_DOMAIN, expected_root=None):
+    """Host verification of proof bundle bytes for n_blocks compressions
+    (flock_verify_blake3); expected_root, if given, must be the bundle's
+    commitment root.  Raises VerifyError on rejection."""
+    b, d = bytes(proof), bytes(domain)
+    err = ctypes.create_string_buffer(256)
+    root = (ctypes.c_uint8 * 32).from_buffer_copy(expected_root) if expected_root is not None else None
+    rc = lib().flock_verify_blake3(b, len(b), n_blocks, d, len(d), root, err, len(err))
+    if rc == -6:
+        raise VerifyError(rc, err.value.decode(errors="replace"))
+    if rc:
+        raise FlockError(rc, err.value.decode(errors="replace"))
+
+
+def blake3(data):
+    """BLAKE3 digest (32 bytes) of `data`, computed by the library."""
+    b = bytes(data)
+    out = (ctypes.c_uint8 * 32)()
+    lib().flock_blake3(b, len(b), out)
+    return bytes(out)
 
 
 def generate_compressions_device(ctx, log2, seed):

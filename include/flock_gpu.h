@@ -130,6 +130,34 @@ int flock_generate_compressions_device(flock_ctx *ctx, uint32_t log2, uint64_t s
 /* Copy the bytes of the context's last proof (kept after FLOCK_E_CAP). */
 int flock_last_proof(flock_ctx *ctx, uint8_t *out, size_t cap, size_t *out_len);
 
+/* ------------------------------------------------------------------ verifying */
+
+/* Blake3Setup::verify (blake3.rs:3698) preceded by the harness's checks of
+ * the bundle (benchmark-tools/harness/src/main.rs:319-366): parse the bundle
+ * bytes of a proof of n_blocks compressions, check the commitment parameters,
+ * replay the BLAKE3 Fiat-Shamir transcript with domain `domain` and check the
+ * zerocheck, lincheck, ring-switch and Ligerito steps (sumchecks, proofs of
+ * work, Merkle openings, final residual).  expected_root (may be NULL) is the
+ * commitment root recomputed from the compressions (flock_commitment_root);
+ * the bundle's must equal it.  Host only: no context or GPU.  Returns
+ * FLOCK_OK when accepted, FLOCK_E_VERIFY when rejected, FLOCK_E_SHAPE for an
+ * unsupported size; the reason goes to err (NUL-terminated, may be NULL). */
+int flock_verify_blake3(const uint8_t *proof, size_t len, uint64_t n_blocks, const uint8_t *domain,
+                        size_t domain_len, const uint8_t expected_root[32], char *err, size_t err_cap);
+
+/* The commitment root of the witness of the compressions: the first step of
+ * prove_fast (pcs::commit of the packed witness, crates/flock-core/src/pcs/
+ * commit.rs:250-320), run on the device.  n_blocks must equal the setup's. */
+int flock_commitment_root(flock_setup *setup, const flock_compression *blocks, uint64_t n_blocks, uint8_t root[32]);
+
+/* Same for generate_compressions(log2, seed) expanded on the device. */
+int flock_commitment_root_seeded(flock_setup *setup, uint32_t log2, uint64_t seed, uint8_t root[32]);
+
+/* BLAKE3 hash (public BLAKE3 specification, 32-byte output) of len bytes on
+ * the host; no counterpart in the reference's API (it calls the blake3 crate),
+ * here for fingerprinting proof bytes. */
+void flock_blake3(const void *data, size_t len, uint8_t out[32]);
+
 /* ------------------------------------------------------------------ serving */
 
 /* A proof server on GPU `device`: continuous batching of independent proofs.

@@ -101,12 +101,32 @@ def test_canary_m32_matches_fixture(ctx):
     g = json.load(open(os.path.join(GOLDEN, "canary_m32.json")))
     _say("prove canary m=32")
     s = flockgpu.Blake3Setup(ctx, 1 << g["log2"])
+    seed = g["raw_seed"] ^ (g["log2"] << 29)
     try:
-        proof = s.prove_seeded(g["log2"], g["raw_seed"] ^ (g["log2"] << 29))
+        proof = s.prove_seeded(g["log2"], seed)
+        assert len(proof) == g["reference_expected_len"]
+        assert O.blake3(proof).hex() == g["proof_blake3"]
+        # the library's own verifier, with the root recomputed on the GPU
+        _say("verify canary m=32")
+        assert s.commitment_root_seeded(g["log2"], seed) == proof[7:39]
+        s.verify_seeded(g["log2"], seed, proof)
     finally:
         s.close()
-    assert len(proof) == g["reference_expected_len"]
-    assert O.blake3(proof).hex() == g["proof_blake3"]
+
+
+def test_commitment_root_and_verifier_on_host_blocks(ctx):
+    _say("commitment root + verify log2=10")
+    blocks = O.generate_compressions(10, 0x5EED)
+    s = flockgpu.Blake3Setup(ctx, blocks.size)
+    try:
+        proof = s.prove_fast(blocks)
+        assert s.commitment_root(blocks) == proof[7:39]
+        s.verify(proof, blocks)
+        other = O.generate_compressions(10, 0x5EEE)
+        with pytest.raises(flockgpu.VerifyError, match="root"):
+            s.verify(proof, other)
+    finally:
+        s.close()
 
 
 def test_shape_errors_are_reported(ctx):
