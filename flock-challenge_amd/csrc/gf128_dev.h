@@ -125,29 +125,50 @@ __device__ __forceinline__ f128 f128_shl4(f128 a) {
 __device__ __forceinline__ f128 f128_reduce256(uint32_t l0, uint32_t l1, uint32_t l2, uint32_t l3, uint32_t h0,
                                                uint32_t h1, uint32_t h2, uint32_t h3) {
   // H x^128 = H (x^7 + x^2 + x + 1); bits pushed past 127 fold once more.
+  const uint32_t o = xor3_u32(h3 >> 31, h3 >> 30, h3 >> 25);
   f128 r;
-  r.x = l0 ^ h0 ^ (h0 << 1) ^ (h0 << 2) ^ (h0 << 7);
-  r.y = l1 ^ h1 ^ __builtin_amdgcn_alignbit(h1, h0, 31) ^ __builtin_amdgcn_alignbit(h1, h0, 30) ^
-        __builtin_amdgcn_alignbit(h1, h0, 25);
-  r.z = l2 ^ h2 ^ __builtin_amdgcn_alignbit(h2, h1, 31) ^ __builtin_amdgcn_alignbit(h2, h1, 30) ^
-        __builtin_amdgcn_alignbit(h2, h1, 25);
-  r.w = l3 ^ h3 ^ __builtin_amdgcn_alignbit(h3, h2, 31) ^ __builtin_amdgcn_alignbit(h3, h2, 30) ^
-        __builtin_amdgcn_alignbit(h3, h2, 25);
-  const uint32_t o = (h3 >> 31) ^ (h3 >> 30) ^ (h3 >> 25);
-  r.x ^= fold87(o);
+  r.x = xor3_u32(xor3_u32(l0, h0, h0 << 1), xor3_u32(h0 << 2, h0 << 7, o),
+                 xor3_u32(o << 1, o << 2, o << 7));
+  r.y = xor3_u32(xor3_u32(l1, h1, __builtin_amdgcn_alignbit(h1, h0, 31)), __builtin_amdgcn_alignbit(h1, h0, 30),
+                 __builtin_amdgcn_alignbit(h1, h0, 25));
+  r.z = xor3_u32(xor3_u32(l2, h2, __builtin_amdgcn_alignbit(h2, h1, 31)), __builtin_amdgcn_alignbit(h2, h1, 30),
+                 __builtin_amdgcn_alignbit(h2, h1, 25));
+  r.w = xor3_u32(xor3_u32(l3, h3, __builtin_amdgcn_alignbit(h3, h2, 31)), __builtin_amdgcn_alignbit(h3, h2, 30),
+                 __builtin_amdgcn_alignbit(h3, h2, 25));
   return r;
 }
 
-// The lane's table T[v] = v(x) * a, entry v of lane L at byte
-// (L >> 4) * 4096 + v * 256 + (L & 15) * 16 of the wavefront's 16 KiB area:
-// the 16 lanes the LDS serves together sit in 16 distinct bank groups, and a
-// lookup address is one v_perm of the lane's offset byte and the nibble.
-__device__ __forceinline__ uint32_t comb_lane_base(uint32_t lane) { return ((lane >> 4) << 12) | ((lane & 15u) << 4); }
+// General product a * b with both operands varying per lane (comb method).
+// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS; then for
+// nibble position j = 7..0 of the four 32-bit words b_i of b, the entries
+// T[nibble_j(b_i)] are XORed into a 256-bit accumulator at dword offset i,
+// and the accumulator shifts by 4 between positions: 32 lookups, 7 wide
+// shifts and one final reduction.  No cross-lane traffic: a lane only reads
+// back what it wrote, so no barrier is needed.
+//
+// Tables of a workgroup's wavefronts share one area mt (16 KiB each).
+// Entry v of lane L in wavefront w sits at byte
+//   w * 16384 + (L >> 4) * 4096 + v * 256 + (L & 15) * 16,
+// so the 16 lanes the LDS serves together hit 16 distinct bank groups, and
+// a lookup address is one v_perm: byte 0 = (L & 15) * 16, byte 1 = the
+// nibble with (L >> 4) and (w & 3) in its high bits, byte 2 = w >> 2.
+struct CombLane {
+  uint32_t lo;    // bytes [(L & 15) * 16, 0, w >> 2, 0]
+  uint32_t hb;    // (L >> 4) << 4 | (w & 3) << 6 in every byte
+  uint32_t base;  // byte offset of the lane's entry 0
+  __device__ __forceinline__ explicit CombLane(uint32_t tid) {
+    const uint32_t L = tid & 63u, w = tid >> 6;
+    lo = ((L & 15u) << 4) | ((w >> 2) << 16);
+    hb = ((((L >> 4) & 3u) << 4) | ((w & 3u) << 6)) * 0x01010101u;
+    base = (w << 14) | ((L >> 4) << 12) | ((L & 15u) << 4);
+  }
+};
 
-__device__ __forceinline__ void f128_mul_build(f128 a, f128 *__restrict__ tab, uint32_t lane) {
+// The lane's table of a (entries v * 256 bytes apart from its base).
+__device__ __forceinline__ void f128_mul_build(f128 a, f128 *__restrict__ mt, const CombLane &C) {
   f128 t1 = a, t2 = f128_mulx(a), t4 = f128_mulx(t2), t8 = f128_mulx(t4);
   f128 t3 = f128_xor(t1, t2), t5 = f128_xor(t4, t1), t6 = f128_xor(t4, t2), t7 = f128_xor(t4, t3);
-  char *T = (char *)tab + comb_lane_base(lane & 63u);
+  char *T = (char *)mt + C.base;
   const f128 v[16] = {f128_zero(),       t1,
                       t2,                t3,
                       t4,                t5,
@@ -161,33 +182,31 @@ __device__ __forceinline__ void f128_mul_build(f128 a, f128 *__restrict__ tab, u
 }
 
 // a * b with the lane's table of a already built (f128_mul_build).
-__device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ tab, uint32_t lane) {
-  const char *T = (const char *)tab;
-  const uint32_t lo = (lane & 15u) << 4;                  // byte 0 of every address
-  const uint32_t hb = ((lane >> 4) & 3u) * 0x10101010u;   // (L >> 4) << 4 in every byte
-  // even / odd nibbles of each word, one per byte, with the lane's 4 KiB block
+__device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ mt, const CombLane &C) {
+  const char *T = (const char *)mt;
+  // even / odd nibbles of each word, one per byte, with the lane's block bits
   const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
   uint32_t ev[4], od[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    ev[i] = (bw[i] & 0x0F0F0F0Fu) | hb;
-    od[i] = ((bw[i] >> 4) & 0x0F0F0F0Fu) | hb;
+    ev[i] = (bw[i] & 0x0F0F0F0Fu) | C.hb;
+    od[i] = ((bw[i] >> 4) & 0x0F0F0F0Fu) | C.hb;
   }
   uint32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0, d6 = 0, d7 = 0;
 #pragma unroll
   for (int j = 7; j >= 0; --j) {
     // nibble j of word i: byte j / 2 of od (j odd) or ev (j even)
-    const uint32_t sel = 4u | ((uint32_t)(j >> 1) << 8) | 0x0C0C0000u;
-    const f128 e0 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[0] : ev[0], sel));
-    const f128 e1 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[1] : ev[1], sel));
-    const f128 e2 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[2] : ev[2], sel));
-    const f128 e3 = *(const f128 *)(T + __builtin_amdgcn_perm(lo, (j & 1) ? od[3] : ev[3], sel));
+    const uint32_t sel = 4u | ((uint32_t)(j >> 1) << 8) | (6u << 16) | (12u << 24);
+    const f128 e0 = *(const f128 *)(T + __builtin_amdgcn_perm(C.lo, (j & 1) ? od[0] : ev[0], sel));
+    const f128 e1 = *(const f128 *)(T + __builtin_amdgcn_perm(C.lo, (j & 1) ? od[1] : ev[1], sel));
+    const f128 e2 = *(const f128 *)(T + __builtin_amdgcn_perm(C.lo, (j & 1) ? od[2] : ev[2], sel));
+    const f128 e3 = *(const f128 *)(T + __builtin_amdgcn_perm(C.lo, (j & 1) ? od[3] : ev[3], sel));
     d0 ^= e0.x;
-    d1 ^= e0.y ^ e1.x;
-    d2 ^= e0.z ^ e1.y ^ e2.x;
-    d3 ^= e0.w ^ e1.z ^ e2.y ^ e3.x;
-    d4 ^= e1.w ^ e2.z ^ e3.y;
-    d5 ^= e2.w ^ e3.z;
+    d1 = xor3_u32(d1, e0.y, e1.x);
+    d2 = xor3_u32(xor3_u32(d2, e0.z, e1.y), e2.x, 0u);
+    d3 = xor3_u32(xor3_u32(d3, e0.w, e1.z), e2.y, e3.x);
+    d4 = xor3_u32(xor3_u32(d4, e1.w, e2.z), e3.y, 0u);
+    d5 = xor3_u32(d5, e2.w, e3.z);
     d6 ^= e3.w;
     if (j) {
       d7 = __builtin_amdgcn_alignbit(d7, d6, 28);
@@ -203,16 +222,9 @@ __device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ t
   return f128_reduce256(d0, d1, d2, d3, d4, d5, d6, d7);
 }
 
-// General product a * b with both operands varying per lane (comb method).
-// Each lane owns a 16-entry table T[v] = v(x) * a mod P in LDS (layout at
-// f128_mul_build); then for nibble position j = 7..0 of the four 32-bit words b_i of b, the
-// entries T[nibble_j(b_i)] are XORed into a 256-bit accumulator at dword
-// offset i, and the accumulator shifts by 4 between positions: 32 lookups,
-// 7 wide shifts and one final reduction.  No cross-lane traffic: a lane only
-// reads back what it wrote, so no barrier is needed.
-__device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ tab, uint32_t lane) {
-  f128_mul_build(a, tab, lane);
-  return f128_mul_comb(b, tab, lane);
+__device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ mt, const CombLane &C) {
+  f128_mul_build(a, mt, C);
+  return f128_mul_comb(b, mt, C);
 }
 
 // LDS bytes a wavefront needs for f128_mul scratch.

@@ -86,7 +86,8 @@ __global__ __launch_bounds__(MUL_THREADS) void zc_round_kernel(const f128 *__res
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
   const uint32_t lane = threadIdx.x & 63;
-  f128 *tab = mt[threadIdx.x >> 6];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
   f128 g1 = f128_zero(), gi = f128_zero();
   for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
     const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1], e = E[x];
@@ -99,11 +100,11 @@ __global__ __launch_bounds__(MUL_THREADS) void zc_round_kernel(const f128 *__res
       o.w = __shfl_xor(e.w, 1);
       if (!(x & 1)) En[x >> 1] = f128_xor(e, o);
     }
-    const f128 p1 = f128_mul(a1, b1, tab, lane);
-    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, lane);
-    f128_mul_build(e, tab, lane);  // one table of E for both weighted sums
-    f128_xor_into(g1, f128_mul_comb(p1, tab, lane));
-    f128_xor_into(gi, f128_mul_comb(pi, tab, lane));
+    const f128 p1 = f128_mul(a1, b1, tab, CL);
+    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
+    f128_mul_build(e, tab, CL);  // one table of E for both weighted sums
+    f128_xor_into(g1, f128_mul_comb(p1, tab, CL));
+    f128_xor_into(gi, f128_mul_comb(pi, tab, CL));
   }
   wg_xor_reduce(g1, red, &partial[2 * blockIdx.x]);
   wg_xor_reduce(gi, red, &partial[2 * blockIdx.x + 1]);
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(ZR0_THREADS) void zc_round0_kernel(const uint64_t *
   for (uint32_t i = threadIdx.x; i < 2048; i += ZR0_THREADS) T[i] = lw8[i];
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, rot8 = threadIdx.x & 7;
-  f128 *tab = mt[threadIdx.x >> 6];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
   f128 g1 = f128_zero(), gi = f128_zero();
   for (uint64_t x = (uint64_t)blockIdx.x * ZR0_THREADS + threadIdx.x; x < half;
        x += (uint64_t)gridDim.x * ZR0_THREADS) {
@@ -195,11 +197,11 @@ __global__ __launch_bounds__(ZR0_THREADS) void zc_round0_kernel(const uint64_t *
       o.w = __shfl_xor(e.w, 1);
       if (!(x & 1)) En[x >> 1] = f128_xor(e, o);
     }
-    const f128 p1 = f128_mul(a1, b1, tab, lane);
-    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, lane);
-    f128_mul_build(e, tab, lane);  // one table of E for both weighted sums
-    f128_xor_into(g1, f128_mul_comb(p1, tab, lane));
-    f128_xor_into(gi, f128_mul_comb(pi, tab, lane));
+    const f128 p1 = f128_mul(a1, b1, tab, CL);
+    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
+    f128_mul_build(e, tab, CL);  // one table of E for both weighted sums
+    f128_xor_into(g1, f128_mul_comb(p1, tab, CL));
+    f128_xor_into(gi, f128_mul_comb(pi, tab, CL));
   }
   __syncthreads();  // T is reused as the reduction buffer
   wg_xor_reduce(g1, T, &partial[2 * blockIdx.x]);
@@ -315,14 +317,15 @@ __global__ __launch_bounds__(MUL_THREADS) void lig_msg_kernel(const f128 *__rest
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
   const uint32_t lane = threadIdx.x & 63;
-  f128 *tab = mt[threadIdx.x >> 6];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
   f128 u0 = f128_zero(), u2 = f128_zero(), y = f128_zero();
   for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
     const f128 f0 = f[2 * x], f1 = f[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
-    const f128 e0 = f128_mul(f0, b0, tab, lane);
+    const f128 e0 = f128_mul(f0, b0, tab, CL);
     f128_xor_into(u0, e0);
-    f128_xor_into(u2, f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, lane));
-    if (with_eval) f128_xor_into(y, f128_xor(e0, f128_mul(f1, b1, tab, lane)));
+    f128_xor_into(u2, f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, CL));
+    if (with_eval) f128_xor_into(y, f128_xor(e0, f128_mul(f1, b1, tab, CL)));
   }
   wg_xor_reduce(u0, red, &partial[3 * blockIdx.x]);
   wg_xor_reduce(u2, red, &partial[3 * blockIdx.x + 1]);
@@ -443,7 +446,8 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_layer_kernel(f128 *__restrict
                                                                 int k, int l, int log_nl, int transposed) {
   __shared__ f128 mt[MUL_WAVES][1024];
   const uint32_t lane = threadIdx.x & 63;
-  f128 *tab = mt[threadIdx.x >> 6];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
   const uint64_t total = 1ull << (k - 1 + log_nl);
   for (uint64_t t = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; t < total; t += (uint64_t)gridDim.x * MUL_THREADS) {
     const uint64_t ln = t & ((1ull << log_nl) - 1), bf = t >> log_nl;
@@ -453,12 +457,12 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_layer_kernel(f128 *__restrict
     const f128 w = tw[(1ull << l) - 1 + block];
     f128 u = d[(top << log_nl) + ln], v = d[(bot << log_nl) + ln];
     if (!transposed) {
-      f128_xor_into(u, f128_mul(v, w, tab, lane));
+      f128_xor_into(u, f128_mul(v, w, tab, CL));
       f128_xor_into(v, u);
     } else {
       const f128 s = f128_xor(u, v);
       u = s;
-      v = f128_xor(f128_mul(s, w, tab, lane), v);
+      v = f128_xor(f128_mul(s, w, tab, CL), v);
     }
     d[(top << log_nl) + ln] = u;
     d[(bot << log_nl) + ln] = v;
@@ -485,7 +489,8 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__re
                                                                        int log_nl, uint64_t n_threads) {
   __shared__ f128 mt[MUL_WAVES][1024];
   const uint32_t lane = threadIdx.x & 63;
-  f128 *tab = mt[threadIdx.x >> 6];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
   const uint64_t t = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x;
   if (t >= n_threads) return;
   const uint64_t ln = t & ((1ull << log_nl) - 1), g = t >> log_nl;
@@ -509,18 +514,18 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__re
     const uint64_t tw_base = (1ull << (l + j)) - 1ull + (B << j);
 #pragma unroll
     for (int c = 0; c < (1 << j); ++c) {
-      f128_mul_build(tw[tw_base + c], tab, lane);  // one table per twiddle
+      f128_mul_build(tw[tw_base + c], tab, CL);  // one table per twiddle
 #pragma unroll
       for (int el = 0; el < (1 << (F - 1 - j)); ++el) {
         const int e0 = (c << (F - j)) | el, e1 = e0 | (1 << (F - 1 - j));
         if (!TRANSPOSED) {
-          const f128 u = f128_xor(v[e0], f128_mul_comb(v[e1], tab, lane));
+          const f128 u = f128_xor(v[e0], f128_mul_comb(v[e1], tab, CL));
           v[e0] = u;
           v[e1] = f128_xor(v[e1], u);
         } else {
           const f128 sum = f128_xor(v[e0], v[e1]);
           v[e0] = sum;
-          v[e1] = f128_xor(f128_mul_comb(sum, tab, lane), v[e1]);
+          v[e1] = f128_xor(f128_mul_comb(sum, tab, CL), v[e1]);
         }
       }
     }
