@@ -50,35 +50,16 @@ const ZcTables &zc_tables() {
         }
         M[j][s] = f8_mul(num, f8_inv(den));
       }
-    uint8_t *ext8 = (uint8_t *)T.ext8;
     for (int v = 0; v < 256; ++v)
       for (int p = 0; p < 8; ++p)
         for (int lam = 0; lam < 64; ++lam) {
           uint8_t acc = 0;
           for (int t = 0; t < 8; ++t)
             if ((v >> t) & 1) acc ^= M[lam][8 * p + t];
-          ext8[(v * 8 + p) * 64 + lam] = acc;
+          const int h = lam >> 5, bit = lam & 31;
+          for (int t = 0; t < 8; ++t)
+            if ((acc >> t) & 1) T.extb[((t >> 2) * 4096 + v * 16 + 2 * p + h) * 4 + (t & 3)] |= 1u << bit;
         }
-    uint8_t x = 1;
-    for (int i = 0; i < 255; ++i) {
-      T.log8[x] = (uint16_t)i;
-      x = f8_mul(x, 3);
-    }
-    T.log8[0] = 1024;
-    for (int i = 0; i < 3 * 1024 + 4; ++i) {
-      uint8_t e = 1;
-      if (i < 1024) {
-        for (int k = 0; k < i % 255; ++k) e = f8_mul(e, 3);
-        T.exp8[i] = e;
-      } else {
-        T.exp8[i] = 0;
-      }
-    }
-    for (int xl = 0; xl < 8; ++xl) {
-      uint8_t eps = 1;
-      for (int i = 0; i < 3; ++i) eps = f8_mul(eps, ((xl >> i) & 1) ? SMALL_CHAL[i] : (uint8_t)(SMALL_CHAL[i] ^ 1));
-      T.log_eps[xl] = T.log8[eps];
-    }
     F128 beta[4];
     for (int i = 0; i < 4; ++i) beta[i] = medium(i);
     for (int xm = 0; xm < 16; ++xm) {
@@ -462,17 +443,19 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     if ((rc = P.eq_device(&r[11], m - 11, dhic))) return rc;
     if ((rc = bitfold_device(ctx, st, z, 4, 1ull << (m - 11), dhic, nullptr, dq))) return rc;
     {
-      // P^AB(lambda_j) = sum_{x_mid, t} E4(x_mid) phi8(2^t) Q[4 x_mid + j / 16][8 (j % 16) + t]
+      // P^AB(lambda_j) = sum_{x_mid, t} E4(x_mid) phi8(2^t) bitfold of bit t of
+      // Y_j: element 4 x_mid + 2 h + t / 4, bit 32 (t % 4) + b for j = 32 h + b
       std::vector<F128> qab(64 * 128);
       if ((rc = P.d2h(qab.data(), dout, qab.size() * 16))) return rc;
       const ZcTables &T = zc_tables();
       for (int j = 0; j < 64; ++j) {
+        const int h = j >> 5, bit = j & 31;
         F128 acc;
         for (int xm = 0; xm < 16; ++xm)
           for (int t = 0; t < 8; ++t) {
             F128 w;
             memcpy(&w, T.pt[xm][t], 16);
-            acc += w * qab[(4 * xm + j / 16) * 128 + 8 * (j % 16) + t];
+            acc += w * qab[(4 * xm + 2 * h + (t >> 2)) * 128 + 32 * (t & 3) + bit];
           }
         r1ab[j] = acc;
       }

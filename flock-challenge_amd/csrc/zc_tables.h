@@ -6,13 +6,12 @@
 namespace flock {
 
 struct ZcTables {
-  // S -> Lambda extension by byte: the 64 Lambda bytes of the contribution
-  // of byte value v at byte position p of a row sit at byte offset
-  // (v * 8 + p) * 64 (16-byte chunk q4 = lambda / 16 at (v * 8 + p) * 4 + q4).
-  alignas(16) uint32_t ext8[256 * 8 * 16];
-  uint16_t log8[256];          // discrete log base 3; log8[0] = 1024 (zero sentinel)
-  uint8_t exp8[3 * 1024 + 4];  // exp8[i] = 3^(i mod 255) for i < 1024, else 0
-  uint16_t log_eps[8];         // log of eps(x_lo)
+  // Bitsliced S -> Lambda extension.  For byte value v at byte position p of
+  // a row, half h of the 64 lambdas and plane group k, uint4 number
+  // k * 4096 + v * 16 + 2 p + h holds 4 dwords: dword d is bit 4 k + d of
+  // the contribution's values at lambdas 32 h .. 32 h + 31 (bit b of the
+  // dword <-> lambda 32 h + b).  Its 16-byte bank group is 2 p + h.
+  alignas(16) uint32_t extb[2 * 4096 * 4];
   alignas(16) uint32_t pt[16][8][4];  // E4(x_mid) * phi8(2^t), F128 as 4 LE dwords
 };
 

@@ -8,7 +8,7 @@
 // GF(2^8) constants, so the eq weight of the low three row bits, eps(x_lo),
 // lives in GF(2^8) and
 //   Y_j(x_mid, x_hi) = sum_{x_lo} eps(x_lo) * ahat_j * bhat_j    (in GF(2^8))
-// is formed without touching GF(2^128).  zc_y_kernel writes these bytes;
+// is formed without touching GF(2^128).  zc_y_kernel writes these values;
 // then P^AB(lambda_j) = sum_{x_mid} E4(x_mid) sum_t phi8(2^t)
 // sum_{x_hi} Ehi(x_hi) bit_t(Y_j(x_mid, x_hi)), whose inner sums are a
 // bit-column fold of Y with the weights Ehi (bitfold.hip).  This matches the
@@ -16,12 +16,14 @@
 // step is a regrouping of the same exact field sum.  P^C is linear in the
 // bits of c and is computed from a bit-column fold of c.
 //
-// Extension by byte tables: ahat over all 64 lambdas is the XOR of 8 table
-// rows, one per byte of the row.  Thread (x_mid, q4) handles lambdas
-// 16 q4 .. 16 q4 + 15 (one 16-byte LDS read per byte).  Entry (v, p, q4) is
-// at chunk (v * 8 + p) * 4 + q4, so its bank group (mod 16) is
-// 4 (p mod 4) + q4; a lane visits byte positions p = step ^ (x_mid mod 4),
-// which makes the 16 lanes served together hit 16 distinct bank groups.
+// Bitsliced: a lane handles 32 lambdas of one (x_mid, x_hi) and keeps each
+// GF(2^8) value as 8 bit-planes of 32 lanes.  The extension of a row is the
+// XOR of 8 table entries (one per row byte, 8 planes each); the product
+// ahat * bhat is a schoolbook AND/XOR network (64 partial products, then
+// x^8 .. x^14 folded back), and the constant eps(x_lo) is a fixed XOR
+// network, with no table lookup at data-dependent addresses.  The bit-column
+// fold reads Y's bits in that plane order (element 4 x_mid + 2 h + t / 4,
+// bit 32 (t mod 4) + lambda mod 32); the host combination follows it.
 #include <hip/hip_runtime.h>
 
 #include "../../include/flock_gpu.h"
@@ -32,69 +34,161 @@
 
 namespace flock {
 
-__device__ __forceinline__ uint32_t f8_prod4(uint32_t ea, uint32_t eb, uint32_t le, const uint16_t *__restrict__ lg,
-                                             const uint8_t *__restrict__ ex) {
-  uint32_t y = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t va = (ea >> (8 * k)) & 255u, vb = (eb >> (8 * k)) & 255u;
-    y |= (uint32_t)ex[lg[va] + lg[vb] + le] << (8 * k);
+// GF(2^8) (AES polynomial 0x11B) at compile time: the eq weights eps(x_lo)
+// of the three low row bits are products of the small challenges
+// (univariate_skip_optimized.rs:103-106), fixed constants.
+__host__ __device__ constexpr uint8_t f8_mul_const(uint8_t a, uint8_t b) {
+  uint8_t r = 0;
+  for (int i = 0; i < 8; ++i) {
+    if (b & 1) r ^= a;
+    b >>= 1;
+    a = (a & 0x80) ? (uint8_t)((a << 1) ^ 0x1B) : (uint8_t)(a << 1);
   }
-  return y;
+  return r;
+}
+__host__ __device__ constexpr uint8_t zc_eps(int xl) {
+  const uint8_t c[3] = {0xF7, 0x53, 0xB5};
+  uint8_t e = 1;
+  for (int i = 0; i < 3; ++i) e = f8_mul_const(e, ((xl >> i) & 1) ? c[i] : (uint8_t)(c[i] ^ 1));
+  return e;
+}
+// x^k mod 0x11B for k = 8..14 (the reduction of a bitsliced product)
+__host__ __device__ constexpr uint8_t f8_xpow(int k) {
+  uint8_t v = 1;
+  for (int i = 0; i < k; ++i) v = f8_mul_const(v, 2);
+  return v;
 }
 
-__device__ __forceinline__ uint4 ext8(uint64_t row, const uint4 *__restrict__ s_ext, uint32_t q4, uint32_t sw) {
-  const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
-  uint4 e = make_uint4(0, 0, 0, 0);
+// acc ^= terms[i] for every i with bit i of `mask` (a compile-time constant
+// after unrolling), two terms per three-input XOR.
+template <int N>
+__device__ __forceinline__ uint32_t xor_masked(uint32_t acc, const uint32_t (&terms)[N], uint32_t mask) {
+  uint32_t pend = 0;
+  bool has = false;
 #pragma unroll
-  for (uint32_t step = 0; step < 8; ++step) {
-    const uint32_t p = step ^ sw;
-    const uint32_t w = step < 4 ? lo : hi;
-    const uint32_t v = __builtin_amdgcn_ubfe(w, 8 * (p & 3), 8);
-    const uint4 t = s_ext[(v * 8 + p) * 4 + q4];
-    e.x ^= t.x;
-    e.y ^= t.y;
-    e.z ^= t.z;
-    e.w ^= t.w;
+  for (int i = 0; i < N; ++i)
+    if ((mask >> i) & 1) {
+      if (has) {
+        acc = xor3_u32(acc, pend, terms[i]);
+        has = false;
+      } else {
+        pend = terms[i];
+        has = true;
+      }
+    }
+  return has ? acc ^ pend : acc;
+}
+
+// Bitsliced GF(2^8) product of 32 lanes: plane t of a dword holds bit t of
+// the 32 values.  Schoolbook partial products, then x^k (k >= 8) folded back.
+__device__ __forceinline__ void f8_bs_mul(const uint32_t (&A)[8], const uint32_t (&B)[8], uint32_t (&C)[8]) {
+  uint32_t c[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) c[k] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c[i + j] = __builtin_amdgcn_bitop3_b32(A[i], B[j], c[i + j], 0x6a);  // (A & B) ^ c
+  uint32_t hi[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) hi[k] = c[8 + k];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 8; k < 15; ++k) mask |= (uint32_t)((f8_xpow(k) >> t) & 1) << (k - 8);
+    C[t] = xor_masked(c[t], hi, mask);
   }
-  return e;
+}
+
+// Bitsliced extension of one row for this lane's 32 lambdas: 8 byte
+// positions, each one entry (2 x 16 B) of the table, XORed into 8 planes.
+// Lane rotation: step s reads byte position p = s ^ r (r = x_mid mod 8), so
+// the 16 lanes the LDS serves together (8 x_mid x 2 halves) hit 16 distinct
+// bank groups 2 p + h.  Addresses are one v_perm each: byte 0 = (2 p + h) * 16
+// from the lane's offset registers, byte 1 = the row byte, byte 2 = k.
+__device__ __forceinline__ void ext_bs(uint64_t row, const char *__restrict__ tab, const uint32_t (&O)[4],
+                                       uint32_t sel_dw, uint32_t sel_b, uint32_t (&P)[8]) {
+  // byte s of (w0, w1) = byte s ^ r of the row
+  const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+  const uint32_t w0 = __builtin_amdgcn_perm(0u, __builtin_amdgcn_perm(hi, lo, sel_dw), sel_b);
+  const uint32_t w1 = __builtin_amdgcn_perm(0u, __builtin_amdgcn_perm(lo, hi, sel_dw), sel_b);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) P[t] = 0;
+#pragma unroll
+  for (int s = 0; s < 8; s += 2) {
+    uint4 e[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int st = s + u;
+      const uint32_t word = st < 4 ? w0 : w1;
+      const uint32_t base = (4u + (uint32_t)(st & 1)) | ((uint32_t)(st & 3) << 8) | (12u << 24);
+      e[u][0] = *(const uint4 *)(tab + __builtin_amdgcn_perm(O[st >> 1], word, base | (12u << 16)));
+      e[u][1] = *(const uint4 *)(tab + __builtin_amdgcn_perm(O[st >> 1], word, base | (6u << 16)));
+    }
+    P[0] = xor3_u32(P[0], e[0][0].x, e[1][0].x);
+    P[1] = xor3_u32(P[1], e[0][0].y, e[1][0].y);
+    P[2] = xor3_u32(P[2], e[0][0].z, e[1][0].z);
+    P[3] = xor3_u32(P[3], e[0][0].w, e[1][0].w);
+    P[4] = xor3_u32(P[4], e[0][1].x, e[1][1].x);
+    P[5] = xor3_u32(P[5], e[0][1].y, e[1][1].y);
+    P[6] = xor3_u32(P[6], e[0][1].z, e[1][1].z);
+    P[7] = xor3_u32(P[7], e[0][1].w, e[1][1].w);
+  }
 }
 
 constexpr int ZCY_THREADS = 1024;
+constexpr int ZCY_XH_PER_WG = ZCY_THREADS / 32;  // 32 lanes (16 x_mid x 2 halves) per x_hi
 
+// Y_lambda(x_mid, x_hi) = sum_{x_lo} eps(x_lo) ahat_lambda bhat_lambda, bitsliced:
+// lane (x_mid, h) of x_hi computes lambdas 32 h .. 32 h + 31 and writes
+// element 2 h + g (g = 0, 1: planes 4 g .. 4 g + 3) of (x_hi, x_mid).
 __global__ __launch_bounds__(ZCY_THREADS) void zc_y_kernel(const uint64_t *__restrict__ a,
                                                            const uint64_t *__restrict__ b,
                                                            const ZcTables *__restrict__ T, uint64_t n_blocks,
                                                            uint4 *__restrict__ Y) {
-  __shared__ uint4 s_ext[256 * 8 * 4];
-  __shared__ uint16_t s_log[256];
-  __shared__ uint8_t s_exp[3 * 1024 + 4];
-  __shared__ uint32_t s_leps[8];
+  __shared__ uint4 s_ext[2 * 4096];
   const uint32_t tid = threadIdx.x;
   {
-    const uint4 *src = (const uint4 *)T->ext8;
-    for (uint32_t i = tid; i < 256 * 8 * 4; i += ZCY_THREADS) s_ext[i] = src[i];
+    const uint4 *src = (const uint4 *)T->extb;
+    for (uint32_t i = tid; i < 2 * 4096; i += ZCY_THREADS) s_ext[i] = src[i];
   }
-  if (tid < 256) s_log[tid] = T->log8[tid];
-  for (uint32_t i = tid; i < 3 * 1024 + 4; i += ZCY_THREADS) s_exp[i] = T->exp8[i];
-  if (tid < 8) s_leps[tid] = T->log_eps[tid];
   __syncthreads();
-  const uint32_t q4 = tid & 3, xm = (tid >> 2) & 15, xo = tid >> 6, sw = xm & 3;
+  const uint32_t h = tid & 1, xm = (tid >> 1) & 15, xo = tid >> 5, r = xm & 7;
+  uint32_t O[4];  // [off(2 i), off(2 i + 1), 1, 0], off(s) = ((s ^ r) * 2 + h) * 16
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i)
+    O[i] = ((((2 * i) ^ r) * 2 + h) << 4) | (((((2 * i + 1) ^ r) * 2 + h) << 4) << 8) | (1u << 16);
+  const uint32_t sel_dw = (r & 4) ? 0x07060504u : 0x03020100u;
+  uint32_t sel_b = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) sel_b |= (j ^ (r & 3)) << (8 * j);
+  const char *tab = (const char *)s_ext;
   for (uint64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
-    const uint64_t xh = blk * 16 + xo;
+    const uint64_t xh = blk * ZCY_XH_PER_WG + xo;
     const uint64_t row0 = (xh << 7) | ((uint64_t)xm << 3);
-    uint4 y = make_uint4(0, 0, 0, 0);
-#pragma unroll 1
-    for (uint32_t xl = 0; xl < 8; ++xl) {
-      const uint4 ea = ext8(a[row0 + xl], s_ext, q4, sw);
-      const uint4 eb = ext8(b[row0 + xl], s_ext, q4, sw);
-      const uint32_t le = s_leps[xl];
-      y.x ^= f8_prod4(ea.x, eb.x, le, s_log, s_exp);
-      y.y ^= f8_prod4(ea.y, eb.y, le, s_log, s_exp);
-      y.z ^= f8_prod4(ea.z, eb.z, le, s_log, s_exp);
-      y.w ^= f8_prod4(ea.w, eb.w, le, s_log, s_exp);
+    uint32_t Yp[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) Yp[t] = 0;
+#pragma unroll
+    for (int xl = 0; xl < 8; ++xl) {
+      uint32_t A[8], B[8], C[8];
+      ext_bs(a[row0 + xl], tab, O, sel_dw, sel_b, A);
+      ext_bs(b[row0 + xl], tab, O, sel_dw, sel_b, B);
+      f8_bs_mul(A, B, C);
+      // Yp += eps(x_lo) C: plane t gathers the planes s with bit t of eps x^s
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        uint32_t mask = 0;
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) mask |= (uint32_t)((f8_mul_const(zc_eps(xl), (uint8_t)(1u << s2)) >> t) & 1) << s2;
+        Yp[t] = xor_masked(Yp[t], C, mask);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    Y[((xh << 4) + xm) * 4 + q4] = y;
+    uint4 *dst = Y + (((xh << 4) + xm) << 2) + 2 * h;
+    dst[0] = make_uint4(Yp[0], Yp[1], Yp[2], Yp[3]);
+    dst[1] = make_uint4(Yp[4], Yp[5], Yp[6], Yp[7]);
   }
 }
 
@@ -127,7 +221,11 @@ int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int
     ctx->last_error = "zerocheck: m too small";
     return FLOCK_E_SHAPE;
   }
-  const uint64_t n_blocks = n_hi / 16;
+  if (n_hi % ZCY_XH_PER_WG) {
+    ctx->last_error = "zerocheck: x_hi count not a multiple of the workgroup's";
+    return FLOCK_E_SHAPE;
+  }
+  const uint64_t n_blocks = n_hi / ZCY_XH_PER_WG;
   uint4 *Y = (uint4 *)ctx->get_scratch("zc.round1.y", (size_t)n_hi * 16 * 64);
   if (!Y) return FLOCK_E_OOM;
   const unsigned grid = (unsigned)(n_blocks < 1024 ? n_blocks : 1024);
