@@ -103,10 +103,11 @@ __global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *__restrict__ data, c
 constexpr int SHARED_TWIDDLES = 15;  // 2^F - 1 for F = 4
 
 // Twiddle n = 2^J - 1 + C of a shared-table pass (layer offset J, block C):
-// build its tables into buffer n & 1, then every butterfly of it.  Template
-// recursion keeps every register index a compile-time constant.
-template <int F, int THREADS, int J, int C>
-__device__ __forceinline__ void shared_pass_step(f128 (&v)[1 << F], f128 *tabs, const f128 *basis,
+// build its tables into buffer n & 1, then every butterfly of it in each of
+// the wave's G groups, two products in flight per lane.  Template recursion
+// keeps every register index a compile-time constant.
+template <int F, int G, int THREADS, int J, int C>
+__device__ __forceinline__ void shared_pass_step(f128 (&v)[G][1 << F], f128 *tabs, const f128 *basis,
                                                  const FulltabLane &FL, uint32_t tid) {
   constexpr int N = (1 << J) - 1 + C;
   constexpr bool HI = N & 1;
@@ -114,38 +115,37 @@ __device__ __forceinline__ void shared_pass_step(f128 (&v)[1 << F], f128 *tabs, 
   build_fulltab<THREADS>(basis + N * 128, tabs + (HI ? 4096 : 0), tid);
   __syncthreads();
   constexpr int nb = 1 << (F - 1 - J);
-  if constexpr (nb >= 2 && THREADS <= 512) {
-    // two products in flight per lane (enough registers at 8 waves)
+  // product slots (group, butterfly) in pairs
+  constexpr int SLOTS = G * nb;
 #pragma unroll
-    for (int el = 0; el < nb; el += 2) {
-      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-      const int f0 = e0 + 1, f1 = e1 + 1;
+  for (int sl = 0; sl < SLOTS; sl += 2) {
+    const int ga = sl / nb, la = sl % nb, gb = (sl + 1) / nb, lb = (sl + 1) % nb;
+    const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
+    if (sl + 1 < SLOTS) {
+      const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
       f128 p0, p1;
-      f128_mul2_fulltab<HI>(v[e1], v[f1], tbase, FL, p0, p1);
-      const f128 u0 = f128_xor(v[e0], p0), u1 = f128_xor(v[f0], p1);
-      v[e0] = u0;
-      f128_xor_into(v[e1], u0);
-      v[f0] = u1;
-      f128_xor_into(v[f1], u1);
-      __builtin_amdgcn_sched_barrier(0);
+      f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
+      const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
+      v[ga][a0] = u0;
+      f128_xor_into(v[ga][a1], u0);
+      v[gb][b0] = u1;
+      f128_xor_into(v[gb][b1], u1);
+    } else {
+      const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
+      v[ga][a0] = u0;
+      f128_xor_into(v[ga][a1], u0);
     }
-  } else {
-#pragma unroll
-    for (int el = 0; el < nb; ++el) {
-      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-      const f128 u0 = f128_xor(v[e0], f128_mul_fulltab<HI>(v[e1], tbase, FL));
-      v[e0] = u0;
-      f128_xor_into(v[e1], u0);
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    __builtin_amdgcn_sched_barrier(0);
   }
   if constexpr (C + 1 < (1 << J))
-    shared_pass_step<F, THREADS, J, C + 1>(v, tabs, basis, FL, tid);
+    shared_pass_step<F, G, THREADS, J, C + 1>(v, tabs, basis, FL, tid);
   else if constexpr (J + 1 < F)
-    shared_pass_step<F, THREADS, J + 1, 0>(v, tabs, basis, FL, tid);
+    shared_pass_step<F, G, THREADS, J + 1, 0>(v, tabs, basis, FL, tid);
 }
 
-template <int F, int WAVES>
+// G groups per wavefront: a workgroup takes WAVES * G groups that share
+// block B (so 2^s >= WAVES * G), and each table serves all of them.
+template <int F, int WAVES, int G>
 __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__restrict__ data,
                                                                      const f128 *__restrict__ tw, int k, int l,
                                                                      const f128 *__restrict__ src,
@@ -158,11 +158,9 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__res
   const uint32_t tid = threadIdx.x, lane = tid & 63, rot = lane & 15;
   const FulltabLane FL(rot);
   const int s = k - l - F;
-  // wavefront-uniform group index (scalar registers for the addresses)
-  const uint64_t g = (uint64_t)blockIdx.x * WAVES + __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint64_t r = g & ((1ull << s) - 1ull);
-  const uint64_t B = g >> s;  // the same for the whole workgroup
-  const uint64_t base = (B << (k - l)) | r;
+  // wavefront-uniform group indices (scalar registers for the addresses)
+  const uint64_t g0 = ((uint64_t)blockIdx.x * WAVES + __builtin_amdgcn_readfirstlane(tid >> 6)) * G;
+  const uint64_t B = g0 >> s;  // the same for the whole workgroup
   // basis of every twiddle: entry n * 128 + i * 16 + kb = t_n x^(8 kb + i),
   // t_n = twiddle n of the pass in (layer offset j, block c) order,
   // n = 2^j - 1 + c
@@ -174,13 +172,22 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__res
   }
   // The first pass of a commit reads the message itself: position p of the
   // replicated codeword holds message position p & src_pos_mask.
-  f128 v[1 << F];
+  f128 v[G][1 << F];
+  uint64_t base[G];
 #pragma unroll
-  for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_pos_mask) << 6) + lane];
+  for (int gi = 0; gi < G; ++gi) {
+    const uint64_t g = g0 + gi;
+    base[gi] = (B << (k - l)) | (g & ((1ull << s) - 1ull));
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e)
+      v[gi][e] = src[(((base[gi] | ((uint64_t)e << s)) & src_pos_mask) << 6) + lane];
+  }
   __syncthreads();  // basis visible
-  shared_pass_step<F, THREADS, 0, 0>(v, tabs, basis, FL, tid);
+  shared_pass_step<F, G, THREADS, 0, 0>(v, tabs, basis, FL, tid);
 #pragma unroll
-  for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
+  for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e) data[((base[gi] | ((uint64_t)e << s)) << 6) + lane] = v[gi][e];
 }
 
 // Replicate the message into every 2^log_inv_rate sub-block: the exact state
@@ -294,10 +301,15 @@ static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k,
     const uint64_t groups = 1ull << (k - F);
     const f128 *in = src ? src : data;
     const uint64_t mask = src ? src_pos_mask : ~0ull;
-    // 8 waves: 169 VGPRs keep all 16 elements and two products in flight
-    // without spills (16 waves would cap a lane at 128)
-    hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 8>), dim3((unsigned)(groups / 8)), dim3(512), 0, st, data, tw, k,
-                       l, in, mask);
+    // 8 waves (a lane's VGPR budget holds two groups of 16 elements); two
+    // groups per wave when 16 groups share the block, so each table serves
+    // twice the products
+    if (k - l - F >= 4)
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 8, 2>), dim3((unsigned)(groups / 16)), dim3(512), 0, st, data,
+                         tw, k, l, in, mask);
+    else
+      hipLaunchKernelGGL((ntt_pass_shared_kernel<4, 8, 1>), dim3((unsigned)(groups / 8)), dim3(512), 0, st, data,
+                         tw, k, l, in, mask);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
