@@ -39,9 +39,10 @@ __device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
   return x;
 }
 
+// The grid runs about four wavefronts per SIMD, so the kernel may use up to
+// 128 VGPRs to keep loads and table reads in flight.
 template <int NC>
-__global__ __launch_bounds__(256) void bitfold_kernel(const uint32_t *__restrict__ el, int il_log,
-                                                      const f128 *__restrict__ hi0, const f128 *__restrict__ hi1,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void bitfold_kernel(const uint32_t *el, int il_log, const f128 *hi0, const f128 *hi1,
                                                       uint64_t n_ih, uint64_t ih_per_wg, f128 *__restrict__ partial) {
   __shared__ f128 tab[2][NC][256];
   const uint32_t t = threadIdx.x, il = t >> 4, b = t & 15;
@@ -50,12 +51,17 @@ __global__ __launch_bounds__(256) void bitfold_kernel(const uint32_t *__restrict
   uint64_t ih_end = ih_begin + ih_per_wg;
   if (ih_end > n_ih) ih_end = n_ih;
   // Table-build role: NC == 2 -> claim t >> 7 builds entries v, v | 128 (v = t & 127);
-  // NC == 1 -> entry t.
+  // NC == 1 -> entry t.  t >> 7 is constant over a wavefront.
   const uint32_t bk = NC == 2 ? (t >> 7) : 0u;
   const uint32_t bv = NC == 2 ? (t & 127u) : t;
-  const f128 *__restrict__ hb = bk ? hi1 : hi0;
+  // The 8 weights of a group are 32 dwords; lane j (mod 32) of each wavefront
+  // loads dword j and the table build reads them back with v_readlane.  Vector
+  // loads, unlike scalar ones, stay in flight across the LDS barrier.
+  const uint32_t *hw = (const uint32_t *)(bk ? hi1 : hi0) + (t & 31u);
   const uint32_t sh = 8 * (b & 3);
-  const uint64_t dw = b >> 2;
+  // byte offset of this lane's dword within a row of 2^il_log elements
+  const uint32_t lane_off = ((uint32_t)ilg << 4) + 4u * (b >> 2);
+  const uint32_t row_shift = (uint32_t)il_log + 4u;
 
   f128 acc[NC][8];
 #pragma unroll
@@ -63,29 +69,35 @@ __global__ __launch_bounds__(256) void bitfold_kernel(const uint32_t *__restrict
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[k][c] = f128_zero();
 
-  int buf = 0;
-  for (uint64_t ih0 = ih_begin; ih0 < ih_end; ih0 += 8, buf ^= 1) {
-    // (1) this group's table
+  auto load = [&](uint64_t ih, uint32_t (&w)[8], uint32_t &hv) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = *(const uint32_t *)((const char *)el + ((ih + j) << row_shift) + lane_off);
+    hv = hw[4 * ih];
+  };
+  auto group = [&](const uint32_t (&w)[8], uint32_t hv, int buf) {
+    // (1) this group's table: entry bv = sum_j bit_j(bv) h_j
     {
       f128 s = f128_zero();
 #pragma unroll
       for (int j = 0; j < (NC == 2 ? 7 : 8); ++j) {
-        const f128 h = hb[ih0 + j];
         const uint32_t m = 0u - ((bv >> j) & 1u);
-        s.x ^= h.x & m;
-        s.y ^= h.y & m;
-        s.z ^= h.z & m;
-        s.w ^= h.w & m;
+        // (h & m) ^ s in one three-input op per dword
+        s.x = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_readlane(hv, 4 * j + 0), m, s.x, 0x6a);
+        s.y = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_readlane(hv, 4 * j + 1), m, s.y, 0x6a);
+        s.z = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_readlane(hv, 4 * j + 2), m, s.z, 0x6a);
+        s.w = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_readlane(hv, 4 * j + 3), m, s.w, 0x6a);
       }
       tab[buf][bk][bv] = s;
-      if (NC == 2) tab[buf][bk][bv | 128u] = f128_xor(s, hb[ih0 + 7]);
+      if (NC == 2)
+        tab[buf][bk][bv | 128u] =
+            make_uint4(s.x ^ __builtin_amdgcn_readlane(hv, 28), s.y ^ __builtin_amdgcn_readlane(hv, 29),
+                       s.z ^ __builtin_amdgcn_readlane(hv, 30), s.w ^ __builtin_amdgcn_readlane(hv, 31));
     }
     // (2) byte b of the 8 elements
     uint32_t lo = 0, hi = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const uint32_t w = el[((ilg + ((ih0 + j) << il_log)) << 2) + dw];
-      const uint32_t byte = (w >> sh) & 255u;
+      const uint32_t byte = (w[j] >> sh) & 255u;
       if (j < 4)
         lo |= byte << (8 * j);
       else
@@ -93,12 +105,41 @@ __global__ __launch_bounds__(256) void bitfold_kernel(const uint32_t *__restrict
     }
     __syncthreads();
     const uint64_t x = transpose8x8((uint64_t)lo | ((uint64_t)hi << 32));
+    // reads issued in batches of B before their XORs (B = 4 for two weight
+    // vectors, whose 64 accumulator registers leave less room)
+    constexpr int B = NC == 2 ? 4 : 8;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const uint32_t idx = (uint32_t)(x >> (8 * c)) & 255u;
+    for (int k = 0; k < NC; ++k)
 #pragma unroll
-      for (int k = 0; k < NC; ++k) f128_xor_into(acc[k][c], tab[buf][k][idx]);
-    }
+      for (int c0 = 0; c0 < 8; c0 += B) {
+        f128 e[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) e[c] = tab[buf][k][(uint32_t)(x >> (8 * (c0 + c))) & 255u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < B; ++c) f128_xor_into(acc[k][c0 + c], e[c]);
+      }
+  };
+
+  // Two register sets alternate: the next group's loads are issued before
+  // this group's work, so their latency overlaps it.  The last group
+  // reloads itself (in bounds, unused).
+  // The empty asm statements keep the compiler from moving the loads into
+  // the other half of the loop (the input pointers are deliberately not
+  // __restrict__, so the memory clobber applies to them).
+  uint32_t wa[8], wb[8], ha, hb;
+  load(ih_begin, wa, ha);
+  for (uint64_t ih0 = ih_begin;;) {
+    load(ih0 + 8 < ih_end ? ih0 + 8 : ih0, wb, hb);
+    asm volatile("" ::: "memory");
+    group(wa, ha, 0);
+    ih0 += 8;
+    if (ih0 >= ih_end) break;
+    load(ih0 + 8 < ih_end ? ih0 + 8 : ih0, wa, ha);
+    asm volatile("" ::: "memory");
+    group(wb, hb, 1);
+    ih0 += 8;
+    if (ih0 >= ih_end) break;
   }
   f128 *out = partial + (uint64_t)(blockIdx.y * gridDim.x + blockIdx.x) * NC * 2048;
 #pragma unroll
