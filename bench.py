@@ -11,7 +11,11 @@ Prints ONE JSON line.  Workloads:
           The same line carries "batched" (64 instances of 2^10..2^18
           compressions submitted at once) and "served" (open-loop Poisson
           arrivals, 2^12..2^18 compressions, p50/p99 latency) results from
-          the proof server, unless --no-serve.
+          the proof server, unless --no-serve, and "harness": the score of
+          the challenge's own protocol (flock_harness driving a fresh
+          flock_worker process per trial: seed on stdin -> proof file
+          published, median of --harness-runs trials, every proof
+          verified), unless --harness-runs 0.
   commit  isolated PCS commit: 2^22 F128 (seeded SplitMix64) -> rate-1/4
           interleaved additive NTT to 2^24 F128 -> BLAKE3 Merkle over 2^18
           1 KiB leaves.  One step = one commit, inputs resident in HBM.
@@ -307,6 +311,32 @@ def bench_served(args):
     return res
 
 
+def bench_harness(args):
+    """flock_harness (benchmark-tools/harness/src/main.rs protocol) over
+    flock_worker at 2^18 compressions: 1 warm-up + args.harness_runs measured
+    trials, each a fresh worker process timed from the seed line to the
+    published proof, each proof verified (commitment root recomputed, full
+    verifier) by flock_verify_proof.  score = 2^18 / median trial seconds."""
+    import subprocess
+    import tempfile
+    lib = os.path.join(ROOT, "flock-challenge_amd", "lib")
+    with tempfile.TemporaryDirectory(prefix="flock_harness_") as d:
+        score, summary = os.path.join(d, "score.json"), os.path.join(d, "summary.md")
+        cmd = [os.path.join(lib, "flock_harness"), os.path.join(lib, "flock_worker"), os.path.join(d, "scratch"),
+               score, summary, "18", str(omp_threads()), "1", str(args.harness_runs)]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("flock_harness failed: " + r.stderr.strip())
+        with open(score) as f:
+            res = json.load(f)
+    m = res["metrics"]
+    return {"score": res["score"], "unit": "compressions/s", "median_seconds": m["median_seconds"],
+            "p10_seconds": m["p10_seconds"], "p90_p10_latency_ratio": m["p90_p10_latency_ratio"],
+            "trial_seconds": m["trial_seconds"], "measured_runs": m["measured_runs"],
+            "warmup_runs": m["warmup_runs"], "proof_bytes": m["proof_bytes"], "verified": m["verified"],
+            "protocol": "fresh flock_worker per trial, seed on stdin -> proof file published"}
+
+
 def bench_commit(args):
     import numpy as np
 
@@ -385,6 +415,8 @@ def main():
     ap.add_argument("--workers", type=int, default=3, help="concurrent proofs in the batched/served runs")
     ap.add_argument("--rate", type=float, default=100.0, help="served: offered load, requests/s")
     ap.add_argument("--requests", type=int, default=200, help="served: number of requests")
+    ap.add_argument("--harness-runs", type=int, default=5,
+                    help="measured trials of the challenge-protocol harness (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration for profiled runs)")
     args = ap.parse_args()
@@ -398,6 +430,8 @@ def main():
         if not args.no_serve:
             out["batched"] = bench_batched(args)
             out["served"] = bench_served(args)
+        if args.harness_runs > 0:
+            out["harness"] = bench_harness(args)
     print(json.dumps(out))
     return 0
 

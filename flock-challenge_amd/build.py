@@ -2,7 +2,9 @@
 
     python flock-challenge_amd/build.py
 
-Outputs flock-challenge_amd/lib/libflockgpu.so.  Object files go to
+Outputs flock-challenge_amd/lib/libflockgpu.so and, from csrc/cli/, the
+benchmark-protocol programs flock_worker, flock_verify_proof (both linked
+against the library) and flock_harness (no GPU code).  Object files go to
 flock-challenge_amd/lib/obj/ and are rebuilt when their source (or any
 header) is newer.
 """
@@ -59,7 +61,27 @@ def build(force=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(OUT + ".tmp", OUT)
+    build_cli()
     return OUT
+
+
+# (program, source, links against libflockgpu.so)
+CLI = [("flock_worker", "worker.cpp", True), ("flock_verify_proof", "verify_proof.cpp", True),
+       ("flock_harness", "harness.cpp", False)]
+
+
+def build_cli():
+    for name, src, with_lib in CLI:
+        out = os.path.join(LIBDIR, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+               os.path.join(CSRC, "cli", src), "-o", out + ".tmp"]
+        if with_lib:
+            cmd += ["-L" + LIBDIR, "-lflockgpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("build failed: " + name)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":
