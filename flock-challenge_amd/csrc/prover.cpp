@@ -162,6 +162,21 @@ struct Prover {
     FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
     return FLOCK_OK;
   }
+  // Two small device ranges (together at most the landing area) with one
+  // wait for the stream.
+  int d2h2(void *h0, const void *d0, size_t n0, void *h1, const void *d1, size_t n1) {
+    if (n0 + n1 > ctx->landing_bytes) {
+      int rc = d2h(h0, d0, n0);
+      return rc ? rc : d2h(h1, d1, n1);
+    }
+    HostPhase hp(ctx, "wait_d2h");
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, d0, n0, hipMemcpyDeviceToHost, st));
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing + n0, d1, n1, hipMemcpyDeviceToHost, st));
+    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    memcpy(h0, ctx->landing, n0);
+    memcpy(h1, ctx->landing + n0, n1);
+    return FLOCK_OK;
+  }
   // A small result a kernel wrote into the mapped readback area: wait for
   // the stream, then read it.
   int fetch(void *h, size_t n) {
@@ -563,7 +578,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       len = half;
     }
     uint4 h[2];
-    if ((rc = P.d2h(&h[0], A, 16)) || (rc = P.d2h(&h[1], Bv, 16))) return rc;
+    if ((rc = P.d2h2(&h[0], A, 16, &h[1], Bv, 16))) return rc;
     final_a = fv(h[0]);
     final_b = fv(h[1]);
     t.observe(final_a);
@@ -579,6 +594,13 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   B.f(final_a);
   B.f(final_b);
   B.f(final_c);
+
+  // The zerocheck point's part of the ring-switch claims (x_full of claim 1)
+  // and its eq tables' buffers: known now, expanded during the lincheck.
+  const std::vector<F128> xf1(r.begin() + 6, r.end());
+  const size_t N = (size_t)1 << L;
+  void *rs_h1 = P.dev("rs.hi1", (N / 16) * 16), *rs_e1 = P.dev("rs.e1", N * 16);
+  if (!rs_h1 || !rs_e1) return FLOCK_E_OOM;
 
   // ---- lincheck (lincheck.rs:1480-1664)
   F128 lc_rounds[8][2], zpart[64], r_inner_skip, w_claim, r_inner_rest[8];
@@ -598,23 +620,49 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         return rc;
       ctx->csc_uploaded = true;
     }
-    // quirky eq table (lincheck.rs:1127-1142): index i_skip + 64 i_rest
-    std::vector<F128> eqi(1 << 14);
+    // quirky eq table (lincheck.rs:1127-1142): index i_skip + 64 i_rest,
+    // the outer product lam(i_skip) eq(rhos[0..8], i_rest) expanded on the device
     {
-      HostPhase hp(ctx, "lincheck_eq");
       F128 lam[64];
-      lagrange(zz, 0, 64, 0, 64, lam);
-      std::vector<F128> er = eq_table(rhos.data(), 8);
-      for (int ir = 0; ir < 256; ++ir)
-        for (int is = 0; is < 64; ++is) eqi[is + 64 * ir] = lam[is] * er[ir];
+      std::vector<F128> er;
+      {
+        HostPhase hp(ctx, "lincheck_eq");
+        lagrange(zz, 0, 64, 0, 64, lam);
+        er = eq_table(rhos.data(), 8);
+      }
+      void *dlam = P.dev("lc.lam", 64 * 16), *der = P.dev("lc.er", 256 * 16);
+      if (!dlam || !der) return FLOCK_E_OOM;
+      if ((rc = P.h2d(dlam, lam, sizeof lam)) || (rc = P.h2d(der, er.data(), er.size() * 16)) ||
+          (rc = eq_expand_device(ctx, st, dlam, 6, der, 8, deq)))
+        return rc;
     }
-    if ((rc = P.h2d(deq, eqi.data(), eqi.size() * 16))) return rc;
     if ((rc = lc_comb_device(ctx, st, dap, dar, dbp, dbr, deq, alpha, beta, r1cs::Z_CONST, dcomb))) return rc;
     if ((rc = P.eq_device(&rhos[8], m - 14, deqo))) return rc;
     // z_vec[128 c + r] = sum_blk eq_outer[blk] bit_r(16-byte chunk c of block blk)
     if ((rc = bitfold_device(ctx, st, z, 7, 1ull << (m - 14), deqo, nullptr, dzv))) return rc;
+    // comb and z_vec come back through the landing area; the wait covers only
+    // their copies, so the ring-switch eq tables of the zerocheck point
+    // (which do not depend on the lincheck rounds) run on the device while
+    // the host does the rounds.
     std::vector<F128> comb(1 << 14), zv(1 << 14);
-    if ((rc = P.d2h(comb.data(), dcomb, comb.size() * 16)) || (rc = P.d2h(zv.data(), dzv, zv.size() * 16))) return rc;
+    const size_t lc_bytes = comb.size() * 16;
+    hipEvent_t lc_ready = ctx->take_event();
+    if (2 * lc_bytes > ctx->landing_bytes || !lc_ready) {
+      ctx->last_error = "lincheck: landing area or event unavailable";
+      return FLOCK_E_DEVICE;
+    }
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, dcomb, lc_bytes, hipMemcpyDeviceToHost, st));
+    FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing + lc_bytes, dzv, lc_bytes, hipMemcpyDeviceToHost, st));
+    FLOCK_HIP_CHECK(ctx, hipEventRecord(lc_ready, st));
+    if ((rc = P.eq_device(&xf1[5], L - 4, rs_h1)) || (rc = P.eq_device(&xf1[1], L, rs_e1))) return rc;
+    {
+      HostPhase hp(ctx, "wait_d2h");
+      const hipError_t e = hipEventSynchronize(lc_ready);
+      ctx->event_pool.push_back(lc_ready);
+      FLOCK_HIP_CHECK(ctx, e);
+    }
+    memcpy(comb.data(), ctx->landing, lc_bytes);
+    memcpy(zv.data(), ctx->landing + lc_bytes, lc_bytes);
     HostPhase hp(ctx, "lincheck_rounds");
     size_t len = 1 << 14;
     F128 rr[8];
@@ -656,23 +704,23 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   std::vector<F128> xf[2];
   xf[0].assign(r_inner_rest, r_inner_rest + 8);
   xf[0].insert(xf[0].end(), rhos.begin() + 8, rhos.end());
-  xf[1].assign(r.begin() + 6, r.end());
+  xf[1] = xf1;
 
   // ---- PCS open: ring switch (ring_switch.rs:2708-2780, 3040-3117)
   t.label("flock-pcs-open-batch-v0");
-  const size_t N = (size_t)1 << L;
-  void *e0 = P.dev("rs.e0", N * 16), *e1 = P.dev("rs.e1", N * 16), *cb = P.dev("lig.cb", N * 16);
+  void *e0 = P.dev("rs.e0", N * 16), *e1 = rs_e1, *cb = P.dev("lig.cb", N * 16);
   void *dtab = P.dev("rs.tab", 2 * 32 * 16 * 16);
   if (!e0 || !e1 || !cb || !dtab) return FLOCK_E_OOM;
   F128 shv[2][128], sclaim[2], eqr[2][128];
   {
     // s_hat_v[k][r] = sum_i bit_r(z[i]) eq_k(i), eq_k(i) = lo_k(i mod 16) hi_k(i / 16)
-    void *h0 = P.dev("rs.hi0", (N / 16) * 16), *h1 = P.dev("rs.hi1", (N / 16) * 16);
+    void *h0 = P.dev("rs.hi0", (N / 16) * 16), *h1 = rs_h1;
     void *dq = P.dev("rs.q", 2 * 2048 * 16);
     if (!h0 || !h1 || !dq) return FLOCK_E_OOM;
-    if ((rc = P.eq_device(&xf[0][5], L - 4, h0)) || (rc = P.eq_device(&xf[1][5], L - 4, h1))) return rc;
+    // (the tables of claim 1, h1 and e1, were expanded during the lincheck)
+    if ((rc = P.eq_device(&xf[0][5], L - 4, h0))) return rc;
     if ((rc = bitfold_device(ctx, st, z, 4, N / 16, h0, h1, dq))) return rc;
-    if ((rc = P.eq_device(&xf[0][1], L, e0)) || (rc = P.eq_device(&xf[1][1], L, e1))) return rc;
+    if ((rc = P.eq_device(&xf[0][1], L, e0))) return rc;
     std::vector<F128> q(2 * 2048);
     if ((rc = P.d2h(q.data(), dq, q.size() * 16))) return rc;
     for (int k = 0; k < 2; ++k) {
