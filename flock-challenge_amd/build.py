@@ -40,7 +40,8 @@ def _compile(src):
     if src.endswith(".hip"):
         cmd = [HIPCC] + HIPFLAGS + COMMON + ["-c", src, "-o", obj]
     else:
-        cmd = [HIPCC] + COMMON + ["-c", src, "-o", obj]
+        # host code: carry-less multiply and SSE4.1 inline everywhere
+        cmd = [HIPCC] + COMMON + ["-mpclmul", "-msse4.1", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -141,6 +141,14 @@ uint8_t *Ctx::pinned_alloc(size_t n) {
   return pinned + at;
 }
 
+HostPool &Ctx::host_pool() {
+  if (!pool) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    pool.reset(new HostPool(hw >= 8 ? 7 : (hw > 1 ? (int)hw - 1 : 0)));
+  }
+  return *pool;
+}
+
 Ctx::~Ctx() {
   hipSetDevice(device);
   for (auto &e : phase_events) {

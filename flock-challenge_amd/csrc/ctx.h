@@ -7,11 +7,13 @@
 
 #include <chrono>
 #include <map>
+#include <memory>
 #include <set>
 #include <string>
 #include <vector>
 
 #include "host_field.h"
+#include "host_pool.h"
 
 namespace flock {
 
@@ -70,6 +72,9 @@ struct Ctx {
   void *get_scratch(const std::string &name, size_t bytes);
   const std::vector<F128> &host_twiddles(int dim);
   const F128 *device_twiddles(int dim);
+  // Host threads for the prover's host-side loops, started on first use.
+  std::unique_ptr<HostPool> pool;
+  HostPool &host_pool();
   ~Ctx();
 };
 

@@ -664,27 +664,58 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     memcpy(comb.data(), ctx->landing, lc_bytes);
     memcpy(zv.data(), ctx->landing + lc_bytes, lc_bytes);
     HostPhase hp(ctx, "lincheck_rounds");
-    size_t len = 1 << 14;
+    // Round k's message (binding the top bit) over the arrays as folded by
+    // rounds 0..k-1.  The fold of round k - 1 is fused into round k's pass:
+    // position i of the pass computes the folded values at i and i + half
+    // from the unfolded ones at i, i + half, i + len, i + half + len and
+    // writes them in place (no position is read by another chunk).  Message
+    // products accumulate unreduced (reduction is linear) per chunk.
+    HostPool &pool = ctx->host_pool();
+    size_t len = 1 << 14;  // length after the folds applied so far
     F128 rr[8];
     for (int rnd = 0; rnd < 8; ++rnd) {
-      size_t half = len / 2;
-      F128 e1, einf;
-      for (size_t i = 0; i < half; ++i) {
-        e1 += comb[i + half] * zv[i + half];
-        einf += (comb[i] + comb[i + half]) * (zv[i] + zv[i + half]);
+      const size_t half = len / 2;
+      const bool fold = rnd > 0;
+      const F128 rp = fold ? rr[rnd - 1] : F128();
+      // chunks of at least 512 positions: smaller rounds run on this thread
+      const size_t grain = std::max<size_t>(512, (half + 63) / 64);
+      const size_t nch = (half + grain - 1) / grain;
+      std::vector<F256> p1(nch), pi(nch);
+      pool.parallel_for(half, grain, [&](size_t c, size_t b, size_t e) {
+        F256 a1, ai;
+        for (size_t i = b; i < e; ++i) {
+          F128 c0 = comb[i], c1 = comb[i + half], z0 = zv[i], z1 = zv[i + half];
+          if (fold) {
+            c0 += rp * (comb[i + len] + c0);
+            c1 += rp * (comb[i + half + len] + c1);
+            z0 += rp * (zv[i + len] + z0);
+            z1 += rp * (zv[i + half + len] + z1);
+            comb[i] = c0;
+            comb[i + half] = c1;
+            zv[i] = z0;
+            zv[i + half] = z1;
+          }
+          a1 ^= mul_unreduced(c1, z1);
+          ai ^= mul_unreduced(c0 + c1, z0 + z1);
+        }
+        p1[c] = a1;
+        pi[c] = ai;
+      });
+      F256 s1, si;
+      for (size_t c = 0; c < nch; ++c) {
+        s1 ^= p1[c];
+        si ^= pi[c];
       }
+      const F128 e1 = reduce(s1), einf = reduce(si);
       lc_rounds[rnd][0] = e1;
       lc_rounds[rnd][1] = einf;
       t.observe(e1);
       t.observe(einf);
-      F128 rv = t.sample();
-      rr[rnd] = rv;
-      for (size_t i = 0; i < half; ++i) {
-        comb[i] = comb[i] + rv * (comb[i + half] + comb[i]);
-        zv[i] = zv[i] + rv * (zv[i + half] + zv[i]);
-      }
+      rr[rnd] = t.sample();
       len = half;
     }
+    // the last round's fold (only z_vec is needed further)
+    for (size_t i = 0; i < len; ++i) zv[i] += rr[7] * (zv[i + len] + zv[i]);
     for (int i = 0; i < 64; ++i) zpart[i] = zv[i];
     t.observe_slice(zpart, 64);
     r_inner_skip = t.sample();
