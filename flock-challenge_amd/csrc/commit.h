@@ -21,8 +21,10 @@ int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t le
 // Fill out with out_len/msg_len copies of msg (F128 counts, powers of two).
 int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len_f128, void *out,
                      uint64_t out_len_f128);
-// Full commit; writes the root to host memory and synchronizes `st`.
+// Full commit; copies the root to host memory `root` and synchronizes `st`,
+// or, when root_ready is given, records it after the copy instead (then
+// `root` must be pinned memory, valid once the event has completed).
 int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
-                  uint8_t root[32], hipStream_t st);
+                  uint8_t root[32], hipStream_t st, hipEvent_t root_ready = nullptr);
 
 }  // namespace flock

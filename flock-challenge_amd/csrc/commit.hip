@@ -380,7 +380,7 @@ int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t le
 }
 
 int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
-                  uint8_t root[32], hipStream_t st) {
+                  uint8_t root[32], hipStream_t st, hipEvent_t root_ready) {
   if (p->merkle_hash != 1) {
     ctx->last_error = "only the BLAKE3 Merkle hash is implemented";
     return FLOCK_E_ARG;
@@ -412,7 +412,11 @@ int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_
   const uint64_t root_idx = (2ull << k) - 2ull;
   FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(root, (const uint8_t *)d_tree + 32 * root_idx, 32, hipMemcpyDeviceToHost, st));
   ph.end();
-  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  if (root_ready) {
+    FLOCK_HIP_CHECK(ctx, hipEventRecord(root_ready, st));
+  } else {
+    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  }
   return FLOCK_OK;
 }
 

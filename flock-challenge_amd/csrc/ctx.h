@@ -51,6 +51,8 @@ struct Ctx {
   std::map<std::string, DeviceBuffer> scratch;
   // The R1CS transposes live in named scratch buffers once uploaded.
   bool csc_uploaded = false;
+  // The zerocheck round-1 constant tables live in a named scratch buffer once uploaded.
+  bool zc_tables_uploaded = false;
   // Bytes of the last proof made on this context.
   std::vector<uint8_t> last_proof;
   // Pinned host staging: an upload ring (asynchronous host-to-device copies

@@ -16,11 +16,12 @@ int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, 
 namespace flock {
 size_t zc_tables_bytes();
 int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out);
-// Zerocheck round 1, product part: d_q[4 x_mid + q4][8 j + t] (64 x 128
-// F128) = sum_{x_hi} Ehi(x_hi) bit_t(Y_{16 q4 + j}(x_mid, x_hi)); P^AB follows
-// on the host.
-int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int m, const void *d_tables,
-                     const void *d_ehi, void *d_q);
+// Zerocheck round 1, product part, in two steps: zc_y_device computes
+// Y_j(x_mid, x_hi) from the rows (no challenge involved), zc_round1_device
+// then d_q[4 x_mid + q4][8 j + t] (64 x 128 F128) = sum_{x_hi} Ehi(x_hi)
+// bit_t(Y_{16 q4 + j}(x_mid, x_hi)); P^AB follows on the host.
+int zc_y_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int m, const void *d_tables);
+int zc_round1_device(Ctx *ctx, hipStream_t st, int m, const void *d_ehi, void *d_q);
 // Q_k[il][r] = sum_ih bit_r(elem[il + 2^il_log ih]) hi_k[ih] over 128-bit
 // elements (2^il_log x n_ih of them); hi1 may be null (one claim).  out:
 // [2^(il_log-4)][n_claims][16][128] F128.

@@ -409,8 +409,31 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   const size_t npos0 = (size_t)1 << k0;
   void *cw0 = P.dev("c.codeword", npos0 * 64 * 16), *tree0 = P.dev("c.tree", (2 * npos0 - 1) * 32);
   if (!cw0 || !tree0) return FLOCK_E_OOM;
+  // The root comes back through the landing area under an event; zerocheck
+  // round 1's products (which depend on the rows only) are queued behind it,
+  // so the device works while the host builds the transcript and derives the
+  // challenges.
   uint8_t root[32];
-  if ((rc = commit_device(ctx, &pp, z, cw0, tree0, root, st))) return rc;
+  hipEvent_t root_ready = ctx->take_event();
+  if (!ctx->landing || !root_ready) {
+    ctx->last_error = "prove: landing area or event unavailable";
+    return FLOCK_E_DEVICE;
+  }
+  if ((rc = commit_device(ctx, &pp, z, cw0, tree0, ctx->landing, st, root_ready))) return rc;
+  void *dzt = P.dev("zc.tables", sizeof(ZcTables));
+  if (!dzt) return FLOCK_E_OOM;
+  if (!ctx->zc_tables_uploaded) {
+    if ((rc = P.h2d(dzt, &zc_tables(), sizeof(ZcTables)))) return rc;
+    ctx->zc_tables_uploaded = true;
+  }
+  if ((rc = zc_y_device(ctx, st, a, b, m, dzt))) return rc;
+  {
+    HostPhase hp(ctx, "wait_d2h");
+    const hipError_t e = hipEventSynchronize(root_ready);
+    ctx->event_pool.push_back(root_ready);
+    FLOCK_HIP_CHECK(ctx, e);
+  }
+  memcpy(root, ctx->landing, 32);
 
   Transcript t(domain, domain_len);
   {
@@ -445,14 +468,12 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   F128 r1ab[64], r1c[64];
   {
     ScopedPhase ph(ctx, "zc_round1_all", st);
-    void *dt = P.dev("zc.tables", sizeof(ZcTables));
     void *dehi = P.dev("zc.ehi", ((size_t)1 << (m - 13)) * 16);
     void *dout = P.dev("zc.r1q", 64 * 128 * 16);
     void *dhic = P.dev("zc.hic", ((size_t)1 << (m - 11)) * 16), *dq = P.dev("zc.cq", 2048 * 16);
-    if (!dt || !dehi || !dout || !dhic || !dq) return FLOCK_E_OOM;
-    if ((rc = P.h2d(dt, &zc_tables(), sizeof(ZcTables)))) return rc;
+    if (!dehi || !dout || !dhic || !dq) return FLOCK_E_OOM;
     if ((rc = P.eq_device(&r[13], m - 13, dehi))) return rc;
-    if ((rc = zc_round1_device(ctx, st, a, b, m, dt, dehi, dout))) return rc;
+    if ((rc = zc_round1_device(ctx, st, m, dehi, dout))) return rc;
     // P^C(lambda_j) = sum_s phi8(M[j][s]) C_s with C_s = sum_x eq(x) bit_s(c[x]);
     // elements are row pairs (x = 2e + h), e = il + 16 ih.
     if ((rc = P.eq_device(&r[11], m - 11, dhic))) return rc;

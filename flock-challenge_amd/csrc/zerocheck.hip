@@ -213,8 +213,9 @@ int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, ui
   return FLOCK_OK;
 }
 
-int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int m, const void *d_tables,
-                     const void *d_ehi, void *d_q) {
+// Y of every (x_mid, x_hi): depends on the rows only, not on the challenges,
+// so it can run while the host derives them.
+int zc_y_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int m, const void *d_tables) {
   ScopedPhase ph(ctx, "zc_round1", st);
   const uint64_t n_hi = 1ull << (m - 13);
   if (n_hi < 16) {
@@ -232,7 +233,16 @@ int zc_round1_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, int
   hipLaunchKernelGGL(zc_y_kernel, dim3(grid), dim3(ZCY_THREADS), 0, st, (const uint64_t *)a, (const uint64_t *)b,
                      (const ZcTables *)d_tables, n_blocks, Y);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  // Q[4 x_mid + q4][8 j + t] = sum_{x_hi} Ehi(x_hi) bit_t(Y_{16 q4 + j}(x_mid, x_hi))
+  return FLOCK_OK;
+}
+
+// Q[4 x_mid + q4][8 j + t] = sum_{x_hi} Ehi(x_hi) bit_t(Y_{16 q4 + j}(x_mid, x_hi)),
+// from the Y of zc_y_device.
+int zc_round1_device(Ctx *ctx, hipStream_t st, int m, const void *d_ehi, void *d_q) {
+  ScopedPhase ph(ctx, "zc_round1", st);
+  const uint64_t n_hi = 1ull << (m - 13);
+  uint4 *Y = (uint4 *)ctx->get_scratch("zc.round1.y", (size_t)n_hi * 16 * 64);
+  if (!Y) return FLOCK_E_OOM;
   return bitfold_device(ctx, st, Y, 6, n_hi, d_ehi, nullptr, d_q);
 }
 
