@@ -68,6 +68,11 @@ struct Ctx {
   uint8_t *readback = nullptr;
   void *readback_dev = nullptr;
   static constexpr size_t READBACK_BYTES = 4096;
+  // A sequence number the device publishes in the readback area (at this
+  // offset) once the work queued before it has finished: the host polls it
+  // instead of synchronizing the stream, which wakes several microseconds later.
+  static constexpr size_t READBACK_FLAG = READBACK_BYTES - 64;
+  uint32_t readback_seq = 0;
   uint8_t *pinned_alloc(size_t n);  // nullptr when the ring is full (or unavailable)
   void pinned_reset() { pinned_off = 0; }
 

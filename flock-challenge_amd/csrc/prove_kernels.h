@@ -15,6 +15,8 @@ int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, 
 
 namespace flock {
 size_t zc_tables_bytes();
+// Write seq to the mapped host word d_flag after the stream's earlier work.
+int signal_device(Ctx *ctx, hipStream_t st, void *d_flag, uint32_t seq);
 int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out);
 // Zerocheck round 1, product part, in two steps: zc_y_device computes
 // Y_j(x_mid, x_hi) from the rows (no challenge involved), zc_round1_device

@@ -148,13 +148,35 @@ struct Prover {
 
   void *dev(const char *name, size_t bytes) { return ctx->get_scratch(name, bytes); }
 
+  // Wait until everything queued on the stream so far has finished: a
+  // signal kernel publishes a new sequence number in the mapped readback
+  // area and the host polls it (with the stream's error state checked now
+  // and then, so a failed kernel ends the wait).
+  int wait_signal() {
+    const uint32_t seq = ++ctx->readback_seq;
+    if (int rc = signal_device(ctx, st, (uint8_t *)ctx->readback_dev + Ctx::READBACK_FLAG, seq)) return rc;
+    const uint32_t *flag = (const uint32_t *)(ctx->readback + Ctx::READBACK_FLAG);
+    for (uint32_t spins = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; ++spins) {
+      if ((spins & 4095) == 0) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e != hipSuccess && e != hipErrorNotReady) FLOCK_HIP_CHECK(ctx, e);
+        if (e == hipSuccess && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
+          ctx->last_error = "readback: the stream finished without publishing its signal";
+          return FLOCK_E_DEVICE;
+        }
+      }
+      __builtin_ia32_pause();
+    }
+    return FLOCK_OK;
+  }
+
   // Device-to-host read; waits for the stream (the host needs the bytes).
   // Small reads land in pinned memory first.
   int d2h(void *h, const void *d, size_t n) {
     HostPhase hp(ctx, "wait_d2h");
     if (n <= ctx->landing_bytes) {
       FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, d, n, hipMemcpyDeviceToHost, st));
-      FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      if (int rc = wait_signal()) return rc;
       memcpy(h, ctx->landing, n);
       return FLOCK_OK;
     }
@@ -172,7 +194,7 @@ struct Prover {
     HostPhase hp(ctx, "wait_d2h");
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, d0, n0, hipMemcpyDeviceToHost, st));
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing + n0, d1, n1, hipMemcpyDeviceToHost, st));
-    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    if (int rc = wait_signal()) return rc;
     memcpy(h0, ctx->landing, n0);
     memcpy(h1, ctx->landing + n0, n1);
     return FLOCK_OK;
@@ -181,7 +203,7 @@ struct Prover {
   // the stream, then read it.
   int fetch(void *h, size_t n) {
     HostPhase hp(ctx, "wait_d2h");
-    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    if (int rc = wait_signal()) return rc;
     memcpy(h, ctx->readback, n);
     return FLOCK_OK;
   }

@@ -618,6 +618,22 @@ int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out) {
   return FLOCK_OK;
 }
 
+// ---------------------------------------------------------------- readback signal
+// Publish seq to the host: the stream runs this after every earlier kernel
+// and copy has finished, so their results in host memory are complete.
+__global__ void signal_kernel(uint32_t *flag, uint32_t seq) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+int signal_device(Ctx *ctx, hipStream_t st, void *d_flag, uint32_t seq) {
+  hipLaunchKernelGGL(signal_kernel, dim3(1), dim3(64), 0, st, (uint32_t *)d_flag, seq);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
 // ---------------------------------------------------------------- gathers
 __global__ void gather_kernel(const uint4 *__restrict__ src, const uint64_t *__restrict__ idx, uint32_t n,
                               uint32_t unit_u4, uint4 *__restrict__ out) {
