@@ -37,12 +37,11 @@ int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void
 int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2,
                     void *E_next);
 // Zerocheck multilinear round 0 and its fold, reading the packed rows a, b and
-// evaluating A = L(row) through tables of L: 2048-entry byte tables d_lw8
-// (entry v * 8 + p) for the round, 256-entry nibble tables d_lw / d_lwr
-// (= rho L, entry v * 16 + p) for the fold.
+// evaluating A = L(row) through 2048-entry byte tables (entry v * 8 + p):
+// d_lw8 of L for the round, d_lw8 and d_lwr8 (of rho L) for the fold.
 int zc_round0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *E,
                      uint64_t half, void *out2, void *E_next);
-int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *d_lwr,
+int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *d_lwr8,
                     uint64_t half, void *A2, void *B2);
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3);

@@ -560,32 +560,19 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     void *A = P.dev("zc.A", rows * 8), *Bv = P.dev("zc.B", rows * 8);
     void *A2 = P.dev("zc.A2", rows * 4), *B2 = P.dev("zc.B2", rows * 4);
     void *E = P.dev("zc.E", rows * 8), *E2 = P.dev("zc.E2", rows * 4);
-    void *dlw = P.dev("zc.lw", 256 * 16), *dlwr = P.dev("zc.lwr", 256 * 16);
-    void *dlw8 = P.dev("zc.lw8", 2048 * 16);
-    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw || !dlwr || !dlw8) return FLOCK_E_OOM;
-    // nibble tables: lw[v * 16 + p] = sum_{t < 4, bit t of v} L_{4p + t}(z)
-    auto nibble_tables = [](const F128 *w, F128 *out) {
-      for (int v = 0; v < 16; ++v)
-        for (int p = 0; p < 16; ++p) {
-          F128 acc;
-          for (int t = 0; t < 4; ++t)
-            if ((v >> t) & 1) acc += w[4 * p + t];
-          out[v * 16 + p] = acc;
-        }
+    void *dlw8 = P.dev("zc.lw8", 2048 * 16), *dlwr8 = P.dev("zc.lwr8", 2048 * 16);
+    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw8 || !dlwr8) return FLOCK_E_OOM;
+    // byte tables: out[v * 8 + p] = sum_{t < 8, bit t of v} w[8 p + t], each
+    // entry one XOR from the entry without v's lowest set bit
+    auto byte_tables = [](const F128 *w, F128 *out) {
+      for (int p = 0; p < 8; ++p) {
+        out[p] = F128();
+        for (int v = 1; v < 256; ++v) out[v * 8 + p] = out[(v & (v - 1)) * 8 + p] + w[8 * p + __builtin_ctz(v)];
+      }
     };
-    F128 lw[256];
-    nibble_tables(ws, lw);
-    if ((rc = P.h2d(dlw, lw, sizeof lw))) return rc;
     {
-      // byte tables: lw8[v * 8 + p] = sum_{t < 8, bit t of v} L_{8p + t}(z)
-      std::vector<F128> lw8(2048);
-      for (int v = 0; v < 256; ++v)
-        for (int p = 0; p < 8; ++p) {
-          F128 acc;
-          for (int t = 0; t < 8; ++t)
-            if ((v >> t) & 1) acc += ws[8 * p + t];
-          lw8[v * 8 + p] = acc;
-        }
+      std::vector<F128> lw8(2048);  // of L_s(z)
+      byte_tables(ws, lw8.data());
       if ((rc = P.h2d(dlw8, lw8.data(), lw8.size() * 16))) return rc;
     }
     size_t len = rows;
@@ -608,11 +595,12 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       F128 rho = t.sample();
       rhos.push_back(rho);
       if (i == 0) {
-        F128 wr[64], lwr[256];
+        F128 wr[64];  // rho L_s(z)
         for (int q = 0; q < 64; ++q) wr[q] = rho * ws[q];
-        nibble_tables(wr, lwr);
-        if ((rc = P.h2d(dlwr, lwr, sizeof lwr))) return rc;
-        if ((rc = zc_fold0_device(ctx, st, a, b, dlw, dlwr, half, A, Bv))) return rc;
+        std::vector<F128> lwr8(2048);
+        byte_tables(wr, lwr8.data());
+        if ((rc = P.h2d(dlwr8, lwr8.data(), lwr8.size() * 16))) return rc;
+        if ((rc = zc_fold0_device(ctx, st, a, b, dlw8, dlwr8, half, A, Bv))) return rc;
       } else {
         if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2))) return rc;
         std::swap(A, A2);

@@ -134,23 +134,10 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
 // After round 1 the multilinear tables are A[x] = sum_s bit_s(a[x]) L_s(z)
 // (fold_at_z_naive, zerocheck/multilinear.rs:598-625), a GF(2)-linear image
 // of the 64-bit row.  Round 0 and its fold read the rows and evaluate that
-// map on the fly instead of materializing A and B: nibble tables
-// T[v * 16 + p] = sum_{t < 4, bit t of v} L_{4p + t}(z) (4 KiB), read in a
-// lane-dependent order p = step ^ (lane mod 16) so lanes hit distinct bank
-// groups (entry (v, p) sits in bank group p).
-__device__ __forceinline__ f128 lagrange_row(uint64_t row, const f128 *__restrict__ T, uint32_t rot) {
-  const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
-  f128 acc = f128_zero();
-#pragma unroll
-  for (uint32_t step = 0; step < 16; step += 2) {
-    const uint32_t p0 = step ^ rot, p1 = (step + 1) ^ rot;
-    const uint32_t v0 = __builtin_amdgcn_ubfe(p0 < 8 ? lo : hi, 4 * (p0 & 7), 4);
-    const uint32_t v1 = __builtin_amdgcn_ubfe(p1 < 8 ? lo : hi, 4 * (p1 & 7), 4);
-    acc = f128_xor3(acc, T[v0 * 16 + p0], T[v1 * 16 + p1]);
-  }
-  return acc;
-}
-
+// map on the fly instead of materializing A and B, through byte tables
+// T[v * 8 + p] = sum_{t < 8, bit t of v} L_{8p + t}(z) (32 KiB), read in a
+// lane-dependent order p = step ^ (lane mod 8).
+//
 // Round 0 uses 8-wavefront workgroups: 8 x 16 KiB of per-lane product
 // tables plus one shared 32 KiB byte table of L, entry (v, p) at v * 8 + p
 // (byte position p = step ^ (lane mod 8), 8 lookups per row); the reduction
@@ -222,27 +209,38 @@ int zc_round0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, con
 }
 
 // A'[x] = A[2x] + rho (A[2x] + A[2x+1]) with A = L(rows): by linearity
-// A'[x] = L(a[2x]) + (rho L)(a[2x] ^ a[2x+1]); lwr holds the tables of rho L.
-__global__ __launch_bounds__(256) void zc_fold0_kernel(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
-                                                       const f128 *__restrict__ lw, const f128 *__restrict__ lwr,
-                                                       uint64_t half, f128 *__restrict__ A2, f128 *__restrict__ B2) {
-  __shared__ f128 T[256], R[256];
-  T[threadIdx.x] = lw[threadIdx.x];
-  R[threadIdx.x] = lwr[threadIdx.x];
+// A'[x] = L(a[2x]) + (rho L)(a[2x] ^ a[2x+1]), each map evaluated through its
+// byte tables (entry (v, p) at v * 8 + p, as in round 0): 8 lookups per row.
+// Two 32 KiB tables per workgroup leave room for two workgroups per CU.
+constexpr int ZF0_THREADS = 512;
+
+__global__ __launch_bounds__(ZF0_THREADS) void zc_fold0_kernel(const uint64_t *__restrict__ a,
+                                                               const uint64_t *__restrict__ b,
+                                                               const f128 *__restrict__ lw8,
+                                                               const f128 *__restrict__ lwr8, uint64_t half,
+                                                               f128 *__restrict__ A2, f128 *__restrict__ B2) {
+  __shared__ f128 T[2048], R[2048];
+  for (uint32_t i = threadIdx.x; i < 2048; i += ZF0_THREADS) {
+    T[i] = lw8[i];
+    R[i] = lwr8[i];
+  }
   __syncthreads();
-  const uint32_t rot = threadIdx.x & 15;
-  for (uint64_t x = (uint64_t)blockIdx.x * 256 + threadIdx.x; x < half; x += (uint64_t)gridDim.x * 256) {
-    const uint64_t a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
-    A2[x] = f128_xor(lagrange_row(a0, T, rot), lagrange_row(a0 ^ a1, R, rot));
-    B2[x] = f128_xor(lagrange_row(b0, T, rot), lagrange_row(b0 ^ b1, R, rot));
+  const uint32_t rot8 = threadIdx.x & 7;
+  const ulonglong2 *a2 = (const ulonglong2 *)a, *b2 = (const ulonglong2 *)b;
+  for (uint64_t x = (uint64_t)blockIdx.x * ZF0_THREADS + threadIdx.x; x < half;
+       x += (uint64_t)gridDim.x * ZF0_THREADS) {
+    const ulonglong2 av = a2[x], bv = b2[x];
+    A2[x] = f128_xor(lagrange_row8(av.x, T, rot8), lagrange_row8(av.x ^ av.y, R, rot8));
+    B2[x] = f128_xor(lagrange_row8(bv.x, T, rot8), lagrange_row8(bv.x ^ bv.y, R, rot8));
   }
 }
 
-int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *d_lwr,
+int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *d_lwr8,
                     uint64_t half, void *A2, void *B2) {
   ScopedPhase ph(ctx, "fold_pair", st);
-  hipLaunchKernelGGL(zc_fold0_kernel, dim3(grid_for(half, 256, 8192)), dim3(256), 0, st, (const uint64_t *)a,
-                     (const uint64_t *)b, (const f128 *)d_lw, (const f128 *)d_lwr, half, (f128 *)A2, (f128 *)B2);
+  hipLaunchKernelGGL(zc_fold0_kernel, dim3(grid_for(half, ZF0_THREADS, 512)), dim3(ZF0_THREADS), 0, st,
+                     (const uint64_t *)a, (const uint64_t *)b, (const f128 *)d_lw8, (const f128 *)d_lwr8, half,
+                     (f128 *)A2, (f128 *)B2);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
