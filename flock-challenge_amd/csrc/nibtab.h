@@ -102,4 +102,22 @@ __device__ __forceinline__ void build_nibtab(f128 t, char *__restrict__ lds, uin
   }
 }
 
+// The same 512 entries built by a workgroup of 256 threads: thread
+// (k = tid mod 32, g = tid / 32) writes entries n = 2g and 2g + 1 of
+// position k (the 16 lanes of a store hit distinct bank groups).
+template <uint32_t TAB>
+__device__ __forceinline__ void build_nibtab256(f128 t, char *__restrict__ lds, uint32_t tid) {
+  const uint32_t k = tid & 31u, g = tid >> 5;
+  const f128 b0 = f128_mul_xpow(t, 4u * k), b1 = f128_mulx(b0), b2 = f128_mulx(b1), b3 = f128_mulx(b2);
+  const uint32_t m1 = 0u - (g & 1u), m2 = 0u - ((g >> 1) & 1u), m3 = 0u - ((g >> 2) & 1u);
+  f128 e;  // entry 2g = g0 b1 + g1 b2 + g2 b3
+  e.x = xor3_u32(b1.x & m1, b2.x & m2, b3.x & m3);
+  e.y = xor3_u32(b1.y & m1, b2.y & m2, b3.y & m3);
+  e.z = xor3_u32(b1.z & m1, b2.z & m2, b3.z & m3);
+  e.w = xor3_u32(b1.w & m1, b2.w & m2, b3.w & m3);
+  char *dst = lds + TAB + (k >> 4) * 4096u + (k & 15u) * 16u + 2u * g * 256u;
+  *(f128 *)dst = e;
+  *(f128 *)(dst + 256) = f128_xor(e, b0);
+}
+
 }  // namespace flock

@@ -36,6 +36,13 @@ int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void
 // eq table into E_next (half / 2 entries) unless it is null.
 int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2,
                     void *E_next);
+// The same round with the eq table split as Elo (x) Ehi, Elo over the low
+// ZS_LOG variables (2^ZS_LOG entries), Ehi over the rest (half >> ZS_LOG
+// entries, at least 2); writes Ehi folded for the next round into Ehi_next
+// (half >> (ZS_LOG + 1) entries) unless it is null.
+constexpr int ZS_LOG = 10;
+int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
+                          uint64_t half, void *out2, void *Ehi_next);
 // Zerocheck multilinear round 0 and its fold, reading the packed rows a, b and
 // evaluating A = L(row) through 2048-entry byte tables (entry v * 8 + p):
 // d_lw8 of L for the round, d_lw8 and d_lwr8 (of rho L) for the fold.

@@ -576,16 +576,50 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       if ((rc = P.h2d(dlw8, lw8.data(), lw8.size() * 16))) return rc;
     }
     size_t len = rows;
+    // Round i's eq table is E_i = eq(r[7 + i ..]) over n - 1 - i variables.
+    // Rounds 1 .. n_split (half >= 2^(ZS_LOG + 1)) use it split as
+    // Elo_i = eq(r[7 + i .. 7 + i + ZS_LOG - 1]) times Ehi_i = eq(r[7 + i + ZS_LOG ..]):
+    // every Elo_i is computed on the host while round 0 runs, Ehi_1 is expanded
+    // on the device and each split round folds Ehi for the next.  Round 0 and
+    // the rounds after n_split use the whole table, folded by each round.
+    const int n_split = std::max(0, n - 2 - ZS_LOG);
+    void *dElo = nullptr, *Ehi = nullptr, *Ehi2 = nullptr;
+    if (n_split > 0) {
+      dElo = P.dev("zc.elo", ((size_t)n_split << ZS_LOG) * 16);
+      Ehi = P.dev("zc.ehi.a", 16ull << (n - 2 - ZS_LOG));
+      Ehi2 = P.dev("zc.ehi.b", 16ull << (n - 2 - ZS_LOG));
+      if (!dElo || !Ehi || !Ehi2) return FLOCK_E_OOM;
+    }
     if ((rc = P.eq_device(&r[7], n - 1, E))) return rc;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
-      // round i's eq table is eq(r[7 + i ..]); the kernel folds it for round i + 1
-      if (i == 0)
-        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
-      else
+      const bool split = i >= 1 && i <= n_split;
+      if (i == 0) {
+        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, ctx->readback_dev,
+                              i + 1 < n && n_split == 0 ? E2 : nullptr);
+        if (!rc && n_split > 0) {
+          std::vector<F128> elo((size_t)n_split << ZS_LOG);
+          {
+            HostPhase hp(ctx, "eq_tables");
+            for (int j = 1; j <= n_split; ++j) {
+              const std::vector<F128> e = eq_table(&r[7 + j], ZS_LOG);
+              std::copy(e.begin(), e.end(), elo.begin() + ((size_t)(j - 1) << ZS_LOG));
+            }
+          }
+          rc = P.h2d(dElo, elo.data(), elo.size() * 16);
+          if (!rc) rc = P.eq_device(&r[7 + 1 + ZS_LOG], n - 2 - ZS_LOG, Ehi);
+        }
+      } else if (split) {
+        rc = zc_round_split_device(ctx, st, A, Bv, (const uint8_t *)dElo + ((size_t)(i - 1) << ZS_LOG) * 16, Ehi,
+                                   half, ctx->readback_dev, i < n_split ? Ehi2 : nullptr);
+        std::swap(Ehi, Ehi2);
+      } else {
+        // the first whole-table round after the split ones starts from eq itself
+        if (i == n_split + 1 && n_split > 0 && (rc = P.eq_device(&r[7 + i], n - 1 - i, E))) return rc;
         rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
+      }
       if (rc) return rc;
-      std::swap(E, E2);
+      if (!split) std::swap(E, E2);
       uint4 h[2];
       if ((rc = P.fetch(h, 32))) return rc;
       F128 g1 = fv(h[0]), gi = fv(h[1]);

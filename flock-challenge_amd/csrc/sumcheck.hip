@@ -17,6 +17,7 @@
 #include "ctx.h"
 #include "fulltab.h"
 #include "gf128_dev.h"
+#include "nibtab.h"
 #include "prove_kernels.h"
 
 namespace flock {
@@ -126,6 +127,86 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
   if (E_next && half < 2) E_next = nullptr;
   hipLaunchKernelGGL(zc_round_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
                      (const f128 *)E, half, part, (f128 *)E_next);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return xor_rows_device(ctx, st, part, g, 2, out2);
+}
+
+// ---------------------------------------------------------------- zerocheck rounds, split eq
+// The same sums with E[x] = Elo[x mod 2^ZS_LOG] Ehi[x >> ZS_LOG] (eq over the
+// low ZS_LOG variables times eq over the rest):
+//   G = sum_{x_lo} Elo[x_lo] sum_{x_hi} Ehi[x_hi] p[x_lo, x_hi].
+// A workgroup takes one x_hi at a time, so Ehi[x_hi] is a workgroup constant:
+// its nibble tables (8 KiB, nibtab.h) serve all 2^ZS_LOG products by it, with
+// no per-product table build, and each thread keeps its ZS_PER x_lo's sums,
+// weighted by Elo once at the end.  Also folds Ehi for the next round,
+// Ehi_next[y] = Ehi[2y] + Ehi[2y+1], unless Ehi_next is null.
+constexpr int ZS_THREADS = 256;
+constexpr int ZS_WAVES = ZS_THREADS / 64;
+constexpr int ZS_PER = (1 << ZS_LOG) / ZS_THREADS;
+
+__global__ __launch_bounds__(ZS_THREADS) void zc_round_split_kernel(const f128 *__restrict__ A,
+                                                                    const f128 *__restrict__ B,
+                                                                    const f128 *__restrict__ Elo,
+                                                                    const f128 *__restrict__ Ehi, uint64_t n_hi,
+                                                                    f128 *__restrict__ partial,
+                                                                    f128 *__restrict__ Ehi_next) {
+  __shared__ f128 mt[ZS_WAVES][1024];  // per-lane product tables; the reduction reuses them
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
+  const uint32_t tid = threadIdx.x;
+  f128 *tab = &mt[0][0];
+  const CombLane CL(tid);
+  const NibtabLane NL(tid & 15u);
+  f128 s1[ZS_PER], si[ZS_PER];
+#pragma unroll
+  for (int s = 0; s < ZS_PER; ++s) s1[s] = si[s] = f128_zero();
+  for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
+    const f128 eh = Ehi[xh];
+    if (Ehi_next && !(xh & 1) && tid == 0) Ehi_next[xh >> 1] = f128_xor(eh, Ehi[xh + 1]);
+    __syncthreads();  // the previous x_hi's table reads are done
+    build_nibtab256<0>(eh, nt, tid);
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < ZS_PER; ++s) {
+      const uint64_t x = (xh << ZS_LOG) + (uint64_t)s * ZS_THREADS + tid;
+      const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1];
+      const f128 p1 = f128_mul(a1, b1, tab, CL);
+      const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
+      const f128 w1 = f128_mul_nibtab<0>(p1, nt, NL), wi = f128_mul_nibtab<0>(pi, nt, NL);
+      // register-indexed accumulators without dynamic indexing
+#pragma unroll
+      for (int q = 0; q < ZS_PER; ++q)
+        if (q == s) {
+          f128_xor_into(s1[q], w1);
+          f128_xor_into(si[q], wi);
+        }
+    }
+  }
+  f128 g1 = f128_zero(), gi = f128_zero();
+#pragma unroll
+  for (int s = 0; s < ZS_PER; ++s) {
+    f128_mul_build(Elo[s * ZS_THREADS + tid], tab, CL);
+    f128_xor_into(g1, f128_mul_comb(s1[s], tab, CL));
+    f128_xor_into(gi, f128_mul_comb(si[s], tab, CL));
+  }
+  __syncthreads();  // every lane's table reads are done before the reduction reuses mt
+  wg_xor_reduce(g1, tab, &partial[2 * blockIdx.x]);
+  wg_xor_reduce(gi, tab, &partial[2 * blockIdx.x + 1]);
+}
+
+int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
+                          uint64_t half, void *out2, void *Ehi_next) {
+  ScopedPhase ph(ctx, "zc_round", st);
+  const uint64_t n_hi = half >> ZS_LOG;
+  if ((half & ((1ull << ZS_LOG) - 1)) || n_hi < 2) {
+    ctx->last_error = "zerocheck: split round needs at least two x_hi blocks";
+    return FLOCK_E_SHAPE;
+  }
+  // two workgroups per CU (72 KiB of LDS each)
+  const unsigned g = (unsigned)(n_hi < 512 ? n_hi : 512);
+  f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL(zc_round_split_kernel, dim3(g), dim3(ZS_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
+                     (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, (f128 *)Ehi_next);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return xor_rows_device(ctx, st, part, g, 2, out2);
 }
