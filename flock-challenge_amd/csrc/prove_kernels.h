@@ -43,6 +43,11 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
 constexpr int ZS_LOG = 10;
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
                           uint64_t half, void *out2, void *Ehi_next);
+// Round 0 split at ZS_LOG + 1 over the packed rows a, b (A = L(row) through
+// the 256-entry nibble tables d_lw, entry v * 16 + p): Elo has 2^(ZS_LOG + 1)
+// entries, Ehi half >> (ZS_LOG + 1) (at least 2); Ehi is not folded.
+int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
+                           const void *Ehi, uint64_t half, void *out2);
 // Zerocheck multilinear round 0 and its fold, reading the packed rows a, b and
 // evaluating A = L(row) through 2048-entry byte tables (entry v * 8 + p):
 // d_lw8 of L for the round, d_lw8 and d_lwr8 (of rho L) for the fold.

@@ -487,6 +487,15 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     for (int i = 0; i < 4; ++i) r[9 + i] = medium(i);
     for (int i = 0; i < m - 13; ++i) r[13 + i] = ro[i];
   }
+  // The multilinear rounds' eq tables in split form, E_i = Elo_i (x) Ehi_i
+  // (round i's table is E_i = eq(r[7 + i ..]) over n - 1 - i variables):
+  // round 0 splits at ZS_LOG + 1 and rounds 1 .. n_split at ZS_LOG, so both
+  // start from Ehi = eq(r[7 + ZS_LOG + 1 ..]), which each later split round
+  // folds for the next.  They depend on r only and are prepared while the
+  // device works on round 1.
+  const int n_split = std::max(0, n - 2 - ZS_LOG);
+  void *dElo = nullptr, *Ehi = nullptr, *Ehi2 = nullptr;
+  const size_t elo0_len = (size_t)2 << ZS_LOG;
   F128 r1ab[64], r1c[64];
   {
     ScopedPhase ph(ctx, "zc_round1_all", st);
@@ -500,6 +509,24 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // elements are row pairs (x = 2e + h), e = il + 16 ih.
     if ((rc = P.eq_device(&r[11], m - 11, dhic))) return rc;
     if ((rc = bitfold_device(ctx, st, z, 4, 1ull << (m - 11), dhic, nullptr, dq))) return rc;
+    if (n_split > 0) {
+      dElo = P.dev("zc.elo", (elo0_len + ((size_t)n_split << ZS_LOG)) * 16);
+      Ehi = P.dev("zc.ehi.a", 16ull << (n - 2 - ZS_LOG));
+      Ehi2 = P.dev("zc.ehi.b", 16ull << (n - 2 - ZS_LOG));
+      if (!dElo || !Ehi || !Ehi2) return FLOCK_E_OOM;
+      std::vector<F128> elo(elo0_len + ((size_t)n_split << ZS_LOG));
+      {
+        HostPhase hp(ctx, "eq_tables");
+        const std::vector<F128> e0 = eq_table(&r[7], ZS_LOG + 1);
+        std::copy(e0.begin(), e0.end(), elo.begin());
+        for (int j = 1; j <= n_split; ++j) {
+          const std::vector<F128> e = eq_table(&r[7 + j], ZS_LOG);
+          std::copy(e.begin(), e.end(), elo.begin() + elo0_len + ((size_t)(j - 1) << ZS_LOG));
+        }
+      }
+      if ((rc = P.h2d(dElo, elo.data(), elo.size() * 16))) return rc;
+      if ((rc = P.eq_device(&r[7 + ZS_LOG + 1], n - 2 - ZS_LOG, Ehi))) return rc;
+    }
     {
       // P^AB(lambda_j) = sum_{x_mid, t} E4(x_mid) phi8(2^t) bitfold of bit t of
       // Y_j: element 4 x_mid + 2 h + t / 4, bit 32 (t % 4) + b for j = 32 h + b
@@ -559,9 +586,12 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // tables are materialized.
     void *A = P.dev("zc.A", rows * 8), *Bv = P.dev("zc.B", rows * 8);
     void *A2 = P.dev("zc.A2", rows * 4), *B2 = P.dev("zc.B2", rows * 4);
-    void *E = P.dev("zc.E", rows * 8), *E2 = P.dev("zc.E2", rows * 4);
-    void *dlw8 = P.dev("zc.lw8", 2048 * 16), *dlwr8 = P.dev("zc.lwr8", 2048 * 16);
-    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw8 || !dlwr8) return FLOCK_E_OOM;
+    // whole eq tables: for every round without a split form, else only for
+    // the rounds after n_split (from 2^ZS_LOG entries down)
+    const size_t e_len = n_split > 0 ? ((size_t)1 << ZS_LOG) : rows / 2;
+    void *E = P.dev("zc.E", e_len * 16), *E2 = P.dev("zc.E2", e_len * 8);
+    void *dlw8 = P.dev("zc.lw8", 2048 * 16), *dlwr8 = P.dev("zc.lwr8", 2048 * 16), *dlw = P.dev("zc.lw", 256 * 16);
+    if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw8 || !dlwr8 || !dlw) return FLOCK_E_OOM;
     // byte tables: out[v * 8 + p] = sum_{t < 8, bit t of v} w[8 p + t], each
     // entry one XOR from the entry without v's lowest set bit
     auto byte_tables = [](const F128 *w, F128 *out) {
@@ -575,43 +605,28 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       byte_tables(ws, lw8.data());
       if ((rc = P.h2d(dlw8, lw8.data(), lw8.size() * 16))) return rc;
     }
-    size_t len = rows;
-    // Round i's eq table is E_i = eq(r[7 + i ..]) over n - 1 - i variables.
-    // Rounds 1 .. n_split (half >= 2^(ZS_LOG + 1)) use it split as
-    // Elo_i = eq(r[7 + i .. 7 + i + ZS_LOG - 1]) times Ehi_i = eq(r[7 + i + ZS_LOG ..]):
-    // every Elo_i is computed on the host while round 0 runs, Ehi_1 is expanded
-    // on the device and each split round folds Ehi for the next.  Round 0 and
-    // the rounds after n_split use the whole table, folded by each round.
-    const int n_split = std::max(0, n - 2 - ZS_LOG);
-    void *dElo = nullptr, *Ehi = nullptr, *Ehi2 = nullptr;
     if (n_split > 0) {
-      dElo = P.dev("zc.elo", ((size_t)n_split << ZS_LOG) * 16);
-      Ehi = P.dev("zc.ehi.a", 16ull << (n - 2 - ZS_LOG));
-      Ehi2 = P.dev("zc.ehi.b", 16ull << (n - 2 - ZS_LOG));
-      if (!dElo || !Ehi || !Ehi2) return FLOCK_E_OOM;
+      // nibble tables: lw[v * 16 + p] = sum_{t < 4, bit t of v} L_{4p + t}(z)
+      F128 lw[256];
+      for (int p = 0; p < 16; ++p) {
+        lw[p] = F128();
+        for (int v = 1; v < 16; ++v) lw[v * 16 + p] = lw[(v & (v - 1)) * 16 + p] + ws[4 * p + __builtin_ctz(v)];
+      }
+      if ((rc = P.h2d(dlw, lw, sizeof lw))) return rc;
     }
-    if ((rc = P.eq_device(&r[7], n - 1, E))) return rc;
+    size_t len = rows;
+    if (n_split == 0 && (rc = P.eq_device(&r[7], n - 1, E))) return rc;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
-      const bool split = i >= 1 && i <= n_split;
-      if (i == 0) {
-        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, ctx->readback_dev,
-                              i + 1 < n && n_split == 0 ? E2 : nullptr);
-        if (!rc && n_split > 0) {
-          std::vector<F128> elo((size_t)n_split << ZS_LOG);
-          {
-            HostPhase hp(ctx, "eq_tables");
-            for (int j = 1; j <= n_split; ++j) {
-              const std::vector<F128> e = eq_table(&r[7 + j], ZS_LOG);
-              std::copy(e.begin(), e.end(), elo.begin() + ((size_t)(j - 1) << ZS_LOG));
-            }
-          }
-          rc = P.h2d(dElo, elo.data(), elo.size() * 16);
-          if (!rc) rc = P.eq_device(&r[7 + 1 + ZS_LOG], n - 2 - ZS_LOG, Ehi);
-        }
+      const bool split = n_split > 0 && i <= n_split;  // round 0 too when there are split rounds
+      if (i == 0 && n_split > 0) {
+        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev);
+      } else if (i == 0) {
+        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
       } else if (split) {
-        rc = zc_round_split_device(ctx, st, A, Bv, (const uint8_t *)dElo + ((size_t)(i - 1) << ZS_LOG) * 16, Ehi,
-                                   half, ctx->readback_dev, i < n_split ? Ehi2 : nullptr);
+        rc = zc_round_split_device(ctx, st, A, Bv,
+                                   (const uint8_t *)dElo + (elo0_len + ((size_t)(i - 1) << ZS_LOG)) * 16, Ehi, half,
+                                   ctx->readback_dev, i < n_split ? Ehi2 : nullptr);
         std::swap(Ehi, Ehi2);
       } else {
         // the first whole-table round after the split ones starts from eq itself

@@ -132,49 +132,86 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
 }
 
 // ---------------------------------------------------------------- zerocheck rounds, split eq
-// The same sums with E[x] = Elo[x mod 2^ZS_LOG] Ehi[x >> ZS_LOG] (eq over the
-// low ZS_LOG variables times eq over the rest):
+// A = L(row) for round 0 read from the packed rows (see zerocheck round 0
+// below) through nibble tables T[v * 16 + p] = sum_{t < 4, bit t of v}
+// L_{4p + t}(z) (4 KiB), read in a lane-dependent order p = step ^ (lane mod
+// 16) so lanes hit distinct bank groups (entry (v, p) sits in bank group p).
+__device__ __forceinline__ f128 lagrange_row(uint64_t row, const f128 *__restrict__ T, uint32_t rot) {
+  const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+  f128 acc = f128_zero();
+#pragma unroll
+  for (uint32_t step = 0; step < 16; step += 2) {
+    const uint32_t p0 = step ^ rot, p1 = (step + 1) ^ rot;
+    const uint32_t v0 = __builtin_amdgcn_ubfe(p0 < 8 ? lo : hi, 4 * (p0 & 7), 4);
+    const uint32_t v1 = __builtin_amdgcn_ubfe(p1 < 8 ? lo : hi, 4 * (p1 & 7), 4);
+    acc = f128_xor3(acc, T[v0 * 16 + p0], T[v1 * 16 + p1]);
+  }
+  return acc;
+}
+
+// The round sums with E[x] = Elo[x mod 2^LOG] Ehi[x >> LOG] (eq over the low
+// LOG variables times eq over the rest):
 //   G = sum_{x_lo} Elo[x_lo] sum_{x_hi} Ehi[x_hi] p[x_lo, x_hi].
 // A workgroup takes one x_hi at a time, so Ehi[x_hi] is a workgroup constant:
-// its nibble tables (8 KiB, nibtab.h) serve all 2^ZS_LOG products by it, with
-// no per-product table build, and each thread keeps its ZS_PER x_lo's sums,
-// weighted by Elo once at the end.  Also folds Ehi for the next round,
-// Ehi_next[y] = Ehi[2y] + Ehi[2y+1], unless Ehi_next is null.
-constexpr int ZS_THREADS = 256;
-constexpr int ZS_WAVES = ZS_THREADS / 64;
-constexpr int ZS_PER = (1 << ZS_LOG) / ZS_THREADS;
-
-__global__ __launch_bounds__(ZS_THREADS) void zc_round_split_kernel(const f128 *__restrict__ A,
-                                                                    const f128 *__restrict__ B,
-                                                                    const f128 *__restrict__ Elo,
-                                                                    const f128 *__restrict__ Ehi, uint64_t n_hi,
-                                                                    f128 *__restrict__ partial,
-                                                                    f128 *__restrict__ Ehi_next) {
-  __shared__ f128 mt[ZS_WAVES][1024];  // per-lane product tables; the reduction reuses them
+// its nibble tables (8 KiB, nibtab.h) serve all 2^LOG products by it, with
+// no per-product table build, and each thread keeps its 2^LOG / THREADS
+// x_lo's sums, weighted by Elo once at the end.  Also folds Ehi for the next
+// round, Ehi_next[y] = Ehi[2y] + Ehi[2y+1], unless Ehi_next is null.
+// ROWS: the tables are L(rows) of the packed rows a, b (round 0, nibble
+// tables of L in lw); otherwise F128 tables A, B.
+template <int THREADS, int LOG, bool ROWS>
+__global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const void *__restrict__ Ain,
+                                                                 const void *__restrict__ Bin,
+                                                                 const f128 *__restrict__ lw,
+                                                                 const f128 *__restrict__ Elo,
+                                                                 const f128 *__restrict__ Ehi, uint64_t n_hi,
+                                                                 f128 *__restrict__ partial,
+                                                                 f128 *__restrict__ Ehi_next) {
+  constexpr int WAVES = THREADS / 64, PER = (1 << LOG) / THREADS;
+  static_assert(THREADS == 256 || THREADS == 512, "the table builds assume 256 or 512 threads");
+  __shared__ f128 mt[WAVES][1024];  // per-lane product tables; the reduction reuses them
   __shared__ __align__(16) char nt[NIBTAB_BYTES];
+  __shared__ f128 T[ROWS ? 256 : 1];
   const uint32_t tid = threadIdx.x;
+  if constexpr (ROWS) {
+    if (tid < 256) T[tid] = lw[tid];
+  }
   f128 *tab = &mt[0][0];
   const CombLane CL(tid);
   const NibtabLane NL(tid & 15u);
-  f128 s1[ZS_PER], si[ZS_PER];
+  f128 s1[PER], si[PER];
 #pragma unroll
-  for (int s = 0; s < ZS_PER; ++s) s1[s] = si[s] = f128_zero();
+  for (int s = 0; s < PER; ++s) s1[s] = si[s] = f128_zero();
   for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
     const f128 eh = Ehi[xh];
     if (Ehi_next && !(xh & 1) && tid == 0) Ehi_next[xh >> 1] = f128_xor(eh, Ehi[xh + 1]);
-    __syncthreads();  // the previous x_hi's table reads are done
-    build_nibtab256<0>(eh, nt, tid);
+    __syncthreads();  // the previous x_hi's table reads are done (and T is visible)
+    if (tid < 256) build_nibtab256<0>(eh, nt, tid);
     __syncthreads();
 #pragma unroll 1
-    for (int s = 0; s < ZS_PER; ++s) {
-      const uint64_t x = (xh << ZS_LOG) + (uint64_t)s * ZS_THREADS + tid;
-      const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1];
+    for (int s = 0; s < PER; ++s) {
+      const uint64_t x = (xh << LOG) + (uint64_t)s * THREADS + tid;
+      f128 a0, a1, b0, b1;
+      if constexpr (ROWS) {
+        const ulonglong2 ar = ((const ulonglong2 *)Ain)[x], br = ((const ulonglong2 *)Bin)[x];
+        const uint32_t rot = tid & 15u;
+        a0 = lagrange_row(ar.x, T, rot);
+        a1 = lagrange_row(ar.y, T, rot);
+        b0 = lagrange_row(br.x, T, rot);
+        b1 = lagrange_row(br.y, T, rot);
+      } else {
+        const f128 *A = (const f128 *)Ain, *B = (const f128 *)Bin;
+        a0 = A[2 * x];
+        a1 = A[2 * x + 1];
+        b0 = B[2 * x];
+        b1 = B[2 * x + 1];
+      }
       const f128 p1 = f128_mul(a1, b1, tab, CL);
       const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
       const f128 w1 = f128_mul_nibtab<0>(p1, nt, NL), wi = f128_mul_nibtab<0>(pi, nt, NL);
       // register-indexed accumulators without dynamic indexing
 #pragma unroll
-      for (int q = 0; q < ZS_PER; ++q)
+      for (int q = 0; q < PER; ++q)
         if (q == s) {
           f128_xor_into(s1[q], w1);
           f128_xor_into(si[q], wi);
@@ -183,8 +220,8 @@ __global__ __launch_bounds__(ZS_THREADS) void zc_round_split_kernel(const f128 *
   }
   f128 g1 = f128_zero(), gi = f128_zero();
 #pragma unroll
-  for (int s = 0; s < ZS_PER; ++s) {
-    f128_mul_build(Elo[s * ZS_THREADS + tid], tab, CL);
+  for (int s = 0; s < PER; ++s) {
+    f128_mul_build(Elo[s * THREADS + tid], tab, CL);
     f128_xor_into(g1, f128_mul_comb(s1[s], tab, CL));
     f128_xor_into(gi, f128_mul_comb(si[s], tab, CL));
   }
@@ -193,6 +230,9 @@ __global__ __launch_bounds__(ZS_THREADS) void zc_round_split_kernel(const f128 *
   wg_xor_reduce(gi, tab, &partial[2 * blockIdx.x + 1]);
 }
 
+// Rounds after round 0: 256-thread workgroups (72 KiB of LDS, two per CU)
+// over F128 tables, split at ZS_LOG.  Round 0: 512-thread workgroups (140 KiB,
+// one per CU) over the rows, split at ZS_LOG + 1.
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
                           uint64_t half, void *out2, void *Ehi_next) {
   ScopedPhase ph(ctx, "zc_round", st);
@@ -201,12 +241,29 @@ int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B
     ctx->last_error = "zerocheck: split round needs at least two x_hi blocks";
     return FLOCK_E_SHAPE;
   }
-  // two workgroups per CU (72 KiB of LDS each)
   const unsigned g = (unsigned)(n_hi < 512 ? n_hi : 512);
   f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
-  hipLaunchKernelGGL(zc_round_split_kernel, dim3(g), dim3(ZS_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
+  hipLaunchKernelGGL((zc_round_split_kernel<256, ZS_LOG, false>), dim3(g), dim3(256), 0, st, A, B, nullptr,
                      (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, (f128 *)Ehi_next);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return xor_rows_device(ctx, st, part, g, 2, out2);
+}
+
+int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
+                           const void *Ehi, uint64_t half, void *out2) {
+  ScopedPhase ph(ctx, "zc_round", st);
+  constexpr int LOG = ZS_LOG + 1;
+  const uint64_t n_hi = half >> LOG;
+  if ((half & ((1ull << LOG) - 1)) || n_hi < 2) {
+    ctx->last_error = "zerocheck: split round 0 needs at least two x_hi blocks";
+    return FLOCK_E_SHAPE;
+  }
+  const unsigned g = (unsigned)(n_hi < 256 ? n_hi : 256);
+  f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL((zc_round_split_kernel<512, LOG, true>), dim3(g), dim3(512), 0, st, a, b, (const f128 *)d_lw,
+                     (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, nullptr);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return xor_rows_device(ctx, st, part, g, 2, out2);
 }
