@@ -48,11 +48,9 @@ int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B
 // entries, Ehi half >> (ZS_LOG + 1) (at least 2); Ehi is not folded.
 int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
                            const void *Ehi, uint64_t half, void *out2);
-// Zerocheck multilinear round 0 and its fold, reading the packed rows a, b and
+// The fold after zerocheck round 0, reading the packed rows a, b and
 // evaluating A = L(row) through 2048-entry byte tables (entry v * 8 + p):
-// d_lw8 of L for the round, d_lw8 and d_lwr8 (of rho L) for the fold.
-int zc_round0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *E,
-                     uint64_t half, void *out2, void *E_next);
+// d_lw8 of L and d_lwr8 of rho L.
 int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *d_lwr8,
                     uint64_t half, void *A2, void *B2);
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);

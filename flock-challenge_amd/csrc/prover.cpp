@@ -493,7 +493,11 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   // start from Ehi = eq(r[7 + ZS_LOG + 1 ..]), which each later split round
   // folds for the next.  They depend on r only and are prepared while the
   // device works on round 1.
-  const int n_split = std::max(0, n - 2 - ZS_LOG);
+  const int n_split = n - 2 - ZS_LOG;
+  if (n_split < 1) {  // m >= 22 for every Ligerito configuration
+    ctx->last_error = "zerocheck: m too small for the split eq tables";
+    return FLOCK_E_SHAPE;
+  }
   void *dElo = nullptr, *Ehi = nullptr, *Ehi2 = nullptr;
   const size_t elo0_len = (size_t)2 << ZS_LOG;
   F128 r1ab[64], r1c[64];
@@ -509,7 +513,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // elements are row pairs (x = 2e + h), e = il + 16 ih.
     if ((rc = P.eq_device(&r[11], m - 11, dhic))) return rc;
     if ((rc = bitfold_device(ctx, st, z, 4, 1ull << (m - 11), dhic, nullptr, dq))) return rc;
-    if (n_split > 0) {
+    {
       dElo = P.dev("zc.elo", (elo0_len + ((size_t)n_split << ZS_LOG)) * 16);
       Ehi = P.dev("zc.ehi.a", 16ull << (n - 2 - ZS_LOG));
       Ehi2 = P.dev("zc.ehi.b", 16ull << (n - 2 - ZS_LOG));
@@ -586,9 +590,8 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // tables are materialized.
     void *A = P.dev("zc.A", rows * 8), *Bv = P.dev("zc.B", rows * 8);
     void *A2 = P.dev("zc.A2", rows * 4), *B2 = P.dev("zc.B2", rows * 4);
-    // whole eq tables: for every round without a split form, else only for
-    // the rounds after n_split (from 2^ZS_LOG entries down)
-    const size_t e_len = n_split > 0 ? ((size_t)1 << ZS_LOG) : rows / 2;
+    // whole eq tables for the rounds after n_split (from 2^ZS_LOG entries down)
+    const size_t e_len = (size_t)1 << ZS_LOG;
     void *E = P.dev("zc.E", e_len * 16), *E2 = P.dev("zc.E2", e_len * 8);
     void *dlw8 = P.dev("zc.lw8", 2048 * 16), *dlwr8 = P.dev("zc.lwr8", 2048 * 16), *dlw = P.dev("zc.lw", 256 * 16);
     if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw8 || !dlwr8 || !dlw) return FLOCK_E_OOM;
@@ -605,7 +608,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       byte_tables(ws, lw8.data());
       if ((rc = P.h2d(dlw8, lw8.data(), lw8.size() * 16))) return rc;
     }
-    if (n_split > 0) {
+    {
       // nibble tables: lw[v * 16 + p] = sum_{t < 4, bit t of v} L_{4p + t}(z)
       F128 lw[256];
       for (int p = 0; p < 16; ++p) {
@@ -615,14 +618,11 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       if ((rc = P.h2d(dlw, lw, sizeof lw))) return rc;
     }
     size_t len = rows;
-    if (n_split == 0 && (rc = P.eq_device(&r[7], n - 1, E))) return rc;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
-      const bool split = n_split > 0 && i <= n_split;  // round 0 too when there are split rounds
-      if (i == 0 && n_split > 0) {
+      const bool split = i <= n_split;
+      if (i == 0) {
         rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev);
-      } else if (i == 0) {
-        rc = zc_round0_device(ctx, st, a, b, dlw8, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
       } else if (split) {
         rc = zc_round_split_device(ctx, st, A, Bv,
                                    (const uint8_t *)dElo + (elo0_len + ((size_t)(i - 1) << ZS_LOG)) * 16, Ehi, half,
@@ -630,7 +630,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         std::swap(Ehi, Ehi2);
       } else {
         // the first whole-table round after the split ones starts from eq itself
-        if (i == n_split + 1 && n_split > 0 && (rc = P.eq_device(&r[7 + i], n - 1 - i, E))) return rc;
+        if (i == n_split + 1 && (rc = P.eq_device(&r[7 + i], n - 1 - i, E))) return rc;
         rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
       }
       if (rc) return rc;

@@ -268,21 +268,13 @@ int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *
   return xor_rows_device(ctx, st, part, g, 2, out2);
 }
 
-// ---------------------------------------------------------------- zerocheck round 0
+// ---------------------------------------------------------------- zerocheck fold after round 0
 // After round 1 the multilinear tables are A[x] = sum_s bit_s(a[x]) L_s(z)
 // (fold_at_z_naive, zerocheck/multilinear.rs:598-625), a GF(2)-linear image
-// of the 64-bit row.  Round 0 and its fold read the rows and evaluate that
-// map on the fly instead of materializing A and B, through byte tables
-// T[v * 8 + p] = sum_{t < 8, bit t of v} L_{8p + t}(z) (32 KiB), read in a
-// lane-dependent order p = step ^ (lane mod 8).
-//
-// Round 0 uses 8-wavefront workgroups: 8 x 16 KiB of per-lane product
-// tables plus one shared 32 KiB byte table of L, entry (v, p) at v * 8 + p
-// (byte position p = step ^ (lane mod 8), 8 lookups per row); the reduction
-// reuses the byte-table space.
-constexpr int ZR0_WAVES = 8;
-constexpr int ZR0_THREADS = 64 * ZR0_WAVES;
-
+// of the 64-bit row.  Round 0 (zc_round_split_kernel with ROWS) and its fold
+// read the rows and evaluate that map on the fly instead of materializing A
+// and B.  The fold uses byte tables T[v * 8 + p] = sum_{t < 8, bit t of v}
+// L_{8p + t}(z) (32 KiB), read in a lane-dependent order p = step ^ (lane mod 8).
 __device__ __forceinline__ f128 lagrange_row8(uint64_t row, const f128 *__restrict__ T, uint32_t rot8) {
   const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
   f128 acc = f128_zero();
@@ -294,56 +286,6 @@ __device__ __forceinline__ f128 lagrange_row8(uint64_t row, const f128 *__restri
     acc = f128_xor3(acc, T[v0 * 8 + p0], T[v1 * 8 + p1]);
   }
   return acc;
-}
-
-__global__ __launch_bounds__(ZR0_THREADS) void zc_round0_kernel(const uint64_t *__restrict__ a,
-                                                                const uint64_t *__restrict__ b,
-                                                                const f128 *__restrict__ lw8,
-                                                                const f128 *__restrict__ E, uint64_t half,
-                                                                f128 *__restrict__ partial, f128 *__restrict__ En) {
-  __shared__ f128 mt[ZR0_WAVES][1024];
-  __shared__ f128 T[2048];
-  for (uint32_t i = threadIdx.x; i < 2048; i += ZR0_THREADS) T[i] = lw8[i];
-  __syncthreads();
-  const uint32_t lane = threadIdx.x & 63, rot8 = threadIdx.x & 7;
-  f128 *tab = &mt[0][0];
-  const CombLane CL(threadIdx.x);
-  f128 g1 = f128_zero(), gi = f128_zero();
-  for (uint64_t x = (uint64_t)blockIdx.x * ZR0_THREADS + threadIdx.x; x < half;
-       x += (uint64_t)gridDim.x * ZR0_THREADS) {
-    const f128 a0 = lagrange_row8(a[2 * x], T, rot8), a1 = lagrange_row8(a[2 * x + 1], T, rot8);
-    const f128 b0 = lagrange_row8(b[2 * x], T, rot8), b1 = lagrange_row8(b[2 * x + 1], T, rot8);
-    const f128 e = E[x];
-    if (En) {
-      f128 o;
-      o.x = __shfl_xor(e.x, 1);
-      o.y = __shfl_xor(e.y, 1);
-      o.z = __shfl_xor(e.z, 1);
-      o.w = __shfl_xor(e.w, 1);
-      if (!(x & 1)) En[x >> 1] = f128_xor(e, o);
-    }
-    const f128 p1 = f128_mul(a1, b1, tab, CL);
-    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
-    f128_mul_build(e, tab, CL);  // one table of E for both weighted sums
-    f128_xor_into(g1, f128_mul_comb(p1, tab, CL));
-    f128_xor_into(gi, f128_mul_comb(pi, tab, CL));
-  }
-  __syncthreads();  // T is reused as the reduction buffer
-  wg_xor_reduce(g1, T, &partial[2 * blockIdx.x]);
-  wg_xor_reduce(gi, T, &partial[2 * blockIdx.x + 1]);
-}
-
-int zc_round0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *E,
-                     uint64_t half, void *out2, void *E_next) {
-  ScopedPhase ph(ctx, "zc_round", st);
-  unsigned g = grid_for(half, ZR0_THREADS, 1024);
-  f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
-  if (!part) return FLOCK_E_OOM;
-  if (E_next && half < 2) E_next = nullptr;
-  hipLaunchKernelGGL(zc_round0_kernel, dim3(g), dim3(ZR0_THREADS), 0, st, (const uint64_t *)a, (const uint64_t *)b,
-                     (const f128 *)d_lw8, (const f128 *)E, half, part, (f128 *)E_next);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2);
 }
 
 // A'[x] = A[2x] + rho (A[2x] + A[2x+1]) with A = L(rows): by linearity
