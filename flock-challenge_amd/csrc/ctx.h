@@ -71,8 +71,18 @@ struct Ctx {
   // A sequence number the device publishes in the readback area (at this
   // offset) once the work queued before it has finished: the host polls it
   // instead of synchronizing the stream, which wakes several microseconds later.
+  // Kernels that write a result into the readback area publish the number
+  // themselves after their writes (the reductions and the grinding take);
+  // otherwise a one-lane signal kernel does.
   static constexpr size_t READBACK_FLAG = READBACK_BYTES - 64;
   uint32_t readback_seq = 0;
+  // The next sequence number to publish; never 0, which kernels read as
+  // "publish nothing".
+  uint32_t next_readback_seq() {
+    if (++readback_seq == 0) ++readback_seq;
+    return readback_seq;
+  }
+  uint32_t *readback_flag_dev() const { return (uint32_t *)((uint8_t *)readback_dev + READBACK_FLAG); }
   uint8_t *pinned_alloc(size_t n);  // nullptr when the ring is full (or unavailable)
   void pinned_reset() { pinned_off = 0; }
 

@@ -81,8 +81,9 @@ __device__ __forceinline__ f128 f128_mul_fulltab(f128 v, const char *__restrict_
   return acc;
 }
 
-// Two independent products by the same table, interleaved for latency.
-template <bool HI = false>
+// Two independent products, interleaved for latency: v0 by the table at
+// base (+ 64 KiB when HI), v1 by the one HI1 selects (the same by default).
+template <bool HI = false, bool HI1 = HI>
 __device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *__restrict__ base,
                                                   const FulltabLane &L, f128 &p0, f128 &p1) {
   const f128 x0 = L.xperm(v0), x1 = L.xperm(v1);
@@ -97,7 +98,7 @@ __device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *
     for (int i = 0; i < 4; ++i) {
       const uint32_t o = i < 2 ? oa : ob;
       t0[i] = *(const f128 *)(base + __builtin_amdgcn_perm(o, w0, fulltab_sel(i, HI)));
-      t1[i] = *(const f128 *)(base + __builtin_amdgcn_perm(o, w1, fulltab_sel(i, HI)));
+      t1[i] = *(const f128 *)(base + __builtin_amdgcn_perm(o, w1, fulltab_sel(i, HI1)));
     }
     p0 = f128_xor3(p0, t0[0], t0[1]);
     p1 = f128_xor3(p1, t1[0], t1[1]);

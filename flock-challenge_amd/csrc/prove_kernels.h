@@ -17,7 +17,11 @@ namespace flock {
 size_t zc_tables_bytes();
 // Write seq to the mapped host word d_flag after the stream's earlier work.
 int signal_device(Ctx *ctx, hipStream_t st, void *d_flag, uint32_t seq);
-int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out);
+// out[w] = xor_r in[r * width + w] (width <= 4); a nonzero seq is then
+// published at the context's readback flag (see Ctx::READBACK_FLAG), which
+// the reductions ending in the readback area use in place of signal_device.
+int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out,
+                    uint32_t seq = 0);
 // Zerocheck round 1, product part, in two steps: zc_y_device computes
 // Y_j(x_mid, x_hi) from the rows (no challenge involved), zc_round1_device
 // then d_q[4 x_mid + q4][8 j + t] (64 x 128 F128) = sum_{x_hi} Ehi(x_hi)
@@ -33,40 +37,48 @@ int bitfold_device(Ctx *ctx, hipStream_t st, const void *elems, int il_log, uint
 int xor_reduce_device(Ctx *ctx, hipStream_t st, const void *in, uint32_t ny, uint32_t nx, uint32_t width, void *out);
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out);
 // (G(1), G(inf)) of one multilinear zerocheck round; writes the next round's
-// eq table into E_next (half / 2 entries) unless it is null.
+// eq table into E_next (half / 2 entries) unless it is null.  Here and in the
+// other reductions below, a nonzero seq is published once out2 is written
+// (xor_rows_device).
 int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2,
-                    void *E_next);
+                    void *E_next, uint32_t seq = 0);
 // The same round with the eq table split as Elo (x) Ehi, Elo over the low
 // ZS_LOG variables (2^ZS_LOG entries), Ehi over the rest (half >> ZS_LOG
 // entries, at least 2); writes Ehi folded for the next round into Ehi_next
 // (half >> (ZS_LOG + 1) entries) unless it is null.
 constexpr int ZS_LOG = 10;
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
-                          uint64_t half, void *out2, void *Ehi_next);
+                          uint64_t half, void *out2, void *Ehi_next, uint32_t seq = 0);
 // Round 0 split at ZS_LOG + 1 over the packed rows a, b (A = L(row) through
 // the 256-entry nibble tables d_lw, entry v * 16 + p): Elo has 2^(ZS_LOG + 1)
 // entries, Ehi half >> (ZS_LOG + 1) (at least 2); Ehi is not folded.
 int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
-                           const void *Ehi, uint64_t half, void *out2);
+                           const void *Ehi, uint64_t half, void *out2, uint32_t seq = 0);
 // The fold after zerocheck round 0, reading the packed rows a, b and
 // evaluating A = L(row) through 2048-entry byte tables (entry v * 8 + p):
 // d_lw8 of L and d_lwr8 of rho L.
 int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *d_lwr8,
                     uint64_t half, void *A2, void *B2);
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
-int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3);
+int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
+                   uint32_t seq = 0);
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);
 int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
                    const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb);
-int rs_basis_device(Ctx *ctx, hipStream_t st, const void *e0, const void *e1, const void *tables, uint64_t n, void *out);
+// out[i] = M_0(lo0[i mod 2^k] hi0[i >> k]) + M_1(lo1[i mod 2^k] hi1[i >> k])
+// for i < 2^(k + n_hi_log), M_c given by its nibble tables (tables[c][p][v]
+// = M_c(v x^(4p)), 2 x 32 x 16 F128).
+int rs_basis_device(Ctx *ctx, hipStream_t st, const void *tables, const void *lo0, const void *lo1, int k,
+                    const void *hi0, const void *hi1, int n_hi_log, void *out);
 // Whole forward (layers l0..k-1) or transposed (k-1..l0) NTT over 2^k
 // positions x 2^log_nl lanes, up to 4 layers per launch.
 int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l0, int log_nl, int transposed);
 int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed);
 int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
                     void *d_best);
-// out (device-visible) = *best, then *best = all-ones for the next window.
-int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out);
+// out (device-visible) = *best, then *best = all-ones for the next window;
+// then publishes a nonzero seq as xor_rows_device does.
+int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out, uint32_t seq = 0);
 int gather_device(Ctx *ctx, hipStream_t st, const void *src, const uint64_t *d_idx, uint32_t n, uint32_t unit_bytes,
                   void *out);
 int scatter_device(Ctx *ctx, hipStream_t st, void *dst, const uint64_t *d_idx, const void *d_val, uint32_t n);

@@ -150,11 +150,15 @@ struct Prover {
 
   // Wait until everything queued on the stream so far has finished: a
   // signal kernel publishes a new sequence number in the mapped readback
-  // area and the host polls it (with the stream's error state checked now
-  // and then, so a failed kernel ends the wait).
+  // area and the host polls it.
   int wait_signal() {
-    const uint32_t seq = ++ctx->readback_seq;
-    if (int rc = signal_device(ctx, st, (uint8_t *)ctx->readback_dev + Ctx::READBACK_FLAG, seq)) return rc;
+    const uint32_t seq = ctx->next_readback_seq();
+    if (int rc = signal_device(ctx, st, ctx->readback_flag_dev(), seq)) return rc;
+    return wait_seq(seq);
+  }
+  // Poll until seq is published (with the stream's error state checked now
+  // and then, so a failed kernel ends the wait).
+  int wait_seq(uint32_t seq) {
     const uint32_t *flag = (const uint32_t *)(ctx->readback + Ctx::READBACK_FLAG);
     for (uint32_t spins = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; ++spins) {
       if ((spins & 4095) == 0) {
@@ -199,11 +203,11 @@ struct Prover {
     memcpy(h1, ctx->landing + n0, n1);
     return FLOCK_OK;
   }
-  // A small result a kernel wrote into the mapped readback area: wait for
-  // the stream, then read it.
-  int fetch(void *h, size_t n) {
+  // A small result a kernel wrote into the mapped readback area and then
+  // published as seq: wait for it, then read it.
+  int fetch(void *h, size_t n, uint32_t seq) {
     HostPhase hp(ctx, "wait_d2h");
-    if (int rc = wait_signal()) return rc;
+    if (int rc = wait_seq(seq)) return rc;
     memcpy(h, ctx->readback, n);
     return FLOCK_OK;
   }
@@ -266,10 +270,11 @@ struct Prover {
       }
       uint64_t start = 0, window = 1ull << std::min<uint32_t>(bits + 2, 26);
       for (;;) {
+        const uint32_t seq = ctx->next_readback_seq();
         int rc = pow_scan_device(ctx, st, dg, start, window, bits, best);
-        if (!rc) rc = pow_take_device(ctx, st, best, ctx->readback_dev);
+        if (!rc) rc = pow_take_device(ctx, st, best, ctx->readback_dev, seq);
         unsigned long long got = 0;
-        if (!rc) rc = fetch(&got, 8);
+        if (!rc) rc = fetch(&got, 8, seq);
         if (rc) return rc;
         if (got != ~0ull) {
           nonce = got;
@@ -349,9 +354,10 @@ struct Prover {
   }
 
   int msg(const void *f, const void *b, uint64_t len, int with_eval, F128 out[3]) {
-    int rc = lig_msg_device(ctx, st, f, b, len / 2, with_eval, ctx->readback_dev);
+    const uint32_t seq = ctx->next_readback_seq();
+    int rc = lig_msg_device(ctx, st, f, b, len / 2, with_eval, ctx->readback_dev, seq);
     uint4 h[3];
-    if (!rc) rc = fetch(h, 48);
+    if (!rc) rc = fetch(h, 48, seq);
     for (int i = 0; i < 3; ++i) out[i] = fv(h[i]);
     return rc;
   }
@@ -621,22 +627,23 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
       const bool split = i <= n_split;
+      const uint32_t seq = ctx->next_readback_seq();
       if (i == 0) {
-        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev);
+        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev, seq);
       } else if (split) {
         rc = zc_round_split_device(ctx, st, A, Bv,
                                    (const uint8_t *)dElo + (elo0_len + ((size_t)(i - 1) << ZS_LOG)) * 16, Ehi, half,
-                                   ctx->readback_dev, i < n_split ? Ehi2 : nullptr);
+                                   ctx->readback_dev, i < n_split ? Ehi2 : nullptr, seq);
         std::swap(Ehi, Ehi2);
       } else {
         // the first whole-table round after the split ones starts from eq itself
         if (i == n_split + 1 && (rc = P.eq_device(&r[7 + i], n - 1 - i, E))) return rc;
-        rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr);
+        rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr, seq);
       }
       if (rc) return rc;
       if (!split) std::swap(E, E2);
       uint4 h[2];
-      if ((rc = P.fetch(h, 32))) return rc;
+      if ((rc = P.fetch(h, 32, seq))) return rc;
       F128 g1 = fv(h[0]), gi = fv(h[1]);
       zc_rounds.push_back({g1, gi});
       t.observe(g1);
@@ -677,10 +684,14 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
 
   // The zerocheck point's part of the ring-switch claims (x_full of claim 1)
   // and its eq tables' buffers: known now, expanded during the lincheck.
+  // eq(x_full[1..]) over L variables is kept as two factors, over the low
+  // rs_k variables and the rest (rs_basis_device).
   const std::vector<F128> xf1(r.begin() + 6, r.end());
   const size_t N = (size_t)1 << L;
-  void *rs_h1 = P.dev("rs.hi1", (N / 16) * 16), *rs_e1 = P.dev("rs.e1", N * 16);
-  if (!rs_h1 || !rs_e1) return FLOCK_E_OOM;
+  const int rs_k = std::max(std::min(13, L - 8), std::min(L - 1, 10));
+  void *rs_h1 = P.dev("rs.hi1", (N / 16) * 16), *rs_lo1 = P.dev("rs.lo1", 16ull << rs_k);
+  void *rs_hk1 = P.dev("rs.hk1", 16ull << (L - rs_k));
+  if (!rs_h1 || !rs_lo1 || !rs_hk1) return FLOCK_E_OOM;
 
   // ---- lincheck (lincheck.rs:1480-1664)
   F128 lc_rounds[8][2], zpart[64], r_inner_skip, w_claim, r_inner_rest[8];
@@ -734,7 +745,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing, dcomb, lc_bytes, hipMemcpyDeviceToHost, st));
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(ctx->landing + lc_bytes, dzv, lc_bytes, hipMemcpyDeviceToHost, st));
     FLOCK_HIP_CHECK(ctx, hipEventRecord(lc_ready, st));
-    if ((rc = P.eq_device(&xf1[5], L - 4, rs_h1)) || (rc = P.eq_device(&xf1[1], L, rs_e1))) return rc;
+    if ((rc = P.eq_device(&xf1[5], L - 4, rs_h1)) || (rc = P.eq_device(&xf1[1], rs_k, rs_lo1)) ||
+        (rc = P.eq_device(&xf1[1 + rs_k], L - rs_k, rs_hk1)))
+      return rc;
     {
       HostPhase hp(ctx, "wait_d2h");
       const hipError_t e = hipEventSynchronize(lc_ready);
@@ -819,19 +832,20 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
 
   // ---- PCS open: ring switch (ring_switch.rs:2708-2780, 3040-3117)
   t.label("flock-pcs-open-batch-v0");
-  void *e0 = P.dev("rs.e0", N * 16), *e1 = rs_e1, *cb = P.dev("lig.cb", N * 16);
-  void *dtab = P.dev("rs.tab", 2 * 32 * 16 * 16);
-  if (!e0 || !e1 || !cb || !dtab) return FLOCK_E_OOM;
+  void *rs_lo0 = P.dev("rs.lo0", 16ull << rs_k), *rs_hk0 = P.dev("rs.hk0", 16ull << (L - rs_k));
+  void *cb = P.dev("lig.cb", N * 16), *dtab = P.dev("rs.tab", 2 * 32 * 16 * 16);
+  if (!rs_lo0 || !rs_hk0 || !cb || !dtab) return FLOCK_E_OOM;
   F128 shv[2][128], sclaim[2], eqr[2][128];
   {
     // s_hat_v[k][r] = sum_i bit_r(z[i]) eq_k(i), eq_k(i) = lo_k(i mod 16) hi_k(i / 16)
     void *h0 = P.dev("rs.hi0", (N / 16) * 16), *h1 = rs_h1;
     void *dq = P.dev("rs.q", 2 * 2048 * 16);
     if (!h0 || !h1 || !dq) return FLOCK_E_OOM;
-    // (the tables of claim 1, h1 and e1, were expanded during the lincheck)
+    // (the tables of claim 1 were expanded during the lincheck)
     if ((rc = P.eq_device(&xf[0][5], L - 4, h0))) return rc;
     if ((rc = bitfold_device(ctx, st, z, 4, N / 16, h0, h1, dq))) return rc;
-    if ((rc = P.eq_device(&xf[0][1], L, e0))) return rc;
+    if ((rc = P.eq_device(&xf[0][1], rs_k, rs_lo0)) || (rc = P.eq_device(&xf[0][1 + rs_k], L - rs_k, rs_hk0)))
+      return rc;
     std::vector<F128> q(2 * 2048);
     if ((rc = P.d2h(q.data(), dq, q.size() * 16))) return rc;
     for (int k = 0; k < 2; ++k) {
@@ -877,7 +891,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         }
     }
     if ((rc = P.h2d(dtab, tab.data(), tab.size() * 16))) return rc;
-    if ((rc = rs_basis_device(ctx, st, e0, e1, dtab, N, cb))) return rc;
+    if ((rc = rs_basis_device(ctx, st, dtab, rs_lo0, rs_lo1, rs_k, rs_hk0, rs_hk1, L - rs_k, cb))) return rc;
   }
   const F128 target = gam[0] * sclaim[0] + gam[1] * sclaim[1];
   B.u64(2);

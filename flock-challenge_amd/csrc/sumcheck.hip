@@ -119,7 +119,7 @@ static unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
 }
 
 int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half,
-                    void *out2, void *E_next) {
+                    void *out2, void *E_next, uint32_t seq) {
   ScopedPhase ph(ctx, "zc_round", st);
   unsigned g = grid_for(half, MUL_THREADS, 4096);
   f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
@@ -128,7 +128,7 @@ int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, cons
   hipLaunchKernelGGL(zc_round_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
                      (const f128 *)E, half, part, (f128 *)E_next);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2);
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
 }
 
 // ---------------------------------------------------------------- zerocheck rounds, split eq
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const void *__r
 // over F128 tables, split at ZS_LOG.  Round 0: 512-thread workgroups (140 KiB,
 // one per CU) over the rows, split at ZS_LOG + 1.
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
-                          uint64_t half, void *out2, void *Ehi_next) {
+                          uint64_t half, void *out2, void *Ehi_next, uint32_t seq) {
   ScopedPhase ph(ctx, "zc_round", st);
   const uint64_t n_hi = half >> ZS_LOG;
   if ((half & ((1ull << ZS_LOG) - 1)) || n_hi < 2) {
@@ -247,11 +247,11 @@ int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B
   hipLaunchKernelGGL((zc_round_split_kernel<256, ZS_LOG, false>), dim3(g), dim3(256), 0, st, A, B, nullptr,
                      (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, (f128 *)Ehi_next);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2);
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
 }
 
 int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
-                           const void *Ehi, uint64_t half, void *out2) {
+                           const void *Ehi, uint64_t half, void *out2, uint32_t seq) {
   ScopedPhase ph(ctx, "zc_round", st);
   constexpr int LOG = ZS_LOG + 1;
   const uint64_t n_hi = half >> LOG;
@@ -265,7 +265,7 @@ int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *
   hipLaunchKernelGGL((zc_round_split_kernel<512, LOG, true>), dim3(g), dim3(512), 0, st, a, b, (const f128 *)d_lw,
                      (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, nullptr);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2);
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
 }
 
 // ---------------------------------------------------------------- zerocheck fold after round 0
@@ -410,7 +410,8 @@ __global__ __launch_bounds__(MUL_THREADS) void lig_msg_kernel(const f128 *__rest
   wg_xor_reduce(y, red, &partial[3 * blockIdx.x + 2]);
 }
 
-int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3) {
+int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
+                   uint32_t seq) {
   ScopedPhase ph(ctx, "lig_msg", st);
   unsigned g = grid_for(half, MUL_THREADS, 4096);
   f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
@@ -418,7 +419,7 @@ int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint6
   hipLaunchKernelGGL(lig_msg_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b, half,
                      with_eval, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 3, out3);
+  return xor_rows_device(ctx, st, part, g, 3, out3, seq);
 }
 
 // acc[i] += beta * x[i]  (SumcheckProver::glue, pcs/ligerito.rs:5766-5790)
@@ -486,32 +487,78 @@ int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t 
 }
 
 // ---------------------------------------------------------------- ring switch
-// b[i] = sum_k sum_bits bit_b(eq_k[i]) w_k[b] with w_k = gamma_k eq(r''_k)
-// (fold_b128_elems, ring_switch.rs:1720-1760, combined as in pcs.rs:1025-1100)
-// through nibble tables T[k][p][v] (2 x 32 x 16 F128, host-built).
-__global__ __launch_bounds__(256) void rs_basis_kernel(const f128 *__restrict__ e0, const f128 *__restrict__ e1,
-                                                       const f128 *__restrict__ T, uint64_t n, f128 *__restrict__ out) {
-  __shared__ f128 tab[2 * 32 * 16];
-  for (uint32_t i = threadIdx.x; i < 2 * 32 * 16; i += 256) tab[i] = T[i];
+// b[i] = sum_k M_k(eq_k[i]), M_k(v) = sum_bits bit_b(v) w_k[b] with
+// w_k = gamma_k eq(r''_k) (fold_b128_elems, ring_switch.rs:1720-1760,
+// combined as in pcs.rs:1025-1100); M_k comes as nibble tables Mt[k][p][v]
+// (2 x 32 x 16 F128, host-built: entry = M_k(v x^(4p))).
+//
+// eq_k[i] = lo_k[i mod 2^k] hi_k[i >> k], and for one x_hi = i >> k the map
+// N_k(v) = M_k(v hi_k[x_hi]) is GF(2)-linear in v, with images
+// N_k(x^b) = M_k(x^b hi_k[x_hi]) of the basis x^b.  The workgroup of an x_hi
+// computes those 2 x 128 images, builds full byte tables of N_0 and N_1 from
+// them (fulltab.h, 64 KiB each) and applies them to lo_k[0 .. 2^k): two
+// conflict-free 16-lookup products per element, and the 2^L-entry eq tables
+// are never written or read.  The images of the workgroup's next x_hi are
+// computed while the current tables are in use (two image buffers), so only
+// the table builds sit between barriers.
+constexpr int RS_THREADS = 1024;
+
+__global__ __launch_bounds__(RS_THREADS) void rs_basis_kernel(const f128 *__restrict__ Mt, const f128 *__restrict__ lo0,
+                                                              const f128 *__restrict__ lo1, int k,
+                                                              const f128 *__restrict__ hi0,
+                                                              const f128 *__restrict__ hi1, uint64_t n_hi,
+                                                              f128 *__restrict__ out) {
+  __shared__ f128 tab[2 * 4096];       // byte tables of N_0, then those of N_1 64 KiB above
+  __shared__ f128 mtab[2 * 32 * 16];   // nibble tables of M_0, M_1
+  __shared__ f128 basis[2][2][128];    // [buffer][claim][i * 16 + kb] = N_claim(x^(8 kb + i))
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < 2 * 32 * 16; i += RS_THREADS) mtab[i] = Mt[i];
   __syncthreads();
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-    const f128 a = e0[i], b = e1[i];
+  // Four adjacent threads compute one image: claim c, power b; quarter q
+  // evaluates M_c on the eight nibbles of word q of x^b hi_c[x_hi].
+  const uint32_t im = tid >> 2, q = tid & 3, c = im >> 7, b = im & 127;
+  auto images = [&](uint64_t xh, int buf) {
+    const f128 y = f128_mul_xpow(c ? hi1[xh] : hi0[xh], b);
+    const uint32_t w = f128_word(y, (int)q);
+    const f128 *t = mtab + c * 512 + q * 128;
     f128 s = f128_zero();
 #pragma unroll
-    for (int p = 0; p < 32; ++p) {
-      const uint32_t wa = p < 8 ? a.x : p < 16 ? a.y : p < 24 ? a.z : a.w;
-      const uint32_t wb = p < 8 ? b.x : p < 16 ? b.y : p < 24 ? b.z : b.w;
-      f128_xor_into(s, tab[p * 16 + ((wa >> (4 * (p & 7))) & 15)]);
-      f128_xor_into(s, tab[512 + p * 16 + ((wb >> (4 * (p & 7))) & 15)]);
+    for (int j = 0; j < 8; ++j) f128_xor_into(s, t[j * 16 + ((w >> (4 * j)) & 15)]);
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+      s.x ^= __shfl_xor(s.x, o);
+      s.y ^= __shfl_xor(s.y, o);
+      s.z ^= __shfl_xor(s.z, o);
+      s.w ^= __shfl_xor(s.w, o);
     }
-    out[i] = s;
+    if (q == 0) basis[buf][c][(b & 7) * 16 + (b >> 3)] = s;
+  };
+  const FulltabLane L(tid & 15);
+  const uint64_t lo_n = 1ull << k;
+  int buf = 0;
+  if (blockIdx.x < n_hi) images(blockIdx.x, 0);
+  for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x, buf ^= 1) {
+    __syncthreads();  // basis[buf] is complete; the previous x_hi's table reads are done
+    build_fulltab<RS_THREADS>(&basis[buf][0][0], tab, tid);
+    build_fulltab<RS_THREADS>(&basis[buf][1][0], tab + 4096, tid);
+    __syncthreads();
+    if (xh + gridDim.x < n_hi) images(xh + gridDim.x, buf ^ 1);
+    for (uint64_t il = tid; il < lo_n; il += RS_THREADS) {
+      f128 p0, p1;
+      f128_mul2_fulltab<false, true>(lo0[il], lo1[il], (const char *)tab, L, p0, p1);
+      out[(xh << k) + il] = f128_xor(p0, p1);
+    }
   }
 }
 
-int rs_basis_device(Ctx *ctx, hipStream_t st, const void *e0, const void *e1, const void *tables, uint64_t n, void *out) {
+int rs_basis_device(Ctx *ctx, hipStream_t st, const void *tables, const void *lo0, const void *lo1, int k,
+                    const void *hi0, const void *hi1, int n_hi_log, void *out) {
   ScopedPhase ph(ctx, "rs_basis", st);
-  hipLaunchKernelGGL(rs_basis_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (const f128 *)e0,
-                     (const f128 *)e1, (const f128 *)tables, n, (f128 *)out);
+  const uint64_t n_hi = 1ull << n_hi_log;
+  // one workgroup per CU (148 KiB of LDS), each keeping M_k for all its x_hi's
+  const unsigned g = (unsigned)(n_hi < 256 ? n_hi : 256);
+  hipLaunchKernelGGL(rs_basis_kernel, dim3(g), dim3(RS_THREADS), 0, st, (const f128 *)tables, (const f128 *)lo0,
+                     (const f128 *)lo1, k, (const f128 *)hi0, (const f128 *)hi1, n_hi, (f128 *)out);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
@@ -682,16 +729,21 @@ int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t
 
 // Hand the window's result to the host and re-arm the scan: out = best,
 // best = none (so the next window needs no upload).
-__global__ void pow_take_kernel(unsigned long long *__restrict__ best, unsigned long long *__restrict__ out) {
+__global__ void pow_take_kernel(unsigned long long *__restrict__ best, unsigned long long *__restrict__ out,
+                                uint32_t *flag, uint32_t seq) {
   if (threadIdx.x == 0) {
     out[0] = best[0];
     best[0] = ~0ull;
+    if (seq) {
+      __threadfence_system();
+      __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
-int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out) {
+int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out, uint32_t seq) {
   hipLaunchKernelGGL(pow_take_kernel, dim3(1), dim3(64), 0, st, (unsigned long long *)d_best,
-                     (unsigned long long *)out);
+                     (unsigned long long *)out, ctx->readback_flag_dev(), seq);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }

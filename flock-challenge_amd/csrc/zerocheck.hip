@@ -192,23 +192,64 @@ __global__ __launch_bounds__(ZCY_THREADS) void zc_y_kernel(const uint64_t *__res
   }
 }
 
-// XOR-sum `count` rows of `width` F128 each: out[w] = xor_r in[r * width + w].
-__global__ void xor_rows_kernel(const f128 *__restrict__ in, uint64_t count, uint32_t width, f128 *__restrict__ out) {
-  const uint32_t w = blockIdx.x;
-  f128 s = f128_zero();
-  for (uint64_t r = threadIdx.x; r < count; r += blockDim.x) f128_xor_into(s, in[r * width + w]);
-  __shared__ f128 red[256];
-  red[threadIdx.x] = s;
+// XOR-sum `count` rows of W F128 each, out[w] = xor_r in[r * W + w], in one
+// workgroup (a 256-thread tree per column; four rows per thread in flight).  With seq != 0 it then publishes
+// seq at *flag after out is written (as signal_kernel does), so a host
+// waiting for out needs no further launch.
+template <int W>
+__global__ __launch_bounds__(256) void xor_rows_kernel(const f128 *__restrict__ in, uint64_t count,
+                                                       f128 *__restrict__ out, uint32_t *flag, uint32_t seq) {
+  __shared__ f128 red[W][256];
+  const uint32_t tid = threadIdx.x;
+  f128 s[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) s[w] = f128_zero();
+  for (uint64_t r0 = 0; r0 < count; r0 += 1024) {
+    f128 v[4][W];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t r = r0 + (uint64_t)u * 256 + tid;
+#pragma unroll
+      for (int w = 0; w < W; ++w) v[u][w] = r < count ? in[r * W + w] : f128_zero();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int w = 0; w < W; ++w) f128_xor_into(s[w], v[u][w]);
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) red[w][tid] = s[w];
   __syncthreads();
-  for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) red[threadIdx.x] = f128_xor(red[threadIdx.x], red[threadIdx.x + st]);
+  for (uint32_t st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+#pragma unroll
+      for (int w = 0; w < W; ++w) red[w][tid] = f128_xor(red[w][tid], red[w][tid + st]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[w] = red[0];
+  if (tid == 0) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) out[w] = red[w][0];
+    if (seq) {
+      __threadfence_system();
+      __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
 }
 
-int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out) {
-  hipLaunchKernelGGL(xor_rows_kernel, dim3(width), dim3(256), 0, st, (const f128 *)in, count, width, (f128 *)out);
+int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out,
+                    uint32_t seq) {
+  const f128 *i = (const f128 *)in;
+  f128 *o = (f128 *)out;
+  uint32_t *flag = ctx->readback_flag_dev();
+  switch (width) {
+    case 1: hipLaunchKernelGGL(xor_rows_kernel<1>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
+    case 2: hipLaunchKernelGGL(xor_rows_kernel<2>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
+    case 3: hipLaunchKernelGGL(xor_rows_kernel<3>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
+    case 4: hipLaunchKernelGGL(xor_rows_kernel<4>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
+    default:
+      ctx->last_error = "xor_rows: width out of range";
+      return FLOCK_E_SHAPE;
+  }
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
