@@ -498,9 +498,14 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   // round 0 splits at ZS_LOG + 1 and rounds 1 .. n_split at ZS_LOG, so both
   // start from Ehi = eq(r[7 + ZS_LOG + 1 ..]), which each later split round
   // folds for the next.  They depend on r only and are prepared while the
-  // device works on round 1.
-  const int n_split = n - 2 - ZS_LOG;
-  if (n_split < 1) {  // m >= 22 for every Ligerito configuration
+  // device works on round 1.  Rounds with half <= 2^ZC_WHOLE_LOG use the
+  // whole table instead: a split round's workgroup walks 2^ZS_LOG pairs per
+  // x_hi in a serial chain, so with few x_hi blocks it is latency-bound
+  // (about 25 us however small), while the whole-table kernel spreads the
+  // pairs over all CUs and materializing E costs at most 4 MiB.
+  constexpr int ZC_WHOLE_LOG = 18;
+  const int n_split = std::max(1, std::min(n - 2 - ZS_LOG, n - 2 - ZC_WHOLE_LOG));
+  if (n - 2 - ZS_LOG < 1) {  // m >= 22 for every Ligerito configuration
     ctx->last_error = "zerocheck: m too small for the split eq tables";
     return FLOCK_E_SHAPE;
   }
@@ -596,8 +601,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // tables are materialized.
     void *A = P.dev("zc.A", rows * 8), *Bv = P.dev("zc.B", rows * 8);
     void *A2 = P.dev("zc.A2", rows * 4), *B2 = P.dev("zc.B2", rows * 4);
-    // whole eq tables for the rounds after n_split (from 2^ZS_LOG entries down)
-    const size_t e_len = (size_t)1 << ZS_LOG;
+    // whole eq tables for the rounds after n_split (from 2^(n - 2 - n_split)
+    // entries down)
+    const size_t e_len = (size_t)1 << (n - 2 - n_split);
     void *E = P.dev("zc.E", e_len * 16), *E2 = P.dev("zc.E2", e_len * 8);
     void *dlw8 = P.dev("zc.lw8", 2048 * 16), *dlwr8 = P.dev("zc.lwr8", 2048 * 16), *dlw = P.dev("zc.lw", 256 * 16);
     if (!A || !Bv || !A2 || !B2 || !E || !E2 || !dlw8 || !dlwr8 || !dlw) return FLOCK_E_OOM;
@@ -623,6 +629,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       }
       if ((rc = P.h2d(dlw, lw, sizeof lw))) return rc;
     }
+    // the first whole-table round (n_split + 1) starts from eq itself; it
+    // depends on the point only, so it is queued now
+    if (n_split + 1 < n && (rc = P.eq_device(&r[8 + n_split], n - 2 - n_split, E))) return rc;
     size_t len = rows;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
@@ -636,8 +645,6 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
                                    ctx->readback_dev, i < n_split ? Ehi2 : nullptr, seq);
         std::swap(Ehi, Ehi2);
       } else {
-        // the first whole-table round after the split ones starts from eq itself
-        if (i == n_split + 1 && (rc = P.eq_device(&r[7 + i], n - 1 - i, E))) return rc;
         rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr, seq);
       }
       if (rc) return rc;
