@@ -235,29 +235,47 @@ struct Prover {
   // 2^8 and 2^(n-16) entries, expanded on the device in two steps.
   int eq_device(const F128 *r, int n, void *out) {
     const int a = std::min(n, 8), b = std::min(n - a, 8), c = n - a - b;
-    std::vector<F128> lo, mid, hi;
+    // the factor tables lo | mid | hi go up in one copy (into a buffer of at
+    // least 1024 entries, so it is not regrown between calls)
+    const size_t na = (size_t)1 << a, nb = (size_t)1 << b, nc = c ? (size_t)1 << c : 0;
+    std::vector<F128> tabs(na + nb + nc);
     {
       HostPhase hp(ctx, "eq_tables");
-      lo = eq_table(r, a);
-      mid = eq_table(r + a, b);
-      if (c) hi = eq_table(r + a + b, c);
+      build_eq(r, a, tabs.data());
+      build_eq(r + a, b, tabs.data() + na);
+      if (c) build_eq(r + a + b, c, tabs.data() + na + nb);
     }
-    void *dlo = dev("eq.lo", lo.size() * 16), *dmid = dev("eq.mid", mid.size() * 16);
-    if (!dlo || !dmid) return FLOCK_E_OOM;
-    int rc = h2d(dlo, lo.data(), lo.size() * 16);
-    if (!rc) rc = h2d(dmid, mid.data(), mid.size() * 16);
-    if (rc) return rc;
+    uint8_t *d = (uint8_t *)dev("eq.tabs", std::max<size_t>(tabs.size(), 1024) * 16);
+    if (!d) return FLOCK_E_OOM;
+    if (int rc = h2d(d, tabs.data(), tabs.size() * 16)) return rc;
+    const uint8_t *dlo = d, *dmid = d + na * 16, *dhi = dmid + nb * 16;
     if (!c) return eq_expand_device(ctx, st, dlo, a, dmid, b, out);
-    void *dt = dev("eq.lomid", (16ull << (a + b))), *dhi = dev("eq.hi", hi.size() * 16);
-    if (!dt || !dhi) return FLOCK_E_OOM;
-    if ((rc = h2d(dhi, hi.data(), hi.size() * 16))) return rc;
-    if ((rc = eq_expand_device(ctx, st, dlo, a, dmid, b, dt))) return rc;
+    void *dt = dev("eq.lomid", 16ull << (a + b));
+    if (!dt) return FLOCK_E_OOM;
+    if (int rc = eq_expand_device(ctx, st, dlo, a, dmid, b, dt)) return rc;
     return eq_expand_device(ctx, st, dt, a + b, dhi, c, out);
   }
 
+  // Fold grinds of up to this many bits run on the host: the expected 2^bits
+  // single-block compressions (a few microseconds) cost less than one device
+  // scan and its readback.
+  static constexpr uint32_t HOST_GRIND_BITS = 6;
+
   int grind(Transcript &t, uint32_t bits, uint64_t *nonce_out) {
     uint64_t nonce = 0;
-    if (bits) {
+    if (bits && bits <= HOST_GRIND_BITS) {
+      // the smallest satisfying nonce, as the device scan finds it
+      HostPhase hp(ctx, "pow_host");
+      uint32_t m[16] = {0};
+      t.state_digest((uint8_t *)m);
+      for (;; ++nonce) {
+        m[8] = (uint32_t)nonce;
+        m[9] = (uint32_t)(nonce >> 32);
+        uint32_t o[16];
+        hb3::compress(hb3::IV, m, 0, 64, hb3::CHUNK_START | hb3::CHUNK_END | hb3::ROOT, o);
+        if ((__builtin_bswap32(o[0]) >> (32 - bits)) == 0) break;
+      }
+    } else if (bits) {
       ScopedPhase ph(ctx, "pow", st);
       uint8_t dg[32];
       t.state_digest(dg);
@@ -721,17 +739,15 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // quirky eq table (lincheck.rs:1127-1142): index i_skip + 64 i_rest,
     // the outer product lam(i_skip) eq(rhos[0..8], i_rest) expanded on the device
     {
-      F128 lam[64];
-      std::vector<F128> er;
+      F128 lr[64 + 256];  // lam | eq(rhos[0..8]), uploaded in one copy
       {
         HostPhase hp(ctx, "lincheck_eq");
-        lagrange(zz, 0, 64, 0, 64, lam);
-        er = eq_table(rhos.data(), 8);
+        lagrange(zz, 0, 64, 0, 64, lr);
+        build_eq(rhos.data(), 8, lr + 64);
       }
-      void *dlam = P.dev("lc.lam", 64 * 16), *der = P.dev("lc.er", 256 * 16);
-      if (!dlam || !der) return FLOCK_E_OOM;
-      if ((rc = P.h2d(dlam, lam, sizeof lam)) || (rc = P.h2d(der, er.data(), er.size() * 16)) ||
-          (rc = eq_expand_device(ctx, st, dlam, 6, der, 8, deq)))
+      uint8_t *dlr = (uint8_t *)P.dev("lc.lamer", sizeof lr);
+      if (!dlr) return FLOCK_E_OOM;
+      if ((rc = P.h2d(dlr, lr, sizeof lr)) || (rc = eq_expand_device(ctx, st, dlr, 6, dlr + 64 * 16, 8, deq)))
         return rc;
     }
     if ((rc = lc_comb_device(ctx, st, dap, dar, dbp, dbr, deq, alpha, beta, r1cs::Z_CONST, dcomb))) return rc;
