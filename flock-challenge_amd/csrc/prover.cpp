@@ -651,11 +651,21 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // depends on the point only, so it is queued now
     if (n_split + 1 < n && (rc = P.eq_device(&r[8 + n_split], n - 2 - n_split, E))) return rc;
     size_t len = rows;
+    // A small whole-table round folds the previous round's tables itself:
+    // then the fold by prev_rho is pending when the round starts.
+    bool fold_pending = false;
+    F128 prev_rho;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
       const bool split = i <= n_split;
       const uint32_t seq = ctx->next_readback_seq();
-      if (i == 0) {
+      if (fold_pending) {
+        rc = zc_fold_round_small_device(ctx, st, A, Bv, prev_rho, E, half, A2, B2, i + 1 < n ? E2 : nullptr,
+                                        ctx->readback_dev, seq);
+        std::swap(A, A2);
+        std::swap(Bv, B2);
+        fold_pending = false;
+      } else if (i == 0) {
         rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev, seq);
       } else if (split) {
         rc = zc_round_split_device(ctx, st, A, Bv,
@@ -682,6 +692,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         byte_tables(wr, lwr8.data());
         if ((rc = P.h2d(dlwr8, lwr8.data(), lwr8.size() * 16))) return rc;
         if ((rc = zc_fold0_device(ctx, st, a, b, dlw8, dlwr8, half, A, Bv))) return rc;
+      } else if (i + 1 < n && i + 1 > n_split && half / 2 <= SMALL_PAIRS) {
+        prev_rho = rho;  // folded by round i + 1's launch
+        fold_pending = true;
       } else {
         if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2))) return rc;
         std::swap(A, A2);
@@ -948,6 +961,18 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     };
     auto fold = [&](F128 rv) -> int {
       void *nf = (f == fb) ? fc : fb, *ncb = (cur_cb == cbb) ? cbc : cbb;
+      if (len >= 4 && len / 4 <= SMALL_PAIRS) {  // fold and message in one launch
+        const uint32_t seq = ctx->next_readback_seq();
+        int e = lig_fold_msg_small_device(ctx, st, f, cur_cb, rv, len / 4, nf, ncb, ctx->readback_dev, seq);
+        uint4 h[3];
+        if (!e) e = P.fetch(h, 48, seq);
+        if (e) return e;
+        f = nf;
+        cur_cb = ncb;
+        len /= 2;
+        push(fv(h[0]), fv(h[1]));
+        return FLOCK_OK;
+      }
       int e = fold_pair_device(ctx, st, f, cur_cb, len / 2, rv, nf, ncb);
       if (e) return e;
       f = nf;

@@ -385,6 +385,143 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
   return FLOCK_OK;
 }
 
+// ---------------------------------------------------------------- small rounds, fused
+// The last rounds of the zerocheck and Ligerito sumchecks have at most
+// SMALL_PAIRS pairs: one workgroup then folds the previous round's tables by
+// its challenge (writing them, fold_pair_kernel), computes the new round's
+// message over the folded values, reduces it and publishes it to the host
+// (xor_rows_kernel): one launch per round instead of three.  Thread x takes
+// pair x, i.e. folded positions 2x, 2x + 1 from positions 4x .. 4x + 3.
+constexpr int SMALL_THREADS = 256;
+
+// v[4x] + r (v[4x] + v[4x+1]) and v[4x+2] + r (v[4x+2] + v[4x+3]), with the
+// lane's comb table holding r.
+__device__ __forceinline__ void fold4(const f128 *__restrict__ v, uint32_t x, const f128 *tab, const CombLane &CL,
+                                      f128 &o0, f128 &o1) {
+  const f128 a0 = v[4 * x], a1 = v[4 * x + 1], a2 = v[4 * x + 2], a3 = v[4 * x + 3];
+  o0 = f128_xor(a0, f128_mul_comb(f128_xor(a0, a1), tab, CL));
+  o1 = f128_xor(a2, f128_mul_comb(f128_xor(a2, a3), tab, CL));
+}
+
+// XOR of s0, s1 over the workgroup into out[0], out[1] (out[2] = 0 when
+// three are written), then seq published at *flag.  red: 512 F128 of LDS no
+// longer in use.
+__device__ __forceinline__ void small_publish(f128 s0, f128 s1, f128 *red, f128 *out, int n_out, uint32_t *flag,
+                                              uint32_t seq) {
+  const uint32_t tid = threadIdx.x;
+  __syncthreads();  // every lane's comb table reads are done
+  red[tid] = s0;
+  red[SMALL_THREADS + tid] = s1;
+  __syncthreads();
+  for (uint32_t st = SMALL_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[tid] = f128_xor(red[tid], red[tid + st]);
+      red[SMALL_THREADS + tid] = f128_xor(red[SMALL_THREADS + tid], red[SMALL_THREADS + tid + st]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = red[0];
+    out[1] = red[SMALL_THREADS];
+    if (n_out > 2) out[2] = f128_zero();
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Zerocheck round over `half` pairs after folding A, B (4 half entries each)
+// by rho into A2, B2 (2 half): (G(1), G(inf)) as zc_round_kernel, and
+// E_next[y] = E[2y] + E[2y+1] unless E_next is null.
+__global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
+    const f128 *__restrict__ A, const f128 *__restrict__ B, f128 rho, const f128 *__restrict__ E, uint32_t half,
+    f128 *__restrict__ A2, f128 *__restrict__ B2, f128 *__restrict__ E_next, f128 *__restrict__ out, uint32_t *flag,
+    uint32_t seq) {
+  __shared__ f128 mt[SMALL_THREADS / 64][1024];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
+  const uint32_t x = threadIdx.x;
+  f128 g1 = f128_zero(), gi = f128_zero();
+  if (x < half) {
+    f128 a0, a1, b0, b1;
+    f128_mul_build(rho, tab, CL);
+    fold4(A, x, tab, CL, a0, a1);
+    fold4(B, x, tab, CL, b0, b1);
+    A2[2 * x] = a0;
+    A2[2 * x + 1] = a1;
+    B2[2 * x] = b0;
+    B2[2 * x + 1] = b1;
+    const f128 p1 = f128_mul(a1, b1, tab, CL);
+    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
+    const f128 e = E[x];
+    if (E_next && !(x & 1)) E_next[x >> 1] = f128_xor(e, E[x + 1]);
+    f128_mul_build(e, tab, CL);
+    g1 = f128_mul_comb(p1, tab, CL);
+    gi = f128_mul_comb(pi, tab, CL);
+  }
+  small_publish(g1, gi, tab, out, 2, flag, seq);
+}
+
+// Ligerito round over `half` pairs after folding f, b (4 half entries each)
+// by r into f2, b2: (u0, u2) as lig_msg_kernel without the evaluation.
+__global__ __launch_bounds__(SMALL_THREADS) void lig_fold_msg_small_kernel(const f128 *__restrict__ f,
+                                                                           const f128 *__restrict__ b, f128 r,
+                                                                           uint32_t half, f128 *__restrict__ f2,
+                                                                           f128 *__restrict__ b2,
+                                                                           f128 *__restrict__ out, uint32_t *flag,
+                                                                           uint32_t seq) {
+  __shared__ f128 mt[SMALL_THREADS / 64][1024];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
+  const uint32_t x = threadIdx.x;
+  f128 u0 = f128_zero(), u2 = f128_zero();
+  if (x < half) {
+    f128 f0, f1, b0, b1;
+    f128_mul_build(r, tab, CL);
+    fold4(f, x, tab, CL, f0, f1);
+    fold4(b, x, tab, CL, b0, b1);
+    f2[2 * x] = f0;
+    f2[2 * x + 1] = f1;
+    b2[2 * x] = b0;
+    b2[2 * x + 1] = b1;
+    u0 = f128_mul(f0, b0, tab, CL);
+    u2 = f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, CL);
+  }
+  small_publish(u0, u2, tab, out, 3, flag, seq);
+}
+
+static f128 to_f128(F128 v) {
+  return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32));
+}
+
+int zc_fold_round_small_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
+                               uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq) {
+  ScopedPhase ph(ctx, "zc_round", st);
+  if (half == 0 || half > SMALL_PAIRS) {
+    ctx->last_error = "zerocheck: fused small round needs 1 .. SMALL_PAIRS pairs";
+    return FLOCK_E_SHAPE;
+  }
+  if (half < 2) E_next = nullptr;
+  hipLaunchKernelGGL(zc_fold_round_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)A,
+                     (const f128 *)B, to_f128(rho), (const f128 *)E, (uint32_t)half, (f128 *)A2, (f128 *)B2,
+                     (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+int lig_fold_msg_small_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half,
+                              void *f2, void *b2, void *out3, uint32_t seq) {
+  ScopedPhase ph(ctx, "lig_msg", st);
+  if (half == 0 || half > SMALL_PAIRS) {
+    ctx->last_error = "ligerito: fused small round needs 1 .. SMALL_PAIRS pairs";
+    return FLOCK_E_SHAPE;
+  }
+  hipLaunchKernelGGL(lig_fold_msg_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)f,
+                     (const f128 *)b, to_f128(r), (uint32_t)half, (f128 *)f2, (f128 *)b2, (f128 *)out3,
+                     ctx->readback_flag_dev(), seq);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
 // ---------------------------------------------------------------- Ligerito sumcheck
 // round_msg_lsb (pcs/ligerito.rs:3725-3772): u0 = sum f0 b0,
 // u2 = sum (f0 + f1)(b0 + b1); with_eval adds y = sum f0 b0 + f1 b1
