@@ -60,17 +60,19 @@ int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *
 int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *d_lwr8,
                     uint64_t half, void *A2, void *B2);
 int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
-// The last rounds, at most SMALL_PAIRS pairs, in one launch each: fold the
-// previous round's tables (4 half entries) by its challenge into A2, B2
-// (2 half), compute the new round's message over them and publish it (seq)
-// as xor_rows_device does.  Zerocheck: out2 = (G(1), G(inf)) with the whole
-// eq table E (half entries), E_next as zc_round_device; Ligerito: out3 =
-// (u0, u2, 0) as lig_msg_device without the evaluation.
+// A round fused with the fold before it: fold the previous round's tables
+// (4 half entries) by its challenge into A2, B2 (2 half), compute the new
+// round's message over them and publish it (seq) as xor_rows_device does.
+// Up to SMALL_PAIRS pairs this is one single-workgroup launch (reduction
+// included), above it a grid plus the reduction.  Zerocheck: out2 =
+// (G(1), G(inf)) with the whole eq table E (half entries), E_next as
+// zc_round_device; Ligerito: out3 = (u0, u2, 0) as lig_msg_device without the
+// evaluation.
 constexpr uint64_t SMALL_PAIRS = 256;
-int zc_fold_round_small_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
-                               uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq);
-int lig_fold_msg_small_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half,
-                              void *f2, void *b2, void *out3, uint32_t seq);
+int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
+                         uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq);
+int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half, void *f2,
+                        void *b2, void *out3, uint32_t seq);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
                    uint32_t seq = 0);
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);

@@ -651,8 +651,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     // depends on the point only, so it is queued now
     if (n_split + 1 < n && (rc = P.eq_device(&r[8 + n_split], n - 2 - n_split, E))) return rc;
     size_t len = rows;
-    // A small whole-table round folds the previous round's tables itself:
-    // then the fold by prev_rho is pending when the round starts.
+    // A whole-table round folds the previous round's tables itself (one
+    // launch fewer per round): then the fold by prev_rho is pending when the
+    // round starts.
     bool fold_pending = false;
     F128 prev_rho;
     for (int i = 0; i < n; ++i) {
@@ -660,8 +661,8 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       const bool split = i <= n_split;
       const uint32_t seq = ctx->next_readback_seq();
       if (fold_pending) {
-        rc = zc_fold_round_small_device(ctx, st, A, Bv, prev_rho, E, half, A2, B2, i + 1 < n ? E2 : nullptr,
-                                        ctx->readback_dev, seq);
+        rc = zc_fold_round_device(ctx, st, A, Bv, prev_rho, E, half, A2, B2, i + 1 < n ? E2 : nullptr,
+                                  ctx->readback_dev, seq);
         std::swap(A, A2);
         std::swap(Bv, B2);
         fold_pending = false;
@@ -692,7 +693,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         byte_tables(wr, lwr8.data());
         if ((rc = P.h2d(dlwr8, lwr8.data(), lwr8.size() * 16))) return rc;
         if ((rc = zc_fold0_device(ctx, st, a, b, dlw8, dlwr8, half, A, Bv))) return rc;
-      } else if (i + 1 < n && i + 1 > n_split && half / 2 <= SMALL_PAIRS) {
+      } else if (i + 1 < n && i + 1 > n_split) {
         prev_rho = rho;  // folded by round i + 1's launch
         fold_pending = true;
       } else {
@@ -961,9 +962,11 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     };
     auto fold = [&](F128 rv) -> int {
       void *nf = (f == fb) ? fc : fb, *ncb = (cur_cb == cbb) ? cbc : cbb;
-      if (len >= 4 && len / 4 <= SMALL_PAIRS) {  // fold and message in one launch
+      // fold and message in one launch, up to 2^16 pairs (above, the separate
+      // fold's full byte tables of rv cost less than the fused kernel's combs)
+      if (len >= 4 && len / 4 <= (1u << 16)) {
         const uint32_t seq = ctx->next_readback_seq();
-        int e = lig_fold_msg_small_device(ctx, st, f, cur_cb, rv, len / 4, nf, ncb, ctx->readback_dev, seq);
+        int e = lig_fold_msg_device(ctx, st, f, cur_cb, rv, len / 4, nf, ncb, ctx->readback_dev, seq);
         uint4 h[3];
         if (!e) e = P.fetch(h, 48, seq);
         if (e) return e;

@@ -396,7 +396,7 @@ constexpr int SMALL_THREADS = 256;
 
 // v[4x] + r (v[4x] + v[4x+1]) and v[4x+2] + r (v[4x+2] + v[4x+3]), with the
 // lane's comb table holding r.
-__device__ __forceinline__ void fold4(const f128 *__restrict__ v, uint32_t x, const f128 *tab, const CombLane &CL,
+__device__ __forceinline__ void fold4(const f128 *__restrict__ v, uint64_t x, const f128 *tab, const CombLane &CL,
                                       f128 &o0, f128 &o1) {
   const f128 a0 = v[4 * x], a1 = v[4 * x + 1], a2 = v[4 * x + 2], a3 = v[4 * x + 3];
   o0 = f128_xor(a0, f128_mul_comb(f128_xor(a0, a1), tab, CL));
@@ -489,37 +489,117 @@ __global__ __launch_bounds__(SMALL_THREADS) void lig_fold_msg_small_kernel(const
   small_publish(u0, u2, tab, out, 3, flag, seq);
 }
 
+// The same fused rounds over more pairs: a grid of MUL_THREADS-thread
+// workgroups (grid-stride over the pairs), one partial per workgroup, reduced
+// and published by xor_rows_kernel.  The fold's products use comb tables of
+// the challenge (rebuilt per pair: the message's products overwrite them).
+__global__ __launch_bounds__(MUL_THREADS) void zc_fold_round_kernel(const f128 *__restrict__ A,
+                                                                    const f128 *__restrict__ B, f128 rho,
+                                                                    const f128 *__restrict__ E, uint64_t half,
+                                                                    f128 *__restrict__ A2, f128 *__restrict__ B2,
+                                                                    f128 *__restrict__ E_next,
+                                                                    f128 *__restrict__ partial) {
+  __shared__ f128 mt[MUL_WAVES][1024];
+  __shared__ f128 red[MUL_THREADS];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
+  f128 g1 = f128_zero(), gi = f128_zero();
+  for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
+    f128 a0, a1, b0, b1;
+    f128_mul_build(rho, tab, CL);
+    fold4(A, x, tab, CL, a0, a1);
+    fold4(B, x, tab, CL, b0, b1);
+    A2[2 * x] = a0;
+    A2[2 * x + 1] = a1;
+    B2[2 * x] = b0;
+    B2[2 * x + 1] = b1;
+    const f128 p1 = f128_mul(a1, b1, tab, CL);
+    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
+    const f128 e = E[x];
+    if (E_next && !(x & 1)) E_next[x >> 1] = f128_xor(e, E[x + 1]);
+    f128_mul_build(e, tab, CL);
+    f128_xor_into(g1, f128_mul_comb(p1, tab, CL));
+    f128_xor_into(gi, f128_mul_comb(pi, tab, CL));
+  }
+  wg_xor_reduce(g1, red, &partial[2 * blockIdx.x]);
+  wg_xor_reduce(gi, red, &partial[2 * blockIdx.x + 1]);
+}
+
+__global__ __launch_bounds__(MUL_THREADS) void lig_fold_msg_kernel(const f128 *__restrict__ f,
+                                                                   const f128 *__restrict__ b, f128 r, uint64_t half,
+                                                                   f128 *__restrict__ f2, f128 *__restrict__ b2,
+                                                                   f128 *__restrict__ partial) {
+  __shared__ f128 mt[MUL_WAVES][1024];
+  __shared__ f128 red[MUL_THREADS];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(threadIdx.x);
+  f128 u0 = f128_zero(), u2 = f128_zero();
+  for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
+    f128 f0, f1, b0, b1;
+    f128_mul_build(r, tab, CL);
+    fold4(f, x, tab, CL, f0, f1);
+    fold4(b, x, tab, CL, b0, b1);
+    f2[2 * x] = f0;
+    f2[2 * x + 1] = f1;
+    b2[2 * x] = b0;
+    b2[2 * x + 1] = b1;
+    f128_xor_into(u0, f128_mul(f0, b0, tab, CL));
+    f128_xor_into(u2, f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, CL));
+  }
+  wg_xor_reduce(u0, red, &partial[3 * blockIdx.x]);
+  wg_xor_reduce(u2, red, &partial[3 * blockIdx.x + 1]);
+  if (threadIdx.x == 0) partial[3 * blockIdx.x + 2] = f128_zero();
+}
+
 static f128 to_f128(F128 v) {
   return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32));
 }
 
-int zc_fold_round_small_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
+int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
                                uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq) {
   ScopedPhase ph(ctx, "zc_round", st);
-  if (half == 0 || half > SMALL_PAIRS) {
-    ctx->last_error = "zerocheck: fused small round needs 1 .. SMALL_PAIRS pairs";
+  if (half == 0) {
+    ctx->last_error = "zerocheck: fused round needs at least one pair";
     return FLOCK_E_SHAPE;
   }
   if (half < 2) E_next = nullptr;
-  hipLaunchKernelGGL(zc_fold_round_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)A,
-                     (const f128 *)B, to_f128(rho), (const f128 *)E, (uint32_t)half, (f128 *)A2, (f128 *)B2,
-                     (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq);
+  if (half <= SMALL_PAIRS) {
+    hipLaunchKernelGGL(zc_fold_round_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)A,
+                       (const f128 *)B, to_f128(rho), (const f128 *)E, (uint32_t)half, (f128 *)A2, (f128 *)B2,
+                       (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq);
+    FLOCK_HIP_CHECK(ctx, hipGetLastError());
+    return FLOCK_OK;
+  }
+  const unsigned g = grid_for(half, MUL_THREADS, 4096);
+  f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL(zc_fold_round_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
+                     to_f128(rho), (const f128 *)E, half, (f128 *)A2, (f128 *)B2, (f128 *)E_next, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return FLOCK_OK;
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
 }
 
-int lig_fold_msg_small_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half,
+int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half,
                               void *f2, void *b2, void *out3, uint32_t seq) {
   ScopedPhase ph(ctx, "lig_msg", st);
-  if (half == 0 || half > SMALL_PAIRS) {
-    ctx->last_error = "ligerito: fused small round needs 1 .. SMALL_PAIRS pairs";
+  if (half == 0) {
+    ctx->last_error = "ligerito: fused round needs at least one pair";
     return FLOCK_E_SHAPE;
   }
-  hipLaunchKernelGGL(lig_fold_msg_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)f,
-                     (const f128 *)b, to_f128(r), (uint32_t)half, (f128 *)f2, (f128 *)b2, (f128 *)out3,
-                     ctx->readback_flag_dev(), seq);
+  if (half <= SMALL_PAIRS) {
+    hipLaunchKernelGGL(lig_fold_msg_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)f,
+                       (const f128 *)b, to_f128(r), (uint32_t)half, (f128 *)f2, (f128 *)b2, (f128 *)out3,
+                       ctx->readback_flag_dev(), seq);
+    FLOCK_HIP_CHECK(ctx, hipGetLastError());
+    return FLOCK_OK;
+  }
+  const unsigned g = grid_for(half, MUL_THREADS, 4096);
+  f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
+  if (!part) return FLOCK_E_OOM;
+  hipLaunchKernelGGL(lig_fold_msg_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b,
+                     to_f128(r), half, (f128 *)f2, (f128 *)b2, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return FLOCK_OK;
+  return xor_rows_device(ctx, st, part, g, 3, out3, seq);
 }
 
 // ---------------------------------------------------------------- Ligerito sumcheck
