@@ -36,16 +36,12 @@ int bitfold_device(Ctx *ctx, hipStream_t st, const void *elems, int il_log, uint
 // out[y][w] = xor_x in[y][x][w] (ny stacks of nx rows of width F128).
 int xor_reduce_device(Ctx *ctx, hipStream_t st, const void *in, uint32_t ny, uint32_t nx, uint32_t width, void *out);
 int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void *hi, int n_hi_log, void *out);
-// (G(1), G(inf)) of one multilinear zerocheck round; writes the next round's
-// eq table into E_next (half / 2 entries) unless it is null.  Here and in the
-// other reductions below, a nonzero seq is published once out2 is written
-// (xor_rows_device).
-int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half, void *out2,
-                    void *E_next, uint32_t seq = 0);
-// The same round with the eq table split as Elo (x) Ehi, Elo over the low
-// ZS_LOG variables (2^ZS_LOG entries), Ehi over the rest (half >> ZS_LOG
-// entries, at least 2); writes Ehi folded for the next round into Ehi_next
-// (half >> (ZS_LOG + 1) entries) unless it is null.
+// (G(1), G(inf)) of one multilinear zerocheck round with the eq table split
+// as Elo (x) Ehi, Elo over the low ZS_LOG variables (2^ZS_LOG entries), Ehi
+// over the rest (half >> ZS_LOG entries, at least 2); writes Ehi folded for
+// the next round into Ehi_next (half >> (ZS_LOG + 1) entries) unless it is
+// null.  Here and in the other reductions below, a nonzero seq is published
+// once the result (out2 / out3) is written (xor_rows_device).
 constexpr int ZS_LOG = 10;
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
                           uint64_t half, void *out2, void *Ehi_next, uint32_t seq = 0);
@@ -65,8 +61,8 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
 // round's message over them and publish it (seq) as xor_rows_device does.
 // Up to SMALL_PAIRS pairs this is one single-workgroup launch (reduction
 // included), above it a grid plus the reduction.  Zerocheck: out2 =
-// (G(1), G(inf)) with the whole eq table E (half entries), E_next as
-// zc_round_device; Ligerito: out3 = (u0, u2, 0) as lig_msg_device without the
+// (G(1), G(inf)) with the whole eq table E (half entries) and the next
+// round's eq table (E folded) into E_next unless it is null; Ligerito: out3 = (u0, u2, 0) as lig_msg_device without the
 // evaluation.
 constexpr uint64_t SMALL_PAIRS = 256;
 int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,

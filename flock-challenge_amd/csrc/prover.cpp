@@ -668,13 +668,11 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         fold_pending = false;
       } else if (i == 0) {
         rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev, seq);
-      } else if (split) {
+      } else {  // 1 <= i <= n_split: every later round is fused with its fold
         rc = zc_round_split_device(ctx, st, A, Bv,
                                    (const uint8_t *)dElo + (elo0_len + ((size_t)(i - 1) << ZS_LOG)) * 16, Ehi, half,
                                    ctx->readback_dev, i < n_split ? Ehi2 : nullptr, seq);
         std::swap(Ehi, Ehi2);
-      } else {
-        rc = zc_round_device(ctx, st, A, Bv, E, half, ctx->readback_dev, i + 1 < n ? E2 : nullptr, seq);
       }
       if (rc) return rc;
       if (!split) std::swap(E, E2);

@@ -43,12 +43,13 @@ int blocks_log2(uint64_t n) {
 
 }  // namespace
 
-// Device bytes a proof of 2^log2 compressions keeps in its context: about 18
-// packed bit-vectors of 2^(m-3) bytes (witness, codeword, zerocheck and
-// ring-switch tables, Ligerito folds) plus fixed tables.
+// Device bytes a proof of 2^log2 compressions keeps in its context: at most
+// 16 packed bit-vectors of 2^(m-3) bytes (witness 3, codeword 2, zerocheck
+// round-1 sums and tables 4, ring-switch and Ligerito tables 3.5, with room
+// to spare) plus fixed tables.
 uint64_t prove_footprint_bytes(int n_log) {
   const int m = r1cs::K_LOG + n_log;
-  return 18ull * (1ull << (m - 3)) + (256ull << 20);
+  return 16ull * (1ull << (m - 3)) + (256ull << 20);
 }
 
 struct Request {

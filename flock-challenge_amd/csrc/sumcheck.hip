@@ -76,59 +76,11 @@ int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void
   return FLOCK_OK;
 }
 
-// ---------------------------------------------------------------- zerocheck rounds
-// (G(1), G(inf)) of round_pair_naive (zerocheck/multilinear.rs:639-680):
-// sum_x' E[x'] A1 B1 and sum_x' E[x'] (A0 + A1)(B0 + B1).
-// Also writes the next round's eq table: eq(r[k+1..], y) = E[2y] + E[2y+1]
-// (the two eq factors of the dropped variable sum to one), when En != null.
-__global__ __launch_bounds__(MUL_THREADS) void zc_round_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
-                                                               const f128 *__restrict__ E, uint64_t half,
-                                                               f128 *__restrict__ partial, f128 *__restrict__ En) {
-  __shared__ f128 mt[MUL_WAVES][1024];
-  __shared__ f128 red[MUL_THREADS];
-  const uint32_t lane = threadIdx.x & 63;
-  f128 *tab = &mt[0][0];
-  const CombLane CL(threadIdx.x);
-  f128 g1 = f128_zero(), gi = f128_zero();
-  for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
-    const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1], e = E[x];
-    if (En) {
-      // x and x ^ 1 sit in adjacent lanes of the same iteration (the stride is even)
-      f128 o;
-      o.x = __shfl_xor(e.x, 1);
-      o.y = __shfl_xor(e.y, 1);
-      o.z = __shfl_xor(e.z, 1);
-      o.w = __shfl_xor(e.w, 1);
-      if (!(x & 1)) En[x >> 1] = f128_xor(e, o);
-    }
-    const f128 p1 = f128_mul(a1, b1, tab, CL);
-    const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
-    f128_mul_build(e, tab, CL);  // one table of E for both weighted sums
-    f128_xor_into(g1, f128_mul_comb(p1, tab, CL));
-    f128_xor_into(gi, f128_mul_comb(pi, tab, CL));
-  }
-  wg_xor_reduce(g1, red, &partial[2 * blockIdx.x]);
-  wg_xor_reduce(gi, red, &partial[2 * blockIdx.x + 1]);
-}
-
 static unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
   uint64_t g = (work + threads - 1) / threads;
   if (g > cap) g = cap;
   if (g == 0) g = 1;
   return (unsigned)g;
-}
-
-int zc_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *E, uint64_t half,
-                    void *out2, void *E_next, uint32_t seq) {
-  ScopedPhase ph(ctx, "zc_round", st);
-  unsigned g = grid_for(half, MUL_THREADS, 4096);
-  f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
-  if (!part) return FLOCK_E_OOM;
-  if (E_next && half < 2) E_next = nullptr;
-  hipLaunchKernelGGL(zc_round_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
-                     (const f128 *)E, half, part, (f128 *)E_next);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
 }
 
 // ---------------------------------------------------------------- zerocheck rounds, split eq
@@ -430,8 +382,11 @@ __device__ __forceinline__ void small_publish(f128 s0, f128 s1, f128 *red, f128 
 }
 
 // Zerocheck round over `half` pairs after folding A, B (4 half entries each)
-// by rho into A2, B2 (2 half): (G(1), G(inf)) as zc_round_kernel, and
-// E_next[y] = E[2y] + E[2y+1] unless E_next is null.
+// by rho into A2, B2 (2 half): (G(1), G(inf)) of round_pair_naive
+// (zerocheck/multilinear.rs:639-680), sum_x E[x] A1 B1 and
+// sum_x E[x] (A0 + A1)(B0 + B1) over the folded pairs, and the next round's
+// eq table E_next[y] = E[2y] + E[2y+1] (the two eq factors of the dropped
+// variable sum to one) unless E_next is null.
 __global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
     const f128 *__restrict__ A, const f128 *__restrict__ B, f128 rho, const f128 *__restrict__ E, uint32_t half,
     f128 *__restrict__ A2, f128 *__restrict__ B2, f128 *__restrict__ E_next, f128 *__restrict__ out, uint32_t *flag,
