@@ -337,13 +337,13 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
   return FLOCK_OK;
 }
 
-// ---------------------------------------------------------------- small rounds, fused
-// The last rounds of the zerocheck and Ligerito sumchecks have at most
-// SMALL_PAIRS pairs: one workgroup then folds the previous round's tables by
-// its challenge (writing them, fold_pair_kernel), computes the new round's
-// message over the folded values, reduces it and publishes it to the host
-// (xor_rows_kernel): one launch per round instead of three.  Thread x takes
-// pair x, i.e. folded positions 2x, 2x + 1 from positions 4x .. 4x + 3.
+// ---------------------------------------------------------------- rounds fused with their fold
+// A round of the zerocheck or Ligerito sumcheck can fold the previous
+// round's tables by its challenge itself (writing them, as fold_pair_kernel
+// does) and compute its message over the folded values.  Up to SMALL_PAIRS
+// pairs one workgroup also reduces the message and publishes it to the host
+// (as xor_rows_kernel does): one launch per round instead of three.  Pair x
+// is folded positions 2x, 2x + 1, from positions 4x .. 4x + 3.
 constexpr int SMALL_THREADS = 256;
 
 // v[4x] + r (v[4x] + v[4x+1]) and v[4x+2] + r (v[4x+2] + v[4x+3]), with the
