@@ -345,6 +345,7 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
 // (as xor_rows_kernel does): one launch per round instead of three.  Pair x
 // is folded positions 2x, 2x + 1, from positions 4x .. 4x + 3.
 constexpr int SMALL_THREADS = 256;
+static_assert(SMALL_PAIRS <= (uint64_t)SMALL_THREADS, "one pair per thread in the single-workgroup rounds");
 
 // v[4x] + r (v[4x] + v[4x+1]) and v[4x+2] + r (v[4x+2] + v[4x+3]), with the
 // lane's comb table holding r.
