@@ -839,8 +839,10 @@ int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, 
   int done = 0;
   const int total = k - l0;
   // Layers per launch: each thread runs F layers of serial products, so keep
-  // at least 2^18 threads in flight (fewer layers per launch when small).
-  const int f_par = std::max(1, std::min(4, k + log_nl - 18));
+  // at least 2^18 threads in flight (fewer layers per launch when small),
+  // but at least two layers: below that size a launch costs more than a
+  // thread's two layers of products (independent within a layer).
+  const int f_par = std::max(2, std::min(4, k + log_nl - 18));
   while (done < total) {
     const int F = std::min(f_par, total - done);
     // forward passes run l0, l0+4, ...; transposed ones run the same groups from the top
