@@ -342,10 +342,11 @@ struct Prover {
     int rc = h2d(didx, all.data(), all.size() * 8);
     if (!rc) rc = gather_device(ctx, st, w.cw, didx, (uint32_t)idx.size(), (uint32_t)(16 * w.lanes), drows);
     if (!rc) rc = gather_device(ctx, st, w.tree, didx + idx.size(), (uint32_t)nodes.size(), 32, dnodes);
-    if (!rc) rc = d2h(o.rows.data(), drows, o.rows.size() * 16);
     o.hashes.resize(nodes.size() * 32);
-    if (!rc && !nodes.empty()) rc = d2h(o.hashes.data(), dnodes, o.hashes.size());
-    return rc;
+    if (rc) return rc;
+    // rows and sibling hashes come back with one wait
+    return nodes.empty() ? d2h(o.rows.data(), drows, o.rows.size() * 16)
+                         : d2h2(o.rows.data(), drows, o.rows.size() * 16, o.hashes.data(), dnodes, o.hashes.size());
   }
 
   // induce_sumcheck_poly (pcs/ligerito.rs:1989-2160) as a transposed NTT:
@@ -355,15 +356,17 @@ struct Prover {
     ScopedPhase ph(ctx, "induce_basis", st);
     std::vector<F128> aw = eq_table(alpha.data(), (int)alpha.size());
     aw.resize(q.size());
+    // positions (padded to an even count) then values, uploaded in one copy
+    const size_t n_idx = (q.size() + 1) & ~(size_t)1;
+    std::vector<uint64_t> up(n_idx + 2 * q.size());
+    std::copy(q.begin(), q.end(), up.begin());
+    memcpy(up.data() + n_idx, aw.data(), q.size() * 16);
     void *sp = dev("induce.sparse", w.block_len * 16);
-    uint64_t *didx = (uint64_t *)dev("induce.idx", q.size() * 8 + 8);
-    void *dval = dev("induce.val", q.size() * 16 + 16);
-    if (!sp || !didx || !dval) return FLOCK_E_OOM;
+    uint64_t *dup = (uint64_t *)dev("induce.up", up.size() * 8 + 16);
+    if (!sp || !dup) return FLOCK_E_OOM;
     FLOCK_HIP_CHECK(ctx, hipMemsetAsync(sp, 0, w.block_len * 16, st));
-    std::vector<uint64_t> idx(q.begin(), q.end());
-    int rc = h2d(didx, idx.data(), idx.size() * 8);
-    if (!rc) rc = h2d(dval, aw.data(), aw.size() * 16);
-    if (!rc) rc = scatter_device(ctx, st, sp, didx, dval, (uint32_t)q.size());
+    int rc = h2d(dup, up.data(), up.size() * 8);
+    if (!rc) rc = scatter_device(ctx, st, sp, dup, dup + n_idx, (uint32_t)q.size());
     const F128 *tw = ctx->device_twiddles(w.dim);
     if (!tw) return FLOCK_E_DEVICE;
     if (!rc) rc = ntt_narrow_device(ctx, st, sp, tw, w.dim, w.rate, 0, 1);
