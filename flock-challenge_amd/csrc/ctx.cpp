@@ -158,6 +158,7 @@ Ctx::~Ctx() {
   for (hipEvent_t e : event_pool) hipEventDestroy(e);
   if (pinned) hipHostFree(pinned);
   if (landing) hipHostFree(landing);
+  if (open_area) hipHostFree(open_area);
   if (readback) hipHostFree(readback);
   for (auto &kv : twiddles_dev) hipFree(kv.second);
   for (auto &kv : scratch)

@@ -62,6 +62,10 @@ struct Ctx {
   size_t pinned_bytes = 0, pinned_off = 0;
   uint8_t *landing = nullptr;
   size_t landing_bytes = 0;
+  // Pinned host area for Merkle openings copied back without waiting (read
+  // only when the proof is serialized); open_area_off is reset per proof.
+  uint8_t *open_area = nullptr;
+  size_t open_area_bytes = 0, open_area_off = 0;
   // Coherent pinned host memory mapped into the device: a kernel writes a
   // small result (a sumcheck message) straight into it, and the host reads it
   // after synchronizing the stream, with no copy kernel in between.

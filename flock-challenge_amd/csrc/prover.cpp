@@ -115,7 +115,17 @@ struct Opening {
   std::vector<F128> rows;  // n x lanes
   size_t lanes = 0, n = 0;
   std::vector<uint8_t> hashes;  // 32 x k
+  // rows then hashes in the context's pinned opening area, copied there
+  // without waiting; valid once the stream has drained past those copies
+  const uint8_t *pending = nullptr;
 };
+
+void finish_opening(Opening &o) {
+  if (!o.pending) return;
+  memcpy(o.rows.data(), o.pending, o.rows.size() * 16);
+  memcpy(o.hashes.data(), o.pending + o.rows.size() * 16, o.hashes.size());
+  o.pending = nullptr;
+}
 
 void put_opening(Bytes &b, const Opening &o) {
   b.u64(o.n);
@@ -145,6 +155,8 @@ struct Prover {
   std::vector<const char *> scratch_names;
 
   bool pow_armed = false;  // the device's best nonce holds all-ones
+  // openings copied back without a wait since the stream last drained
+  int openings_in_flight = 0;
 
   void *dev(const char *name, size_t bytes) { return ctx->get_scratch(name, bytes); }
 
@@ -171,6 +183,7 @@ struct Prover {
       }
       __builtin_ia32_pause();
     }
+    openings_in_flight = 0;  // everything queued before seq has finished
     return FLOCK_OK;
   }
 
@@ -186,6 +199,7 @@ struct Prover {
     }
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, st));
     FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    openings_in_flight = 0;
     return FLOCK_OK;
   }
   // Two small device ranges (together at most the landing area) with one
@@ -326,7 +340,10 @@ struct Prover {
     return d2h(w.root, (const uint8_t *)w.tree + 32 * (2 * w.block_len - 2), 32);
   }
 
-  int open(const Witness &w, const std::vector<size_t> &q, Opening &o) {
+  // With defer, the rows and hashes go to the pinned opening area without a
+  // wait (finish_opening reads them after the stream has drained), unless
+  // the area is full.
+  int open(const Witness &w, const std::vector<size_t> &q, Opening &o, bool defer) {
     ScopedPhase ph(ctx, "open_rows", st);
     o.n = q.size();
     o.lanes = w.lanes;
@@ -344,6 +361,16 @@ struct Prover {
     if (!rc) rc = gather_device(ctx, st, w.tree, didx + idx.size(), (uint32_t)nodes.size(), 32, dnodes);
     o.hashes.resize(nodes.size() * 32);
     if (rc) return rc;
+    const size_t rb = o.rows.size() * 16, hb = o.hashes.size();
+    if (defer && ctx->open_area && ctx->open_area_off + rb + hb <= ctx->open_area_bytes) {
+      uint8_t *dst = ctx->open_area + ctx->open_area_off;
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(dst, drows, rb, hipMemcpyDeviceToHost, st));
+      if (hb) FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(dst + rb, dnodes, hb, hipMemcpyDeviceToHost, st));
+      ctx->open_area_off += (rb + hb + 255) & ~(size_t)255;
+      o.pending = dst;
+      ++openings_in_flight;
+      return FLOCK_OK;
+    }
     // rows and sibling hashes come back with one wait
     return nodes.empty() ? d2h(o.rows.data(), drows, o.rows.size() * 16)
                          : d2h2(o.rows.data(), drows, o.rows.size() * 16, o.hashes.data(), dnodes, o.hashes.size());
@@ -432,6 +459,14 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       ctx->landing_bytes = 1 << 20;
     }
   }
+  if (!ctx->open_area) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, 4 << 20, hipHostMallocDefault) == hipSuccess) {
+      ctx->open_area = (uint8_t *)p;
+      ctx->open_area_bytes = 4 << 20;
+    }
+  }
+  ctx->open_area_off = 0;
   if (!ctx->readback) {
     void *p = nullptr, *dp = nullptr;
     FLOCK_HIP_CHECK(ctx, hipHostMalloc(&p, Ctx::READBACK_BYTES, hipHostMallocMapped | hipHostMallocCoherent));
@@ -1040,7 +1075,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     nonces.push_back(nc);
     std::vector<size_t> q = sample_queries(t, prev.block_len, (size_t)cfg->levels[0].queries);
     std::vector<F128> al = t.sample_vec((size_t)ceil_log2(q.size()));
-    if ((rc = P.open(prev, q, open0))) return rc;
+    if ((rc = P.open(prev, q, open0, true))) return rc;
     if ((rc = P.induce(prev, q, al, n1, basis))) return rc;
     if ((rc = P.msg(f, basis, len, 0, o3))) return rc;
     push(o3[0], o3[1]);
@@ -1068,7 +1103,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
         if ((rc = P.grind(t, (uint32_t)cfg->levels[i + 1].grinding_bits, &nc))) return rc;
         nonces.push_back(nc);
         q = sample_queries(t, prev.block_len, (size_t)cfg->levels[i + 1].queries);
-        if ((rc = P.open(prev, q, open_last))) return rc;
+        if ((rc = P.open(prev, q, open_last, false))) return rc;
         break;
       }
       int nn = 0;
@@ -1086,7 +1121,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       q = sample_queries(t, prev.block_len, (size_t)cfg->levels[i + 1].queries);
       al = t.sample_vec((size_t)ceil_log2(q.size()));
       Opening op;
-      if ((rc = P.open(prev, q, op))) return rc;
+      if ((rc = P.open(prev, q, op, true))) return rc;
       rec_open.push_back(op);
       if ((rc = P.induce(prev, q, al, nn, basis))) return rc;
       if ((rc = P.msg(f, basis, len, 0, o3))) return rc;
@@ -1096,6 +1131,12 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       prev = w2;
     }
   }
+  // the deferred openings' bytes are in host memory once the stream has
+  // drained past their copies (the last opening's wait normally did that)
+  if (P.openings_in_flight && (rc = P.wait_signal())) return rc;
+  finish_opening(open0);
+  for (auto &o : rec_open) finish_opening(o);
+  finish_opening(open_last);
   // LigeritoProof (pcs/ligerito.rs:1504-1545)
   {
     uint8_t root0[32];
