@@ -606,14 +606,14 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       const ZcTables &T = zc_tables();
       for (int j = 0; j < 64; ++j) {
         const int h = j >> 5, bit = j & 31;
-        F128 acc;
+        F256 acc;  // unreduced (reduction is linear)
         for (int xm = 0; xm < 16; ++xm)
           for (int t = 0; t < 8; ++t) {
             F128 w;
             memcpy(&w, T.pt[xm][t], 16);
-            acc += w * qab[(4 * xm + 2 * h + (t >> 2)) * 128 + 32 * (t & 3) + bit];
+            acc ^= mul_unreduced(w, qab[(4 * xm + 2 * h + (t >> 2)) * 128 + 32 * (t & 3) + bit]);
           }
-        r1ab[j] = acc;
+        r1ab[j] = reduce(acc);
       }
     }
     std::vector<F128> q(2048);
@@ -623,18 +623,18 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     F128 cs[64];
     const F128 w0 = ONE + r[6], w1 = r[6];
     for (int sb = 0; sb < 64; ++sb) {
-      F128 s0, s1;
+      F256 s0, s1;
       for (int il = 0; il < 16; ++il) {
-        s0 += lo[il] * q[il * 128 + sb];
-        s1 += lo[il] * q[il * 128 + 64 + sb];
+        s0 ^= mul_unreduced(lo[il], q[il * 128 + sb]);
+        s1 ^= mul_unreduced(lo[il], q[il * 128 + 64 + sb]);
       }
-      cs[sb] = w0 * s0 + w1 * s1;
+      cs[sb] = w0 * reduce(s0) + w1 * reduce(s1);
     }
     for (int j = 0; j < 64; ++j) {
-      F128 acc;
+      F256 acc;
       for (int sb = 0; sb < 64; ++sb)
-        if (EXT_M[j][sb]) acc += phi8(EXT_M[j][sb]) * cs[sb];
-      r1c[j] = acc;
+        if (EXT_M[j][sb]) acc ^= mul_unreduced(phi8(EXT_M[j][sb]), cs[sb]);
+      r1c[j] = reduce(acc);
     }
   }
   t.observe_slice(r1ab, 64);
@@ -924,9 +924,9 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     for (int k = 0; k < 2; ++k) {
       std::vector<F128> lo = eq_table(&xf[k][1], 4);
       for (int rb = 0; rb < 128; ++rb) {
-        F128 acc;
-        for (int il = 0; il < 16; ++il) acc += lo[il] * q[k * 2048 + il * 128 + rb];
-        shv[k][rb] = acc;
+        F256 acc;
+        for (int il = 0; il < 16; ++il) acc ^= mul_unreduced(lo[il], q[k * 2048 + il * 128 + rb]);
+        shv[k][rb] = reduce(acc);
       }
     }
   }
