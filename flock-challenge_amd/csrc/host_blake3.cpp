@@ -5,13 +5,22 @@ namespace hb3 {
 
 static inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
 
-void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, uint32_t len, uint32_t flags,
+// Message word order of each round: the BLAKE3 permutation
+// (2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8) applied r times, so
+// the rounds index the block directly instead of permuting it.
+static const uint8_t SCHEDULE[7][16] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15},
+    {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8},
+    {3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1},
+    {10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6},
+    {12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4},
+    {9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7},
+    {11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13}};
+
+void compress(const uint32_t cv[8], const uint32_t m[16], uint64_t counter, uint32_t len, uint32_t flags,
               uint32_t out[16]) {
-  static const uint8_t P[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
   uint32_t v[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7], IV[0], IV[1], IV[2], IV[3],
                     (uint32_t)counter, (uint32_t)(counter >> 32), len, flags};
-  uint32_t m[16], t[16];
-  memcpy(m, m_in, 64);
   auto g = [&](int a, int b, int c, int d, uint32_t x, uint32_t y) {
     v[a] = v[a] + v[b] + x;
     v[d] = rotr(v[d] ^ v[a], 16);
@@ -23,16 +32,15 @@ void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, u
     v[b] = rotr(v[b] ^ v[c], 7);
   };
   for (int r = 0; r < 7; ++r) {
-    g(0, 4, 8, 12, m[0], m[1]);
-    g(1, 5, 9, 13, m[2], m[3]);
-    g(2, 6, 10, 14, m[4], m[5]);
-    g(3, 7, 11, 15, m[6], m[7]);
-    g(0, 5, 10, 15, m[8], m[9]);
-    g(1, 6, 11, 12, m[10], m[11]);
-    g(2, 7, 8, 13, m[12], m[13]);
-    g(3, 4, 9, 14, m[14], m[15]);
-    for (int i = 0; i < 16; ++i) t[i] = m[P[i]];
-    memcpy(m, t, 64);
+    const uint8_t *s = SCHEDULE[r];
+    g(0, 4, 8, 12, m[s[0]], m[s[1]]);
+    g(1, 5, 9, 13, m[s[2]], m[s[3]]);
+    g(2, 6, 10, 14, m[s[4]], m[s[5]]);
+    g(3, 7, 11, 15, m[s[6]], m[s[7]]);
+    g(0, 5, 10, 15, m[s[8]], m[s[9]]);
+    g(1, 6, 11, 12, m[s[10]], m[s[11]]);
+    g(2, 7, 8, 13, m[s[12]], m[s[13]]);
+    g(3, 4, 9, 14, m[s[14]], m[s[15]]);
   }
   for (int i = 0; i < 8; ++i) {
     out[i] = v[i] ^ v[i + 8];

@@ -1,8 +1,7 @@
 """The library's host BLAKE3 (flock-challenge_amd/csrc/host_blake3.cpp: the
 compression function and the incremental hasher) against the oracle's on
-random inputs: a
-small C++ driver (tests/native/blake3_check.cpp) built with gcc/g++ and run
-on the CPU."""
+random inputs: a small C++ driver (tests/native/blake3_check.cpp) built
+with gcc/g++ and run on the CPU."""
 import os
 import shutil
 import subprocess
