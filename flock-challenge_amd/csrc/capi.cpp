@@ -20,6 +20,18 @@ struct flock_ctx {
 
 using flock::Ctx;
 
+// A context whose wait for the device timed out may still have work running
+// on its stream: every later call on it fails.
+static int usable(const flock_ctx *ctx) {
+  if (!ctx) return FLOCK_E_ARG;
+  return ctx->c.timed_out ? FLOCK_E_DEVICE : FLOCK_OK;
+}
+
+// A failure that came with (or after) a timed-out wait is a device error,
+// whatever the failing step reported (a scratch allocation that had to wait
+// first reports out-of-memory).
+static int device_rc(const Ctx *c, int rc) { return rc && c->timed_out ? FLOCK_E_DEVICE : rc; }
+
 extern "C" int flock_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -39,41 +51,46 @@ extern "C" int flock_device_alloc(flock_ctx *ctx, size_t bytes, void **d_ptr) {
 }
 
 extern "C" int flock_device_free(flock_ctx *ctx, void *d_ptr) {
+  if (int u = usable(ctx)) return u;
   if (!ctx) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);
-  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  if (int rc = ctx->c.wait_stream(ctx->c.stream)) return rc;
   FLOCK_HIP_CHECK(&ctx->c, hipFree(d_ptr));
   return FLOCK_OK;
 }
 
 extern "C" int flock_memcpy_h2d(flock_ctx *ctx, void *d_dst, const void *src, size_t bytes) {
+  if (int u = usable(ctx)) return u;
   if (!ctx || (!d_dst && bytes) || (!src && bytes)) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);
   FLOCK_HIP_CHECK(&ctx->c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
-  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  if (int rc = ctx->c.wait_stream(ctx->c.stream)) return rc;
   return FLOCK_OK;
 }
 
 extern "C" int flock_memcpy_d2h(flock_ctx *ctx, void *dst, const void *d_src, size_t bytes) {
+  if (int u = usable(ctx)) return u;
   if (!ctx || (!d_src && bytes) || (!dst && bytes)) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);
   FLOCK_HIP_CHECK(&ctx->c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
-  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  if (int rc = ctx->c.wait_stream(ctx->c.stream)) return rc;
   return FLOCK_OK;
 }
 
 extern "C" int flock_memcpy_d2d(flock_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
+  if (int u = usable(ctx)) return u;
   if (!ctx || (!d_src && bytes) || (!d_dst && bytes)) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);
   FLOCK_HIP_CHECK(&ctx->c, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
-  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  if (int rc = ctx->c.wait_stream(ctx->c.stream)) return rc;
   return FLOCK_OK;
 }
 
 extern "C" int flock_synchronize(flock_ctx *ctx) {
+  if (int u = usable(ctx)) return u;
   if (!ctx) return FLOCK_E_ARG;
   hipSetDevice(ctx->c.device);
-  FLOCK_HIP_CHECK(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+  if (int rc = ctx->c.wait_stream(ctx->c.stream)) return rc;
   return FLOCK_OK;
 }
 
@@ -101,26 +118,45 @@ extern "C" int flock_ctx_create(int device, flock_ctx **out) {
 
 extern "C" void flock_ctx_destroy(flock_ctx *ctx) { delete ctx; }
 
+extern "C" int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds) {
+  if (!ctx || !(seconds == seconds)) return FLOCK_E_ARG;
+  ctx->c.wait_timeout_s = seconds;
+  return FLOCK_OK;
+}
+
 extern "C" const char *flock_last_error(const flock_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
 
 static hipStream_t pick_stream(flock_ctx *ctx, void *stream) {
   return stream ? (hipStream_t)stream : ctx->c.stream;
 }
 
+extern "C" int flock_pcs_params_sizes(const flock_pcs_params *params, uint64_t *msg_f128, uint64_t *codeword_f128,
+                                      uint64_t *tree_nodes) {
+  if (!params) return FLOCK_E_ARG;
+  if (flock::commit_params_unsupported(params)) return FLOCK_E_SHAPE;
+  const int k = (int)params->m - 7 - (int)params->log_batch_size + (int)params->log_inv_rate;
+  if (msg_f128) *msg_f128 = 1ull << (params->m - 7);
+  if (codeword_f128) *codeword_f128 = 1ull << (k + (int)params->log_batch_size);
+  if (tree_nodes) *tree_nodes = (2ull << k) - 1;
+  return FLOCK_OK;
+}
+
 extern "C" int flock_commit_device(flock_ctx *ctx, const flock_pcs_params *params, const void *d_z, void *d_codeword,
                                    void *d_tree, uint8_t root[32], void *stream) {
+  if (int u = usable(ctx)) return u;
   if (!ctx || !params || !d_z || !d_codeword || !d_tree || !root) return FLOCK_E_ARG;
   if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
-  return flock::commit_device(&ctx->c, params, d_z, d_codeword, d_tree, root, pick_stream(ctx, stream));
+  return device_rc(&ctx->c, flock::commit_device(&ctx->c, params, d_z, d_codeword, d_tree, root, pick_stream(ctx, stream)));
 }
 
 extern "C" int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, const void *z_packed, size_t z_bytes,
                             uint8_t root[32]) {
+  if (int u = usable(ctx)) return u;
   if (!ctx || !params || !z_packed || !root) return FLOCK_E_ARG;
   Ctx *c = &ctx->c;
   if (hipSetDevice(c->device) != hipSuccess) return FLOCK_E_DEVICE;
-  if (params->m < 7 + params->log_batch_size || params->m > 40) {
-    c->last_error = "commit: bad m";
+  if (const char *why = flock::commit_params_unsupported(params)) {
+    c->last_error = std::string("commit: ") + why;
     return FLOCK_E_SHAPE;
   }
   const size_t msg_bytes = (size_t)16 << (params->m - 7);
@@ -134,9 +170,9 @@ extern "C" int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, cons
   void *dz = c->get_scratch("commit.z", msg_bytes);
   void *dcw = c->get_scratch("commit.codeword", cw_bytes);
   void *dtree = c->get_scratch("commit.tree", tree_bytes);
-  if (!dz || !dcw || !dtree) return FLOCK_E_OOM;
+  if (!dz || !dcw || !dtree) return device_rc(c, FLOCK_E_OOM);
   FLOCK_HIP_CHECK(c, hipMemcpyAsync(dz, z_packed, msg_bytes, hipMemcpyHostToDevice, c->stream));
-  return flock::commit_device(c, params, dz, dcw, dtree, root, c->stream);
+  return device_rc(c, flock::commit_device(c, params, dz, dcw, dtree, root, c->stream));
 }
 
 // ------------------------------------------------------------------ prover
@@ -202,6 +238,7 @@ static int finish_proof(Ctx *c, std::vector<uint8_t> &bytes, uint8_t *out, size_
 
 static int check_prove_args(flock_setup *setup, uint64_t n_blocks, const uint8_t *domain, size_t domain_len) {
   if (!setup || !setup->ctx || (!domain && domain_len)) return FLOCK_E_ARG;
+  if (int u = usable(setup->ctx)) return u;
   if (n_blocks != setup->n_blocks) {
     setup->ctx->c.last_error = "prove: n_blocks " + std::to_string(n_blocks) + " != setup's " +
                                std::to_string(setup->n_blocks);
@@ -220,7 +257,7 @@ extern "C" int flock_prove_blake3_device(flock_setup *setup, const flock_compres
   Ctx *c = &setup->ctx->c;
   std::vector<uint8_t> bytes;
   rc = flock::prove_blake3_device(c, pick_stream(setup->ctx, stream), d_blocks, n_blocks, domain, domain_len, bytes);
-  if (rc) return rc;
+  if (rc) return device_rc(c, rc);
   return finish_proof(c, bytes, out, cap, out_len);
 }
 
@@ -236,12 +273,13 @@ extern "C" int flock_prove_blake3(flock_setup *setup, const flock_compression *b
   FLOCK_HIP_CHECK(c, hipMemcpyAsync(d, blocks, n_blocks * sizeof(flock_compression), hipMemcpyHostToDevice, c->stream));
   std::vector<uint8_t> bytes;
   rc = flock::prove_blake3_device(c, c->stream, (const flock_compression *)d, n_blocks, domain, domain_len, bytes);
-  if (rc) return rc;
+  if (rc) return device_rc(c, rc);
   return finish_proof(c, bytes, out, cap, out_len);
 }
 
 extern "C" int flock_generate_compressions_device(flock_ctx *ctx, uint32_t log2, uint64_t seed,
                                                   flock_compression *d_out) {
+  if (int u = usable(ctx)) return u;
   if (!ctx || !d_out || log2 > 30) return FLOCK_E_ARG;
   if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
   return flock::generate_compressions_device(&ctx->c, ctx->c.stream, log2, seed, d_out);
@@ -258,7 +296,7 @@ extern "C" int flock_prove_blake3_seeded(flock_setup *setup, uint32_t log2, uint
   if ((rc = flock::generate_compressions_device(c, c->stream, log2, seed, (flock_compression *)d))) return rc;
   std::vector<uint8_t> bytes;
   rc = flock::prove_blake3_device(c, c->stream, (const flock_compression *)d, 1ull << log2, domain, domain_len, bytes);
-  if (rc) return rc;
+  if (rc) return device_rc(c, rc);
   return finish_proof(c, bytes, out, cap, out_len);
 }
 
@@ -284,7 +322,7 @@ extern "C" int flock_commitment_root(flock_setup *setup, const flock_compression
   void *d = c->get_scratch("prove.blocks", n_blocks * sizeof(flock_compression));
   if (!d) return FLOCK_E_OOM;
   FLOCK_HIP_CHECK(c, hipMemcpyAsync(d, blocks, n_blocks * sizeof(flock_compression), hipMemcpyHostToDevice, c->stream));
-  return flock::commitment_root_device(c, c->stream, (const flock_compression *)d, n_blocks, root);
+  return device_rc(c, flock::commitment_root_device(c, c->stream, (const flock_compression *)d, n_blocks, root));
 }
 
 extern "C" int flock_commitment_root_seeded(flock_setup *setup, uint32_t log2, uint64_t seed, uint8_t root[32]) {
@@ -295,7 +333,7 @@ extern "C" int flock_commitment_root_seeded(flock_setup *setup, uint32_t log2, u
   void *d = c->get_scratch("prove.blocks", (1ull << log2) * sizeof(flock_compression));
   if (!d) return FLOCK_E_OOM;
   if ((rc = flock::generate_compressions_device(c, c->stream, log2, seed, (flock_compression *)d))) return rc;
-  return flock::commitment_root_device(c, c->stream, (const flock_compression *)d, 1ull << log2, root);
+  return device_rc(c, flock::commitment_root_device(c, c->stream, (const flock_compression *)d, 1ull << log2, root));
 }
 
 extern "C" void flock_blake3(const void *data, size_t len, uint8_t out[32]) {

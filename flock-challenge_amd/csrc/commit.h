@@ -21,6 +21,9 @@ int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t le
 // Fill out with out_len/msg_len copies of msg (F128 counts, powers of two).
 int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len_f128, void *out,
                      uint64_t out_len_f128);
+// Why the commit kernels cannot take these parameters, or nullptr when they
+// can (64 lanes, log_inv_rate 1..4, m 14..40, BLAKE3 Merkle hash).
+const char *commit_params_unsupported(const flock_pcs_params *p);
 // Full commit; copies the root to host memory `root` and synchronizes `st`,
 // or, when root_ready is given, records it after the copy instead (then
 // `root` must be pinned memory, valid once the event has completed).

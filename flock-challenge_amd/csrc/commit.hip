@@ -379,14 +379,26 @@ int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t le
   return FLOCK_OK;
 }
 
+const char *commit_params_unsupported(const flock_pcs_params *p) {
+  // Shapes the kernels support, checked before anything is allocated or
+  // launched: 64 interleaved lanes (one wavefront per group), rate 1/2 ..
+  // 1/16, at least one NTT layer above the replication, and sizes whose
+  // codeword byte count fits 64 bits with room to spare.
+  if (p->merkle_hash != 1) return "only the BLAKE3 Merkle hash is implemented";
+  if (p->log_batch_size != 6) return "only log_batch_size = 6 (64 interleaved lanes) is implemented";
+  if (p->log_inv_rate < 1 || p->log_inv_rate > 4) return "log_inv_rate must be in 1..4";
+  if (p->m < 14 || p->m > 40) return "m must be in 14..40";
+  return nullptr;
+}
+
 int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
                   uint8_t root[32], hipStream_t st, hipEvent_t root_ready) {
   if (p->merkle_hash != 1) {
     ctx->last_error = "only the BLAKE3 Merkle hash is implemented";
     return FLOCK_E_ARG;
   }
-  if (p->m < 7 + p->log_batch_size || p->log_inv_rate < 1) {
-    ctx->last_error = "commit: m too small or log_inv_rate < 1";
+  if (const char *why = commit_params_unsupported(p)) {
+    ctx->last_error = std::string("commit: ") + why;
     return FLOCK_E_SHAPE;
   }
   const int log_msg = (int)p->m - 7;
@@ -414,8 +426,8 @@ int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_
   ph.end();
   if (root_ready) {
     FLOCK_HIP_CHECK(ctx, hipEventRecord(root_ready, st));
-  } else {
-    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  } else if (int e = ctx->wait_stream(st)) {
+    return e;
   }
   return FLOCK_OK;
 }

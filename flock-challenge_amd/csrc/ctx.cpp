@@ -66,11 +66,72 @@ const F128 *Ctx::device_twiddles(int dim) {
   return d;
 }
 
+int Ctx::wait_failed(WaitEnd how, hipError_t err, const char *what) {
+  if (how == WaitEnd::TimedOut) {
+    timed_out = true;
+    last_error = std::string(what) + ": the device did not finish within " + std::to_string(wait_timeout_s) +
+                 " s; the context is unusable";
+  } else if (err != hipSuccess) {
+    last_error = std::string(what) + ": " + hipGetErrorString(err);
+  } else {
+    last_error = std::string(what) + ": the stream finished without publishing its signal";
+  }
+  return FLOCK_E_DEVICE;
+}
+
+int Ctx::wait_flag_seq(const uint32_t *flag, uint32_t seq, hipStream_t st) {
+  hipError_t err = hipSuccess;
+  // hipStreamQuery now and then: a failed kernel, or a stream that drained
+  // without publishing, ends the wait
+  const WaitEnd how = wait_flag(
+      flag, seq,
+      [&] {
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipErrorNotReady) return 0;
+        if (q == hipSuccess) return 1;
+        err = q;
+        return -1;
+      },
+      wait_timeout_s);
+  return how == WaitEnd::Done ? FLOCK_OK : wait_failed(how, err, "readback");
+}
+
+int Ctx::wait_stream(hipStream_t st) {
+  hipError_t err = hipSuccess;
+  bool fin = false;
+  const WaitEnd how = bounded_wait([&] { return fin; },
+                                   [&] {
+                                     const hipError_t q = hipStreamQuery(st);
+                                     if (q == hipErrorNotReady) return 0;
+                                     if (q == hipSuccess) return (int)(fin = true);
+                                     err = q;
+                                     return -1;
+                                   },
+                                   wait_timeout_s, 16);
+  return how == WaitEnd::Done ? FLOCK_OK : wait_failed(how, err, "stream wait");
+}
+
+int Ctx::wait_event(hipEvent_t e) {
+  hipError_t err = hipSuccess;
+  bool fin = false;
+  const WaitEnd how = bounded_wait([&] { return fin; },
+                                   [&] {
+                                     const hipError_t q = hipEventQuery(e);
+                                     if (q == hipErrorNotReady) return 0;
+                                     if (q == hipSuccess) return (int)(fin = true);
+                                     err = q;
+                                     return -1;
+                                   },
+                                   wait_timeout_s, 16);
+  return how == WaitEnd::Done ? FLOCK_OK : wait_failed(how, err, "event wait");
+}
+
 void *Ctx::get_scratch(const std::string &name, size_t bytes) {
   auto &b = scratch[name];
   if (b.bytes >= bytes && b.ptr) return b.ptr;
   if (b.ptr) {
-    hipStreamSynchronize(stream);  // this context's queued kernels may still use the old buffer
+    // this context's queued kernels may still use the old buffer
+    if (wait_stream(stream)) return nullptr;
     hipFree(b.ptr);
   }
   b.ptr = nullptr;
@@ -82,6 +143,17 @@ void *Ctx::get_scratch(const std::string &name, size_t bytes) {
   }
   b.bytes = bytes;
   return b.ptr;
+}
+
+int Ctx::release_scratch() {
+  if (int rc = wait_stream(stream)) return rc;
+  for (auto &kv : scratch)
+    if (kv.second.ptr) hipFree(kv.second.ptr);
+  scratch.clear();
+  // the tables these flags stand for lived in scratch buffers
+  csc_uploaded = false;
+  zc_tables_uploaded = false;
+  return FLOCK_OK;
 }
 
 hipEvent_t Ctx::take_event() {
@@ -150,6 +222,9 @@ HostPool &Ctx::host_pool() {
 }
 
 Ctx::~Ctx() {
+  // Work that timed out may still be running: leave its memory and stream
+  // alone (held until the process ends) rather than block on it.
+  if (timed_out) return;
   hipSetDevice(device);
   for (auto &e : phase_events) {
     hipEventDestroy(e.begin);

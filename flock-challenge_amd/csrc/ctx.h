@@ -14,6 +14,7 @@
 
 #include "host_field.h"
 #include "host_pool.h"
+#include "host_wait.h"
 
 namespace flock {
 
@@ -44,6 +45,20 @@ struct Ctx {
   void resolve_phases();  // after the stream has been synchronized
   hipStream_t stream = nullptr;
   std::string last_error;
+  // Every host wait for the device gives up after this many seconds
+  // (<= 0: no limit).  A wait that gives up marks the context timed out:
+  // its work may still be running, so every later call on it fails, and
+  // destroying it leaks its device memory instead of waiting on that work.
+  double wait_timeout_s = 120.0;
+  bool timed_out = false;
+  // Bounded waits: until a sequence number appears at `flag` (published by
+  // the device in the mapped readback area), until the stream has drained,
+  // until the event has completed.  FLOCK_OK, or FLOCK_E_DEVICE with
+  // last_error set.
+  int wait_flag_seq(const uint32_t *flag, uint32_t seq, hipStream_t st);
+  int wait_stream(hipStream_t st);
+  int wait_event(hipEvent_t e);
+  int wait_failed(WaitEnd how, hipError_t err, const char *what);
   // Breadth-first NTT twiddle table per domain dimension (device copy).
   std::map<int, F128 *> twiddles_dev;
   std::map<int, std::vector<F128>> twiddles_host;
@@ -91,6 +106,9 @@ struct Ctx {
   void pinned_reset() { pinned_off = 0; }
 
   void *get_scratch(const std::string &name, size_t bytes);
+  // Free every named scratch buffer (after the stream has drained); the
+  // next prove allocates again.  FLOCK_OK or the wait's error.
+  int release_scratch();
   const std::vector<F128> &host_twiddles(int dim);
   const F128 *device_twiddles(int dim);
   // Host threads for the prover's host-side loops, started on first use.

@@ -82,7 +82,6 @@ int rs_basis_device(Ctx *ctx, hipStream_t st, const void *tables, const void *lo
 // Whole forward (layers l0..k-1) or transposed (k-1..l0) NTT over 2^k
 // positions x 2^log_nl lanes, up to 4 layers per launch.
 int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l0, int log_nl, int transposed);
-int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed);
 int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
                     void *d_best);
 // out (device-visible) = *best, then *best = all-ones for the next window;

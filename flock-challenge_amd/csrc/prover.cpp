@@ -169,21 +169,16 @@ struct Prover {
     return wait_seq(seq);
   }
   // Poll until seq is published (with the stream's error state checked now
-  // and then, so a failed kernel ends the wait).
+  // and then, so a failed kernel ends the wait, and a wall-clock deadline,
+  // so a kernel that never finishes does too).
   int wait_seq(uint32_t seq) {
-    const uint32_t *flag = (const uint32_t *)(ctx->readback + Ctx::READBACK_FLAG);
-    for (uint32_t spins = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; ++spins) {
-      if ((spins & 4095) == 0) {
-        const hipError_t e = hipStreamQuery(st);
-        if (e != hipSuccess && e != hipErrorNotReady) FLOCK_HIP_CHECK(ctx, e);
-        if (e == hipSuccess && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
-          ctx->last_error = "readback: the stream finished without publishing its signal";
-          return FLOCK_E_DEVICE;
-        }
-      }
-      __builtin_ia32_pause();
-    }
+    if (int rc = ctx->wait_flag_seq((const uint32_t *)(ctx->readback + Ctx::READBACK_FLAG), seq, st)) return rc;
     openings_in_flight = 0;  // everything queued before seq has finished
+    return FLOCK_OK;
+  }
+  int sync() {
+    if (int rc = ctx->wait_stream(st)) return rc;
+    openings_in_flight = 0;
     return FLOCK_OK;
   }
 
@@ -198,9 +193,7 @@ struct Prover {
       return FLOCK_OK;
     }
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, st));
-    FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
-    openings_in_flight = 0;
-    return FLOCK_OK;
+    return sync();
   }
   // Two small device ranges (together at most the landing area) with one
   // wait for the stream.
@@ -230,14 +223,13 @@ struct Prover {
   int h2d(void *d, const void *h, size_t n) {
     uint8_t *p = ctx->pinned_alloc(n);
     if (!p) {
-      FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      if (int rc = sync()) return rc;
       ctx->pinned_reset();
       p = ctx->pinned_alloc(n);
     }
     if (!p) {  // larger than the ring: plain synchronous copy
       FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st));
-      FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
-      return FLOCK_OK;
+      return sync();
     }
     memcpy(p, h, n);
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d, p, n, hipMemcpyHostToDevice, st));
@@ -428,8 +420,7 @@ int commitment_root_device(Ctx *ctx, hipStream_t st, const flock_compression *d_
   void *cw0 = ctx->get_scratch("c.codeword", npos0 * 64 * 16), *tree0 = ctx->get_scratch("c.tree", (2 * npos0 - 1) * 32);
   if (!cw0 || !tree0) return FLOCK_E_OOM;
   if ((rc = commit_device(ctx, &pp, z, cw0, tree0, root, st))) return rc;
-  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
-  return FLOCK_OK;
+  return ctx->wait_stream(st);
 }
 
 int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
@@ -450,7 +441,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   }
   Prover P{ctx, st, m, m - 7, cfg, {}};
   // The stream is idle between proves, so the upload ring can be recycled.
-  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  if (int e = ctx->wait_stream(st)) return e;
   ctx->pinned_reset();
   if (!ctx->landing) {
     void *p = nullptr;
@@ -513,9 +504,8 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
   if ((rc = zc_y_device(ctx, st, a, b, m, dzt))) return rc;
   {
     HostPhase hp(ctx, "wait_d2h");
-    const hipError_t e = hipEventSynchronize(root_ready);
+    if (int e = ctx->wait_event(root_ready)) return e;
     ctx->event_pool.push_back(root_ready);
-    FLOCK_HIP_CHECK(ctx, e);
   }
   memcpy(root, ctx->landing, 32);
 
@@ -823,9 +813,8 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
       return rc;
     {
       HostPhase hp(ctx, "wait_d2h");
-      const hipError_t e = hipEventSynchronize(lc_ready);
+      if (int e = ctx->wait_event(lc_ready)) return e;
       ctx->event_pool.push_back(lc_ready);
-      FLOCK_HIP_CHECK(ctx, e);
     }
     memcpy(comb.data(), ctx->landing, lc_bytes);
     memcpy(zv.data(), ctx->landing + lc_bytes, lc_bytes);
@@ -1158,7 +1147,7 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     B.put(fold_nonces.data(), fold_nonces.size() * 8);
   }
   ph_all.end();
-  FLOCK_HIP_CHECK(ctx, hipStreamSynchronize(st));
+  if ((rc = ctx->wait_stream(st))) return rc;
   out.swap(B.v);
   return FLOCK_OK;
 }

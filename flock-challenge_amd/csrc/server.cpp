@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/flock_gpu.h"
+#include "admission.h"
 #include "ctx.h"
 #include "ligerito_configs.h"
 #include "prover.h"
@@ -74,8 +75,7 @@ struct Result {
 class Server {
  public:
   Server(int device, int n_workers, uint64_t budget, int reserve_log2)
-      : device_(device), budget_(budget), reserve_log2_(reserve_log2) {
-    reserved_.assign(n_workers, 0);
+      : device_(device), reserve_log2_(reserve_log2), adm_(budget, n_workers), live_(n_workers) {
     for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this, i] { worker_main(i); });
   }
   ~Server() {
@@ -87,9 +87,16 @@ class Server {
     for (auto &t : workers_) t.join();
   }
 
-  int submit(Request &&r, uint64_t *id) {
+  int submit(Request &&r, uint64_t *id, std::string &why) {
     std::lock_guard<std::mutex> lk(mu_);
-    if (prove_footprint_bytes(r.log2) > budget_) return FLOCK_E_OOM;
+    if (!adm_.ever_fits(prove_footprint_bytes(r.log2))) {
+      why = "request exceeds the HBM budget";
+      return FLOCK_E_OOM;
+    }
+    if (live_ == 0) {
+      why = "no worker left: every worker's context timed out";
+      return FLOCK_E_DEVICE;
+    }
     r.id = next_id_++;
     r.t_submit = now_s();
     *id = r.id;
@@ -119,13 +126,21 @@ class Server {
     return queue_.size() + running_;
   }
 
+  void set_wait_timeout(double s) {
+    std::lock_guard<std::mutex> lk(mu_);
+    wait_timeout_s_ = s;
+  }
+
  private:
-  // Can worker w take the head request without exceeding the HBM budget?
-  bool admissible(int w) const {
-    const uint64_t need = prove_footprint_bytes(queue_.front().log2);
-    uint64_t total = 0;
-    for (size_t i = 0; i < reserved_.size(); ++i) total += (int)i == w ? std::max(reserved_[i], need) : reserved_[i];
-    return total <= budget_;
+  // Fail a request without running it (under mu_).
+  void fail_locked(Request &&r, int rc, const std::string &why) {
+    Result res;
+    res.id = r.id;
+    res.rc = rc;
+    res.error = why;
+    res.t_submit = res.t_start = r.t_submit;
+    res.t_done = now_s();
+    done_.push_back(std::move(res));
   }
 
   void worker_main(int w) {
@@ -133,6 +148,10 @@ class Server {
     ctx.device = device_;
     hipSetDevice(device_);
     const bool stream_ok = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking) == hipSuccess;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      ctx.wait_timeout_s = wait_timeout_s_;
+    }
     if (stream_ok && reserve_log2_ > 0) {
       // Size every device buffer once for the largest request (one proof of
       // generated compressions), so serving never reallocates.
@@ -142,7 +161,7 @@ class Server {
       warm.seeded = true;
       {
         std::lock_guard<std::mutex> lk(mu_);
-        reserved_[w] = prove_footprint_bytes(warm.log2);
+        adm_.reserve(w, prove_footprint_bytes(warm.log2));
       }
       std::vector<uint8_t> scratch_proof;
       run(ctx, warm, scratch_proof);
@@ -151,11 +170,41 @@ class Server {
       Request req;
       {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return stop_ || (!queue_.empty() && admissible(w)); });
+        if (ctx.timed_out) {
+          // This worker's context may still have work on the device: it
+          // takes no more requests.  With no worker left, the queue can
+          // never drain, so its requests fail now.
+          --live_;
+          if (live_ == 0)
+            while (!queue_.empty()) {
+              fail_locked(std::move(queue_.front()), FLOCK_E_DEVICE, "no worker left: every worker's context timed out");
+              queue_.pop_front();
+            }
+          done_cv_.notify_all();
+          cv_.notify_all();
+          return;
+        }
+        Admission::Action act = Admission::Wait;
+        cv_.wait(lk, [&] {
+          if (stop_) return true;
+          if (queue_.empty()) return false;
+          act = adm_.decide(w, prove_footprint_bytes(queue_.front().log2));
+          return act != Admission::Wait;
+        });
         if (stop_) return;
+        if (act == Admission::Release) {
+          // The head request fits no idle worker: give up this worker's
+          // buffers so that the reservations shrink until it does.
+          adm_.release(w);
+          lk.unlock();
+          ctx.release_scratch();
+          cv_.notify_all();
+          continue;
+        }
         req = std::move(queue_.front());
         queue_.pop_front();
-        reserved_[w] = std::max(reserved_[w], prove_footprint_bytes(req.log2));
+        adm_.take(w, prove_footprint_bytes(req.log2));
+        ctx.wait_timeout_s = wait_timeout_s_;
         ++running_;
       }
       Result res;
@@ -163,11 +212,15 @@ class Server {
       res.t_submit = req.t_submit;
       res.t_start = now_s();
       res.rc = stream_ok ? run(ctx, req, res.proof) : FLOCK_E_DEVICE;
-      if (res.rc) res.error = ctx.last_error;
+      if (res.rc) {
+        if (ctx.timed_out) res.rc = FLOCK_E_DEVICE;
+        res.error = ctx.last_error;
+      }
       res.t_done = now_s();
       {
         std::lock_guard<std::mutex> lk(mu_);
         --running_;
+        adm_.finish(w);
         done_.push_back(std::move(res));
       }
       done_cv_.notify_all();
@@ -192,10 +245,11 @@ class Server {
   }
 
   int device_;
-  uint64_t budget_;
   int reserve_log2_;
   std::vector<std::thread> workers_;
-  std::vector<uint64_t> reserved_;
+  Admission adm_;  // guarded by mu_
+  int live_;       // workers whose context has not timed out
+  double wait_timeout_s_ = 120.0;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
   std::deque<Request> queue_;
@@ -267,8 +321,9 @@ static int submit_common(flock_server *s, flock::Request &&r, uint64_t *id) {
     s->last_error = "no Ligerito configuration for this size (2^8 .. 2^21 compressions)";
     return FLOCK_E_SHAPE;
   }
-  int rc = s->s->submit(std::move(r), id);
-  if (rc) s->last_error = "request exceeds the HBM budget";
+  std::string why;
+  int rc = s->s->submit(std::move(r), id, why);
+  if (rc) s->last_error = why;
   return rc;
 }
 
@@ -314,6 +369,12 @@ extern "C" int flock_server_poll(flock_server *s, int wait_ms, uint64_t *request
   }
   memcpy(out, r.proof.data(), r.proof.size());
   return 1;
+}
+
+extern "C" int flock_server_set_wait_timeout(flock_server *s, double seconds) {
+  if (!s || !(seconds == seconds)) return FLOCK_E_ARG;
+  s->s->set_wait_timeout(seconds);
+  return FLOCK_OK;
 }
 
 extern "C" int flock_server_pending(flock_server *s, uint64_t *n) {

@@ -736,45 +736,6 @@ int rs_basis_device(Ctx *ctx, hipStream_t st, const void *tables, const void *lo
   return FLOCK_OK;
 }
 
-// ---------------------------------------------------------------- generic NTT layer
-// One butterfly layer over 2^k positions x NL lanes; twiddle per block from
-// the breadth-first table.  Forward (additive_ntt_f128.rs:1281-1320):
-// u += t v, v += u.  Transposed: s = u + v, u = s, v = t s + v.
-__global__ __launch_bounds__(MUL_THREADS) void ntt_layer_kernel(f128 *__restrict__ d, const f128 *__restrict__ tw,
-                                                                int k, int l, int log_nl, int transposed) {
-  __shared__ f128 mt[MUL_WAVES][1024];
-  const uint32_t lane = threadIdx.x & 63;
-  f128 *tab = &mt[0][0];
-  const CombLane CL(threadIdx.x);
-  const uint64_t total = 1ull << (k - 1 + log_nl);
-  for (uint64_t t = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; t < total; t += (uint64_t)gridDim.x * MUL_THREADS) {
-    const uint64_t ln = t & ((1ull << log_nl) - 1), bf = t >> log_nl;
-    const int sh = k - l - 1;
-    const uint64_t block = bf >> sh, row = bf & ((1ull << sh) - 1);
-    const uint64_t top = (block << (sh + 1)) + row, bot = top + (1ull << sh);
-    const f128 w = tw[(1ull << l) - 1 + block];
-    f128 u = d[(top << log_nl) + ln], v = d[(bot << log_nl) + ln];
-    if (!transposed) {
-      f128_xor_into(u, f128_mul(v, w, tab, CL));
-      f128_xor_into(v, u);
-    } else {
-      const f128 s = f128_xor(u, v);
-      u = s;
-      v = f128_xor(f128_mul(s, w, tab, CL), v);
-    }
-    d[(top << log_nl) + ln] = u;
-    d[(bot << log_nl) + ln] = v;
-  }
-}
-
-int ntt_layer_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l, int log_nl, int transposed) {
-  const uint64_t total = 1ull << (k - 1 + log_nl);
-  hipLaunchKernelGGL(ntt_layer_kernel, dim3(grid_for(total, MUL_THREADS, 8192)), dim3(MUL_THREADS), 0, st, (f128 *)d,
-                     (const f128 *)tw, k, l, log_nl, transposed);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return FLOCK_OK;
-}
-
 // Several layers per launch for narrow codewords (2^log_nl interleaved lanes,
 // log_nl small): thread = (group, lane) holds the 2^F positions of its group
 // (the positions that exchange data in layers l .. l+F-1) in registers.

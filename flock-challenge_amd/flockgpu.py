@@ -59,6 +59,16 @@ class PcsParams(ctypes.Structure):
     def codeword_len_f128(self):
         return self.n_positions() * self.num_ntts()
 
+    def sizes(self):
+        """(message F128, codeword F128, tree nodes) from the library, which
+        raises FlockError(FLOCK_E_SHAPE) for parameters the commit kernels do
+        not support."""
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        rc = lib().flock_pcs_params_sizes(ctypes.byref(self), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        if rc:
+            raise FlockError(rc, "unsupported PCS parameters")
+        return a.value, b.value, c.value
+
 
 _lib = None
 
@@ -77,6 +87,8 @@ def lib():
         L.flock_last_error.argtypes = [P]
         L.flock_commit.argtypes = [P, ctypes.POINTER(PcsParams), P, ctypes.c_size_t, P]
         L.flock_commit_device.argtypes = [P, ctypes.POINTER(PcsParams), P, P, P, P, P]
+        PU64_ = ctypes.POINTER(ctypes.c_uint64)
+        L.flock_pcs_params_sizes.argtypes = [ctypes.POINTER(PcsParams), PU64_, PU64_, PU64_]
         L.flock_device_count.argtypes = []
         L.flock_device_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
         L.flock_device_free.argtypes = [P, P]
@@ -112,6 +124,8 @@ def lib():
         L.flock_server_submit_seeded.argtypes = [P, U32, U64, P, SZ, PU64]
         L.flock_server_poll.argtypes = [P, ctypes.c_int, PU64, P, SZ, PSZ, PD, PD]
         L.flock_server_pending.argtypes = [P, PU64]
+        L.flock_ctx_set_wait_timeout.argtypes = [P, ctypes.c_double]
+        L.flock_server_set_wait_timeout.argtypes = [P, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -119,13 +133,15 @@ def lib():
 def exported_symbols():
     """Names the header declares, for the ABI smoke test."""
     return ["flock_ctx_create", "flock_ctx_destroy", "flock_last_error", "flock_commit", "flock_commit_device",
+            "flock_pcs_params_sizes",
             "flock_timing_enable", "flock_timing_select", "flock_timing_report", "flock_device_count", "flock_device_alloc",
             "flock_device_free", "flock_memcpy_h2d", "flock_memcpy_d2h", "flock_memcpy_d2d", "flock_synchronize", "flock_setup_create",
             "flock_setup_destroy", "flock_setup_info", "flock_prove_blake3", "flock_prove_blake3_device",
             "flock_prove_blake3_seeded", "flock_generate_compressions_device", "flock_last_proof",
             "flock_verify_blake3", "flock_commitment_root", "flock_commitment_root_seeded", "flock_blake3",
             "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
-            "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending"]
+            "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending",
+            "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout"]
 
 
 def device_count():

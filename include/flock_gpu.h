@@ -60,12 +60,32 @@ int flock_ctx_create(int device, flock_ctx **out);
 void flock_ctx_destroy(flock_ctx *ctx);
 const char *flock_last_error(const flock_ctx *ctx);
 
+/* Deadline of every host wait for the device on this context (readbacks,
+ * stream and event waits), in seconds; <= 0 waits without limit.  Default
+ * 120 s.  A wait past it returns FLOCK_E_DEVICE and marks the context timed
+ * out: its work may still be running, so every later call on it returns
+ * FLOCK_E_DEVICE, and flock_ctx_destroy releases its host state but leaves
+ * its device memory to the process's end instead of waiting on that work.
+ * No counterpart in the reference's prover; its harness bounds a whole
+ * trial from outside (benchmark-tools/harness/src/main.rs:20-21). */
+int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds);
+
 /* PCS commit of an F128-packed witness: replicate, interleaved additive NTT
  * from layer log_inv_rate, BLAKE3 Merkle tree over positions.  Replaces
  * `pcs::commit` (crates/flock-core/src/pcs/commit.rs:250) returning
  * `Commitment.root`.  z_packed: host, 2^(m-7) x 16 bytes. */
 int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, const void *z_packed, size_t z_bytes,
                  uint8_t root[32]);
+
+/* The buffer sizes flock_commit_device needs for `params` (PcsParams::
+ * log_msg_len / codeword_len_f128 / n_leaves, crates/flock-core/src/pcs/
+ * commit.rs:54-90): message and codeword in F128 elements, tree in 32-byte
+ * nodes (2 n_leaves - 1).  Returns FLOCK_E_SHAPE for parameters the commit
+ * kernels do not support (log_batch_size != 6, log_inv_rate outside 1..4,
+ * m outside 14..40, a hash other than BLAKE3); both commit entry points
+ * reject those before touching the device.  Host only, no context. */
+int flock_pcs_params_sizes(const flock_pcs_params *params, uint64_t *msg_f128, uint64_t *codeword_f128,
+                           uint64_t *tree_nodes);
 
 /* Same, device-resident: d_z (2^(m-7) x 16 B), d_codeword (2^(k_code) x
  * 2^log_batch_size x 16 B) and d_tree ((2^(k_code+1) - 1) x 32 B, flat layout of
@@ -104,9 +124,13 @@ int flock_setup_info(const flock_setup *setup, uint32_t *m, uint8_t digest[32]);
 /* Blake3Setup::prove_fast (blake3.rs:3405) followed by the worker's
  * R1csProofBundleLigerito::to_bytes (crates/flock-core/src/proof_io.rs:161-208,
  * worker/src/main.rs:42-45): the proof bundle bytes ("FLOCK" 04 02 + bincode)
- * for `n_blocks` compressions under a BLAKE3 Fiat-Shamir transcript with
- * domain separator `domain` (FsChallenger::new, crates/flock-core/src/
- * challenger.rs:246).  n_blocks must equal the setup's (the reference
+ * for `n_blocks` compressions under the BLAKE3 Fiat-Shamir transcript with
+ * domain separator `domain`: the challenger the benchmark worker builds,
+ * FsChallenger::with_hash(domain, HashKind::Blake3) (benchmark-tools/worker/
+ * src/main.rs:29,41; crates/flock-core/src/challenger.rs:271-283).  The ABI
+ * is BLAKE3-only: the reference's FsChallenger::new(domain) is the SHA-256
+ * transcript (challenger.rs:261-263), which gives different bytes and is not
+ * implemented here.  n_blocks must equal the setup's (the reference
  * asserts, blake3.rs:3520).  If cap is too small, returns FLOCK_E_CAP with
  * *out_len set; the bytes stay available through flock_last_proof. */
 int flock_prove_blake3(flock_setup *setup, const flock_compression *blocks, uint64_t n_blocks,
@@ -167,6 +191,9 @@ void flock_blake3(const void *data, size_t len, uint8_t out[32]);
  * n_workers proofs run concurrently, each on its own HIP stream and buffers;
  * a request is admitted (FIFO) when a worker is free and the HBM footprint of
  * the admitted work stays within hbm_budget_bytes (0: 80% of free memory).
+ * A worker keeps the buffers of the largest proof it has run; when the head
+ * request fits no idle worker, idle workers free theirs until it does, so a
+ * request within the budget is always admitted eventually.
  * reserve_log2 > 0 sizes each worker's buffers for 2^reserve_log2
  * compressions at start (one throwaway proof), so serving never reallocates. */
 typedef struct flock_server flock_server;
@@ -188,6 +215,11 @@ int flock_server_poll(flock_server *server, int wait_ms, uint64_t *request_id, u
                       size_t *out_len, double *latency_s, double *queue_s);
 /* Requests queued or running. */
 int flock_server_pending(flock_server *server, uint64_t *n);
+/* The wait deadline of every worker's context (flock_ctx_set_wait_timeout),
+ * applied from each worker's next request on.  A request whose wait times
+ * out fails with FLOCK_E_DEVICE and its worker retires; when no worker is
+ * left, queued requests fail and submits return FLOCK_E_DEVICE. */
+int flock_server_set_wait_timeout(flock_server *server, double seconds);
 
 /* Device phase timing (HIP events on the launching stream), the counterpart
  * of the reference's FLOCK_PHASE_TIMING / prove_fast_timed
