@@ -328,12 +328,12 @@ static int ntt_pass(Ctx *ctx, hipStream_t st, f128 *data, const f128 *tw, int k,
 }
 
 int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k, int num_lanes, int start_layer,
-                                   int max_fuse, const void *d_msg) {
+                                   int max_fuse, const void *d_msg, const void *d_twiddles) {
   if (num_lanes != 64) {
     ctx->last_error = "ntt: only 64 interleaved lanes are implemented";
     return FLOCK_E_SHAPE;
   }
-  const F128 *tw = ctx->device_twiddles(k);
+  const F128 *tw = d_twiddles ? (const F128 *)d_twiddles : ctx->device_twiddles(k);
   if (!tw) return FLOCK_E_DEVICE;
   int l = start_layer;
   bool first = true;
@@ -392,7 +392,7 @@ const char *commit_params_unsupported(const flock_pcs_params *p) {
 }
 
 int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_codeword, void *d_tree,
-                  uint8_t root[32], hipStream_t st, hipEvent_t root_ready) {
+                  uint8_t root[32], hipStream_t st, hipEvent_t root_ready, const void *d_twiddles) {
   if (p->merkle_hash != 1) {
     ctx->last_error = "only the BLAKE3 Merkle hash is implemented";
     return FLOCK_E_ARG;
@@ -417,7 +417,7 @@ int commit_device(Ctx *ctx, const flock_pcs_params *p, const void *d_z, void *d_
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
   }
   int rc = ntt_forward_interleaved_device(ctx, st, d_codeword, k, 1 << p->log_batch_size, rate, 4,
-                                          fused ? d_z : nullptr);
+                                          fused ? d_z : nullptr, d_twiddles);
   if (rc) return rc;
   rc = merkle_tree_device(ctx, st, d_codeword, 16u << p->log_batch_size, 1ull << k, d_tree);
   if (rc) return rc;

@@ -13,7 +13,7 @@ const Phi8 &phi8_table() {
 // Normalized subspace-polynomial rows evals[i][k] = W_i(b_{i+k}) / W_i(b_i)
 // for the standard basis b_i = x^i, then the breadth-first span table
 // (ntt/additive_ntt_f128.rs:57-89, 106-128).
-std::vector<F128> ntt_twiddles(int dim) {
+std::vector<std::vector<F128>> ntt_subspace_rows(int dim) {
   std::vector<std::vector<F128>> ev(dim);
   ev[0].resize(dim);
   for (int i = 0; i < dim; ++i) ev[0][i] = i < 64 ? F128(1ull << i, 0) : F128(0, 1ull << (i - 64));
@@ -28,6 +28,11 @@ std::vector<F128> ntt_twiddles(int dim) {
     F128 iv = inv(ev[i][0]);
     for (auto &x : ev[i]) x = x * iv;
   }
+  return ev;
+}
+
+std::vector<F128> ntt_twiddles(int dim) {
+  const std::vector<std::vector<F128>> ev = ntt_subspace_rows(dim);
   std::vector<F128> tw(dim ? ((size_t)1 << dim) - 1 : 0);
   for (int layer = 0; layer < dim; ++layer) {
     size_t start = ((size_t)1 << layer) - 1;

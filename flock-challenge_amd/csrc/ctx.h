@@ -168,5 +168,8 @@ struct Error {
 // (crates/flock-core/src/ntt/additive_ntt_f128.rs:57-128): layer l, block b at
 // index (1 << l) - 1 + b.
 std::vector<F128> ntt_twiddles(int dim);
+// The normalized subspace-polynomial rows the table is spanned from:
+// twiddle (l, b) = sum over the set bits i of b of rows[dim - l - 1][1 + i].
+std::vector<std::vector<F128>> ntt_subspace_rows(int dim);
 
 }  // namespace flock
