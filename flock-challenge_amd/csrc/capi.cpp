@@ -7,6 +7,7 @@
 
 #include "../../include/flock_gpu.h"
 #include "commit.h"
+#include "commit_noalloc.h"
 #include "ctx.h"
 #include "ligerito_configs.h"
 #include "prover.h"
@@ -112,6 +113,11 @@ extern "C" int flock_ctx_create(int device, flock_ctx **out) {
     delete c;
     return FLOCK_E_DEVICE;
   }
+  // this GPU's copy of the no-allocation commit's fixed array
+  if (flock::commit_noalloc_init(&c->c)) {
+    delete c;
+    return FLOCK_E_DEVICE;
+  }
   *out = c;
   return FLOCK_OK;
 }
@@ -147,6 +153,15 @@ extern "C" int flock_commit_device(flock_ctx *ctx, const flock_pcs_params *param
   if (!ctx || !params || !d_z || !d_codeword || !d_tree || !root) return FLOCK_E_ARG;
   if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
   return device_rc(&ctx->c, flock::commit_device(&ctx->c, params, d_z, d_codeword, d_tree, root, pick_stream(ctx, stream)));
+}
+
+extern "C" int flock_commit_device_noalloc(flock_ctx *ctx, const flock_pcs_params *params, const void *d_z,
+                                           void *d_codeword, void *d_tree, uint8_t root[32], void *stream) {
+  if (int u = usable(ctx)) return u;
+  if (!params || !d_z || !d_codeword || !d_tree || !root) return FLOCK_E_ARG;
+  if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
+  return device_rc(&ctx->c, flock::commit_device_noalloc(&ctx->c, params, d_z, d_codeword, d_tree, root,
+                                                         pick_stream(ctx, stream)));
 }
 
 extern "C" int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, const void *z_packed, size_t z_bytes,

@@ -87,6 +87,7 @@ def lib():
         L.flock_last_error.argtypes = [P]
         L.flock_commit.argtypes = [P, ctypes.POINTER(PcsParams), P, ctypes.c_size_t, P]
         L.flock_commit_device.argtypes = [P, ctypes.POINTER(PcsParams), P, P, P, P, P]
+        L.flock_commit_device_noalloc.argtypes = [P, ctypes.POINTER(PcsParams), P, P, P, P, P]
         PU64_ = ctypes.POINTER(ctypes.c_uint64)
         L.flock_pcs_params_sizes.argtypes = [ctypes.POINTER(PcsParams), PU64_, PU64_, PU64_]
         L.flock_device_count.argtypes = []
@@ -141,7 +142,7 @@ def exported_symbols():
             "flock_verify_blake3", "flock_commitment_root", "flock_commitment_root_seeded", "flock_blake3",
             "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
             "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending",
-            "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout"]
+            "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout", "flock_commit_device_noalloc"]
 
 
 def device_count():

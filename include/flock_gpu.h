@@ -94,6 +94,17 @@ int flock_pcs_params_sizes(const flock_pcs_params *params, uint64_t *msg_f128, u
 int flock_commit_device(flock_ctx *ctx, const flock_pcs_params *params, const void *d_z, void *d_codeword,
                         void *d_tree, uint8_t root[32], void *stream);
 
+/* flock_commit_device for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same kernels and
+ * results; the twiddle table, the only device memory flock_commit_device
+ * allocates itself, is rebuilt by every call in a fixed array of 16 MiB
+ * per GPU (held until the process ends), so k_code = m - 13 + log_inv_rate
+ * must be at most 20 (FLOCK_E_SHAPE otherwise).  Calls that overlap on one
+ * GPU share that array and race: serialize them. */
+int flock_commit_device_noalloc(flock_ctx *ctx, const flock_pcs_params *params, const void *d_z, void *d_codeword,
+                                void *d_tree, uint8_t root[32], void *stream);
+
 /* Device memory helpers for callers without their own HIP runtime handle
  * (tests, benchmarks, FFI hosts).  Pointers are device pointers on the
  * context's GPU; copies are synchronous with respect to the context stream. */
