@@ -14,6 +14,7 @@ import pytest
 import gpu_util
 import flockgpu
 import oracle_lib as O
+import bundle_layout as BL
 import proof_format as PF
 
 pytestmark = pytest.mark.gpu
@@ -106,6 +107,8 @@ def test_canary_m32_matches_fixture(ctx):
         proof = s.prove_seeded(g["log2"], seed)
         assert len(proof) == g["reference_expected_len"]
         assert O.blake3(proof).hex() == g["proof_blake3"]
+        # field order and counts fixed by the declarations and m32_fast.toml
+        BL.check_layout(proof, 32)
         # the library's own verifier, with the root recomputed on the GPU
         _say("verify canary m=32")
         assert s.commitment_root_seeded(g["log2"], seed) == proof[7:39]

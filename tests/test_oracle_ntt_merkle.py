@@ -152,3 +152,86 @@ def test_commit_composition_small():
     assert np.array_equal(full, cw)
     assert tree.tobytes() == _py_tree(cw.tobytes(), npos)
     assert root == tree[-32:].tobytes()
+
+
+# ---- multiproof known answers: the reference's own structural tests of
+# merkle_multi_proof (crates/flock-core/src/merkle.rs:1461-1550), restated
+# against the oracle, plus a set-level definition of the emission order.
+
+def _oracle_multiproof(tree, n, positions):
+    pos = np.array(positions, dtype=np.uint64)
+    cap = max(1, len(positions)) * max(1, n.bit_length())
+    out = np.zeros(cap * 32, np.uint8)
+    k = O.lib().omerkle_multi_proof(O.ptr(tree), n, O.ptr(pos), pos.size, O.ptr(out), cap)
+    return [out[32 * i:32 * i + 32].tobytes() for i in range(k)]
+
+
+def _node(tree, idx):
+    return tree[32 * idx:32 * idx + 32].tobytes()
+
+
+def _single_path(tree, n, i):
+    """The single opening path (merkle_proof, merkle.rs:819-838): the sibling
+    at every level, leaves first, in the flat layout (level after level)."""
+    out, start, length = [], 0, n
+    while length > 1:
+        out.append(_node(tree, start + (i ^ 1)))
+        start += length
+        length >>= 1
+        i >>= 1
+    return out
+
+
+def _set_order(tree, n, positions):
+    """The multiproof's hashes defined over sets: at each level from the leaves
+    up, the siblings of the active nodes that are not active themselves, in
+    ascending position order; the parents become the next active set."""
+    act, out, start, length = set(positions), [], 0, n
+    while length > 1:
+        need = sorted({p ^ 1 for p in act} - act)
+        out += [_node(tree, start + p) for p in need]
+        act = {p >> 1 for p in act}
+        start += length
+        length >>= 1
+    return out
+
+
+def _tree(n, leaf, seed):
+    rng = np.random.default_rng(seed)
+    return O.merkle_tree(rng.integers(0, 256, n * leaf, dtype=np.uint8), n)
+
+
+def test_multiproof_single_position_equals_single_path():
+    n = 16
+    tree = _tree(n, 8, 42)
+    for i in range(n):
+        assert _oracle_multiproof(tree, n, [i]) == _single_path(tree, n, i)
+
+
+def test_multiproof_sibling_pair_emits_no_leaf_level_hash():
+    n = 8
+    tree = _tree(n, 4, 7)
+    got = _oracle_multiproof(tree, n, [0, 1])
+    # one sibling per level above the leaves: node 1 of level 1, node 1 of level 2
+    assert got == [_node(tree, 8 + 1), _node(tree, 12 + 1)]
+
+
+def test_multiproof_full_query_set_is_empty():
+    n = 16
+    tree = _tree(n, 8, 99)
+    assert _oracle_multiproof(tree, n, list(range(n))) == []
+
+
+def test_multiproof_random_subsets_follow_the_set_order():
+    rng = np.random.default_rng(2024)
+    for n in (2, 8, 64, 256):
+        tree = _tree(n, 16, n)
+        for _ in range(20):
+            k = int(rng.integers(1, n + 1))
+            pos = sorted(set(int(p) for p in rng.integers(0, n, k)))
+            got = _oracle_multiproof(tree, n, pos)
+            assert got == _set_order(tree, n, pos), (n, pos)
+            # unsorted input with repeats gives the same hashes
+            shuffled = list(pos) * 2
+            rng.shuffle(shuffled)
+            assert _oracle_multiproof(tree, n, shuffled) == got

@@ -197,10 +197,53 @@ def test_lagrange_weights_interpolate():
         assert acc == p(z)
 
 
-def test_statement_digest_is_stable_and_m_dependent():
-    d22 = O.statement_digest(22)
-    assert d22 == O.statement_digest(22)
-    assert d22 != O.statement_digest(23)
+class _Csr(ctypes.Structure):
+    _fields_ = [("row_ptr", ctypes.POINTER(ctypes.c_uint32)), ("cols", ctypes.POINTER(ctypes.c_uint32)),
+                ("nnz", ctypes.c_size_t)]
+
+
+def _csr_arrays(handle):
+    s = ctypes.cast(handle, ctypes.POINTER(_Csr)).contents
+    ptr = np.ctypeslib.as_array(s.row_ptr, shape=(K + 1,)).astype(np.uint64)
+    cols = np.ctypeslib.as_array(s.cols, shape=(s.nnz,)).astype(np.uint64)
+    return ptr, cols
+
+
+def _framed_matrix(ptr, cols):
+    """absorb_matrix (crates/flock-core/src/r1cs.rs:431-440) as one byte
+    string: rows, cols, then per row its length and its column indices, all
+    u64 little-endian; assembled with numpy, row by row in one vector."""
+    lens = np.diff(ptr)
+    # each row contributes [len, col...]: place the lengths at the row starts
+    out = np.empty(K + cols.size, np.uint64)
+    starts = ptr[:-1] + np.arange(K, dtype=np.uint64)
+    mask = np.ones(out.size, bool)
+    mask[starts.astype(np.int64)] = False
+    out[starts.astype(np.int64)] = lens
+    out[mask] = cols
+    return np.array([K, K], np.uint64).tobytes() + out.astype("<u8").tobytes()
+
+
+def test_statement_digest_framing():
+    """BlockR1cs::statement_digest (r1cs.rs:343-365, 431-459) assembled
+    independently from the oracle's matrices and hashed by the library's own
+    BLAKE3 (a separate implementation): equal to the oracle's digest for
+    m = 22 and 32; different m give different digests.  The matrices
+    themselves are pinned by R1CS satisfaction above; the digest value has
+    no reference datum (only the canary proof's length depends on it)."""
+    import gpu_util  # noqa: F401  (puts the package on sys.path)
+    import flockgpu
+    a, b = _sparse()
+    fa, fb = _framed_matrix(*_csr_arrays(a)), _framed_matrix(*_csr_arrays(b))
+    ident = np.empty(2 * K, np.uint64)
+    ident[0::2] = 1
+    ident[1::2] = np.arange(K, dtype=np.uint64)
+    fc = np.array([K, K], np.uint64).tobytes() + ident.astype("<u8").tobytes()
+    for m in (22, 32):
+        head = b"flock-r1cs-stmt-v1" + np.array([m, 14, 6], "<u8").tobytes() + b"\x00"  # RowMajor
+        want = flockgpu.blake3(head + fa + fb + fc)
+        assert O.statement_digest(m) == want, m
+    assert O.statement_digest(22) != O.statement_digest(23)
 
 
 @pytest.fixture(scope="module")
