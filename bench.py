@@ -31,6 +31,13 @@ sys.path.insert(0, os.path.join(ROOT, "flock-challenge_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# 32-bit integer VALU peak of MI355X: 256 CUs x 64 lanes x 2.4 GHz (one
+# wave64 VALU instruction per CU per clock), in lane operations per second.
+VALU_PEAK_OPS = 256 * 64 * 2.4e9
+# Compulsory HBM bytes of one proven compression at m = 32 (witness write,
+# commit read/write and tree, zerocheck, lincheck, ring-switch and Ligerito
+# reads), 7.07 GiB per 2^18 proof.
+PROOF_BYTES_PER_COMPRESSION = 28928
 
 
 def cpu_cores():
@@ -42,6 +49,34 @@ def cpu_cores():
 
 WORKER_SEED = 0x00C0FFEEBEEFD15C
 PROOF_BOUND = 500_000  # harness/src/main.rs proof-size bound
+
+
+def cpu_topology():
+    """Physical cores of this process's affinity mask (distinct (package,
+    core) pairs from /sys/devices/system/cpu/*/topology, the reference's own
+    count, crates/flock-core/src/lib.rs:218-273) and the CPU model."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except Exception:
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            pkg = open(os.path.join(base, "physical_package_id")).read().strip()
+            core = open(os.path.join(base, "core_id")).read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"physical_cores": len(cores), "logical_cpus": len(cpus), "cpu_model": model}
 
 
 def omp_threads():
@@ -82,6 +117,18 @@ def load_traffic(kernel):
         with open(path) as f:
             d = json.load(f)
         return d["kernels"][kernel]["traffic_bytes"]
+    except Exception:
+        return None
+
+
+def load_valu(kernel):
+    """VALU wave instructions per launch of `kernel` (SQ_INSTS_VALU per
+    dispatch) from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "prove_m32_valu.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["valu_insts_per_dispatch"]
     except Exception:
         return None
 
@@ -142,12 +189,13 @@ def bench_prove(args):
     ctx.timing(False)
     for p in proofs:
         assert len(p) < PROOF_BOUND and len(proof_format.fields(p)) > 0
-    # Acceptance of a timed proof by the library's verifier, its commitment
-    # root recomputed from the compressions on the GPU (untimed).
+    # Acceptance of every timed proof by the library's verifier, its
+    # commitment root recomputed from the compressions on the GPU (untimed).
     verified = None
     if not args.no_verify:
-        setup.verify_seeded(log2, timed_seeds[0], proofs[0])
-        verified = True
+        for s, p in zip(timed_seeds, proofs):
+            setup.verify_seeded(log2, s, p)
+        verified = len(proofs)
 
     ms_per_step = 1000.0 * wall / args.steps
     dev_ms = rep["prove"][1] / rep["prove"][0]
@@ -178,7 +226,20 @@ def bench_prove(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(kname), "kernel": kname,
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": abytes},
+        # the whole proof against its compulsory HBM bytes
+        "whole_proof": {"algorithmic_bytes": PROOF_BYTES_PER_COMPRESSION * n,
+                        "achieved_GBps": PROOF_BYTES_PER_COMPRESSION * n / (ms_per_step * 1e-3) / 1e9,
+                        "frac_hbm": PROOF_BYTES_PER_COMPRESSION * n / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "hbm_floor_ms": PROOF_BYTES_PER_COMPRESSION * n / (HBM_PEAK_GBS * 1e9) * 1e3},
     }
+    valu = load_valu(kname)
+    if valu:
+        # the dominant kernel's VALU throughput: committed SQ_INSTS_VALU per
+        # launch (wave instructions, x 64 lanes) over the live launch time
+        ops = valu * 64 / (avg_ms * 1e-3)
+        out["roofline"]["valu"] = {"bound": "valu", "achieved": ops / 1e12, "peak": VALU_PEAK_OPS / 1e12,
+                                   "unit": "T lane-ops/s", "frac": ops / VALU_PEAK_OPS,
+                                   "wave_insts_per_launch": valu}
     if not args.no_cpu_baseline:
         import oracle_lib as O
         # Oracle proofs of 2^12 compressions (m = 26), repeated for ~10 s.
@@ -191,10 +252,23 @@ def bench_prove(args):
             if time.perf_counter() - t0 > 10.0:
                 break
         dt = time.perf_counter() - t0
+        topo = cpu_topology()
         out["cpu_baseline"] = {"value": k * 4096 / dt, "unit": "compressions/s", "cores": omp_threads(),
-                               "kind": "port",
-                               "sample": f"{k} oracle proofs of 2^12 compressions (m=26) in {dt:.1f} s; the "
-                                         "2^18 oracle proof takes minutes, so the sample is the 2^12 proof"}
+                               "kind": "port", **topo,
+                               "sample": f"{k} oracle proofs of 2^12 compressions (m=26) in {dt:.1f} s on "
+                                         f"{omp_threads()} OpenMP threads; one 2^18 oracle proof takes minutes "
+                                         "(bench.py --cpu-full)"}
+        if args.cpu_full:
+            # one oracle proof at the headline size, timed once
+            cb = O.generate_compressions(18, WORKER_SEED)
+            t0 = time.perf_counter()
+            p = O.prove(cb)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline_full"] = {"value": n / dt, "unit": "compressions/s", "seconds": dt,
+                                        "cores": omp_threads(), "kind": "port", **topo,
+                                        "sample": "one oracle proof of 2^18 compressions (m=32), worker seed",
+                                        "equals_gpu": p == setup.prove_fast_device(bufs[WORKER_SEED].ptr, n)
+                                        if WORKER_SEED in bufs else None}
     return out
 
 
@@ -244,24 +318,25 @@ def summarize_served(out, span):
             "p99_ms": 1000 * percentile(lats, 99), "max_ms": 1000 * lats[-1]}
 
 
-def check_sample(out, n_check):
-    """Verifier acceptance (the library's verifier, commitment roots
-    recomputed on the GPU) for the n_check smallest results (untimed)."""
+def check_sample(out):
+    """Verifier acceptance of every result (the library's verifier, the
+    commitment root recomputed on the GPU), untimed; returns the count."""
     import flockgpu
     import proof_format
     for v in out.values():
         assert len(v[2]) < PROOF_BOUND and len(proof_format.fields(v[2])) > 0
     ok = 0
     ctx = flockgpu.Context(0)
+    setups = {}
     try:
-        for log2, seed, proof, _ in sorted(out.values())[:n_check]:
-            s = flockgpu.Blake3Setup(ctx, 1 << log2)
-            try:
-                s.verify_seeded(log2, seed, proof)
-            finally:
-                s.close()
+        for log2, seed, proof, _ in sorted(out.values()):
+            if log2 not in setups:
+                setups[log2] = flockgpu.Blake3Setup(ctx, 1 << log2)
+            setups[log2].verify_seeded(log2, seed, proof)
             ok += 1
     finally:
+        for s in setups.values():
+            s.close()
         ctx.close()
     return ok
 
@@ -284,20 +359,28 @@ def bench_batched(args):
         server.close()
     res = summarize_served(out, span)
     res["workers"] = args.workers
-    res["verified_sample"] = check_sample(out, 2) if not args.no_verify else 0
+    res["verified"] = check_sample(out) if not args.no_verify else 0
+    # latency by instance size (all submitted at t = 0: completion order)
+    by = {}
+    for log2, _, _, lat in out.values():
+        by.setdefault(log2, []).append(lat)
+    res["latency_ms_by_log2"] = {str(k): {"n": len(v), "p50": 1000 * percentile(v, 50),
+                                          "max": 1000 * max(v)} for k, v in sorted(by.items())}
     return res
 
 
-def bench_served(args):
-    """Open-loop Poisson arrivals at args.rate requests/s, sizes 2^12..2^18
-    (log2 uniform), seeded inputs; p50/p99 latency submit -> bytes ready."""
+def bench_served(args, rate=None, verify=True):
+    """Open-loop Poisson arrivals at `rate` (default args.rate) requests/s,
+    sizes 2^12..2^18 (log2 uniform), seeded inputs; p50/p99 latency submit ->
+    bytes ready."""
     import flockgpu
     import random
+    rate = rate or args.rate
     rng = random.Random(SERVE_SEED)
     g = splitmix64(SERVE_SEED)
     t, reqs = 0.0, []
     for _ in range(args.requests):
-        t += rng.expovariate(args.rate)
+        t += rng.expovariate(rate)
         reqs.append((t, 12 + next(g) % 7, next(g)))
     server = flockgpu.ProofServer(0, n_workers=args.workers, reserve_log2=18)
     try:
@@ -306,8 +389,8 @@ def bench_served(args):
     finally:
         server.close()
     res = summarize_served(out, span)
-    res.update({"offered_rate_per_s": args.rate, "workers": args.workers,
-                "verified_sample": check_sample(out, 3) if not args.no_verify else 0})
+    res.update({"offered_rate_per_s": rate, "workers": args.workers,
+                "verified": check_sample(out) if verify and not args.no_verify else 0})
     return res
 
 
@@ -414,6 +497,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--workers", type=int, default=3, help="concurrent proofs in the batched/served runs")
     ap.add_argument("--rate", type=float, default=100.0, help="served: offered load, requests/s")
+    ap.add_argument("--sweep", default="150,200,250",
+                    help="served: further offered loads (requests/s) measured after --rate; '' to skip")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time one 2^18 oracle proof on the host (minutes)")
     ap.add_argument("--requests", type=int, default=200, help="served: number of requests")
     ap.add_argument("--harness-runs", type=int, default=5,
                     help="measured trials of the challenge-protocol harness (0: skip)")
@@ -430,6 +517,16 @@ def main():
         if not args.no_serve:
             out["batched"] = bench_batched(args)
             out["served"] = bench_served(args)
+            sweep = [out["served"]]
+            for r in [float(x) for x in args.sweep.split(",") if x.strip()]:
+                sweep.append(bench_served(args, rate=r, verify=False))
+            out["served_sweep"] = [{k: v[k] for k in ("offered_rate_per_s", "compressions_per_s", "p50_ms", "p99_ms",
+                                                      "wall_s", "requests")} for v in sweep]
+            # the knee: the highest offered load whose p99 stays within twice
+            # the lowest load's p99
+            base = sweep[0]["p99_ms"]
+            ok = [v["offered_rate_per_s"] for v in sweep if v["p99_ms"] <= 2 * base]
+            out["served_knee_rate_per_s"] = max(ok) if ok else None
         if args.harness_runs > 0:
             out["harness"] = bench_harness(args)
     print(json.dumps(out))

@@ -12,6 +12,7 @@
 #include "ligerito_configs.h"
 #include "prover.h"
 #include "r1cs.h"
+#include "stmt_digest.h"
 #include "host_blake3.h"
 #include "verifier.h"
 
@@ -226,12 +227,33 @@ extern "C" int flock_setup_create(flock_ctx *ctx, uint64_t n_blocks, uint32_t pr
   s->n_blocks = n_blocks;
   s->m = m;
   flock::r1cs::matrices();
+  // the digest's m-independent spine: persisted, else hashed on this GPU
+  flock::r1cs::spine(&ctx->c);
   flock::r1cs::statement_digest((int)m, s->digest);
   *out = s;
   return FLOCK_OK;
 }
 
 extern "C" void flock_setup_destroy(flock_setup *setup) { delete setup; }
+
+extern "C" int flock_statement_digest(uint32_t m, uint8_t digest[32]) {
+  if (!digest || m < 14 || m > 40) return FLOCK_E_ARG;
+  flock::r1cs::statement_digest((int)m, digest);
+  return FLOCK_OK;
+}
+
+extern "C" int flock_statement_setup(flock_ctx *ctx, char *from, size_t from_cap) {
+  if (int u = usable(ctx)) return u;
+  if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
+  std::string src;
+  flock::r1cs::spine(&ctx->c, &src);
+  if (from && from_cap) {
+    const size_t k = src.size() < from_cap - 1 ? src.size() : from_cap - 1;
+    memcpy(from, src.data(), k);
+    from[k] = 0;
+  }
+  return FLOCK_OK;
+}
 
 extern "C" int flock_setup_info(const flock_setup *setup, uint32_t *m, uint8_t digest[32]) {
   if (!setup) return FLOCK_E_ARG;

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "host_blake3.h"
+#include "stmt_digest.h"
 
 namespace flock {
 namespace r1cs {
@@ -183,23 +184,9 @@ const Matrices &matrices() {
   return m;
 }
 
-static void absorb_u64(hb3::Hasher &h, uint64_t v) { h.update(&v, 8); }
-
-static void absorb_csr(hb3::Hasher &h, const Csr &c) {
-  absorb_u64(h, K);
-  absorb_u64(h, K);
-  std::vector<uint64_t> buf;
-  for (int r = 0; r < K; ++r) {
-    uint32_t b = c.ptr[r], e = c.ptr[r + 1];
-    buf.assign(1, (uint64_t)(e - b));
-    for (uint32_t k = b; k < e; ++k) buf.push_back(c.cols[k]);
-    h.update(buf.data(), 8 * buf.size());
-  }
-}
-
 void statement_digest(int m, uint8_t out[32]) {
   // One digest per m, kept for the process (several sizes may be proved
-  // concurrently by a server).
+  // concurrently by a server); each is folded from the spine (stmt_digest.h).
   static std::mutex mu;
   static std::map<int, std::array<uint8_t, 32>> cache;
   {
@@ -210,23 +197,7 @@ void statement_digest(int m, uint8_t out[32]) {
       return;
     }
   }
-  const Matrices &mx = matrices();
-  hb3::Hasher h;
-  h.update("flock-r1cs-stmt-v1", 18);
-  absorb_u64(h, (uint64_t)m);
-  absorb_u64(h, K_LOG);
-  absorb_u64(h, K_SKIP);
-  uint8_t layout = 0;
-  h.update(&layout, 1);
-  absorb_csr(h, mx.a);
-  absorb_csr(h, mx.b);
-  absorb_u64(h, K);
-  absorb_u64(h, K);
-  for (uint64_t i = 0; i < K; ++i) {
-    uint64_t row[2] = {1, i};
-    h.update(row, 16);
-  }
-  h.finalize_xof(out, 32);
+  digest_from_spine(spine(), m, out);
   std::array<uint8_t, 32> d;
   memcpy(d.data(), out, 32);
   std::lock_guard<std::mutex> lk(mu);
@@ -234,13 +205,8 @@ void statement_digest(int m, uint8_t out[32]) {
 }
 
 void precompute_digests(int m_lo, int m_hi) {
-  std::vector<std::thread> ts;
-  for (int m = m_lo; m <= m_hi; ++m)
-    ts.emplace_back([m] {
-      uint8_t d[32];
-      statement_digest(m, d);
-    });
-  for (auto &t : ts) t.join();
+  uint8_t d[32];
+  for (int m = m_lo; m <= m_hi; ++m) statement_digest(m, d);
 }
 
 }  // namespace r1cs

@@ -27,6 +27,7 @@
 #include "ligerito_configs.h"
 #include "prover.h"
 #include "r1cs.h"
+#include "stmt_digest.h"
 
 namespace flock {
 
@@ -287,7 +288,15 @@ extern "C" int flock_server_create(int device, int n_workers, uint64_t hbm_budge
     hbm_budget_bytes = free_b / 10 * 8;
   }
   flock::r1cs::matrices();  // build once before the workers start
-  // statement digests of every supported size (setup work, as in the reference)
+  // statement digests of every supported size (setup work, as in the
+  // reference): the spine from the persisted cache, else hashed on the GPU
+  {
+    flock::Ctx setup_ctx;
+    setup_ctx.device = device;
+    hipSetDevice(device);
+    if (hipStreamCreateWithFlags(&setup_ctx.stream, hipStreamNonBlocking) == hipSuccess)
+      flock::r1cs::spine(&setup_ctx);
+  }
   flock::r1cs::precompute_digests(flock::r1cs::K_LOG + 8, flock::r1cs::K_LOG + 21);
   flock_server *s = new (std::nothrow) flock_server();
   if (!s) return FLOCK_E_OOM;

@@ -126,6 +126,8 @@ def lib():
         L.flock_server_poll.argtypes = [P, ctypes.c_int, PU64, P, SZ, PSZ, PD, PD]
         L.flock_server_pending.argtypes = [P, PU64]
         L.flock_ctx_set_wait_timeout.argtypes = [P, ctypes.c_double]
+        L.flock_statement_digest.argtypes = [U32, P]
+        L.flock_statement_setup.argtypes = [P, ctypes.c_char_p, SZ]
         L.flock_server_set_wait_timeout.argtypes = [P, ctypes.c_double]
         _lib = L
     return _lib
@@ -142,7 +144,8 @@ def exported_symbols():
             "flock_verify_blake3", "flock_commitment_root", "flock_commitment_root_seeded", "flock_blake3",
             "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
             "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending",
-            "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout", "flock_commit_device_noalloc"]
+            "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout", "flock_commit_device_noalloc",
+            "flock_statement_digest", "flock_statement_setup"]
 
 
 def device_count():
@@ -250,6 +253,13 @@ class Context:
         return bytes(root)
 
     # ---------------------------------------------------------------- timing
+    def statement_setup(self):
+        """Prepare the statement digest's spine on this GPU (persisted cache
+        first); returns where it came from."""
+        buf = ctypes.create_string_buffer(16)
+        self._check(lib().flock_statement_setup(self.handle, buf, len(buf)))
+        return buf.value.decode()
+
     def timing(self, on=True, only=None):
         """Enable phase timing; `only` = iterable of phase names to time (None: all)."""
         names = ",".join(only).encode() if only else None
@@ -368,6 +378,15 @@ def verify(proof, n_blocks, domain=BENCH_DOMAIN, expected_root=None):
         raise VerifyError(rc, err.value.decode(errors="replace"))
     if rc:
         raise FlockError(rc, err.value.decode(errors="replace"))
+
+
+def statement_digest(m):
+    """The R1CS statement digest for m (host, no GPU)."""
+    out = (ctypes.c_uint8 * 32)()
+    rc = lib().flock_statement_digest(m, out)
+    if rc:
+        raise FlockError(rc, f"statement digest for m = {m}")
+    return bytes(out)
 
 
 def blake3(data):

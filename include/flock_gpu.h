@@ -132,6 +132,23 @@ void flock_setup_destroy(flock_setup *setup);
  * crates/flock-core/src/r1cs.rs:343-365). */
 int flock_setup_info(const flock_setup *setup, uint32_t *m, uint8_t digest[32]);
 
+/* The R1CS statement digest for setups of m = 14 + log2(padded blocks)
+ * (BlockR1cs::statement_digest, crates/flock-core/src/r1cs.rs:343-365): the
+ * digest the transcript binds first.  Host only, no context: it folds the
+ * digest of this m from the statement's m-independent spine (the BLAKE3 tree
+ * of the 170 MB matrix framing, which only its first chunk makes depend on
+ * m), loading the spine from the persisted cache or hashing it on host
+ * threads when neither this process nor the cache has it. */
+int flock_statement_digest(uint32_t m, uint8_t digest[32]);
+
+/* Setup work for the statement digest on the context's GPU: the spine is
+ * taken from this process's memory, else from the persisted cache (a small
+ * file under $FLOCK_CACHE_DIR, else lib/cache next to the library, keyed by a
+ * fingerprint of the matrices), else hashed on the GPU and persisted.
+ * flock_setup_create does this itself.  `from` (may be NULL) receives where
+ * it came from: "memory", "file", "gpu" or "host" (the GPU hash failed). */
+int flock_statement_setup(flock_ctx *ctx, char *from, size_t from_cap);
+
 /* Blake3Setup::prove_fast (blake3.rs:3405) followed by the worker's
  * R1csProofBundleLigerito::to_bytes (crates/flock-core/src/proof_io.rs:161-208,
  * worker/src/main.rs:42-45): the proof bundle bytes ("FLOCK" 04 02 + bincode)
