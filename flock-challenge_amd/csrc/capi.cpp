@@ -6,6 +6,7 @@
 #include <new>
 
 #include "../../include/flock_gpu.h"
+#include "batch.h"
 #include "commit.h"
 #include "commit_noalloc.h"
 #include "ctx.h"
@@ -423,5 +424,45 @@ extern "C" int flock_timing_report(flock_ctx *ctx, char *buf, size_t cap) {
   }
   if (out.size() + 1 > cap) return FLOCK_E_CAP;
   memcpy(buf, out.c_str(), out.size() + 1);
+  return FLOCK_OK;
+}
+
+extern "C" int flock_prove_batch(flock_ctx *ctx, const flock_batch_item *items, uint32_t n_items,
+                                 const uint8_t *domain, size_t domain_len, uint32_t n_workers, int use_graph,
+                                 uint8_t *out, size_t cap, uint64_t *offsets, double *done_s, double *front_s,
+                                 int *replayed) {
+  if (int u = usable(ctx)) return u;
+  if ((!items && n_items) || !offsets || (!domain && domain_len)) return FLOCK_E_ARG;
+  if (hipSetDevice(ctx->c.device) != hipSuccess) return FLOCK_E_DEVICE;
+  std::vector<flock::BatchItem> v(n_items);
+  for (uint32_t i = 0; i < n_items; ++i) {
+    v[i].blocks = items[i].d_blocks;
+    v[i].n_blocks = items[i].n_blocks;
+    v[i].log2 = items[i].log2;
+    v[i].seed = items[i].seed;
+    if (!v[i].blocks && v[i].log2 > 21) return FLOCK_E_ARG;
+  }
+  std::vector<std::vector<uint8_t>> proofs;
+  std::vector<double> done;
+  flock::BatchStats stats;
+  const int rc = flock::prove_batch(&ctx->c, v, domain, domain_len, (int)n_workers, use_graph != 0, proofs, done,
+                                    &stats);
+  if (rc) return device_rc(&ctx->c, rc);
+  uint64_t at = 0;
+  for (uint32_t i = 0; i < n_items; ++i) {
+    offsets[i] = at;
+    at += proofs[i].size();
+  }
+  offsets[n_items] = at;
+  if (done_s)
+    for (uint32_t i = 0; i < n_items; ++i) done_s[i] = done[i];
+  if (front_s) *front_s = stats.front_s;
+  if (replayed) *replayed = stats.graph_replayed ? 1 : 0;
+  if (!out || cap < at) {
+    ctx->c.last_error = "batch: the proofs need " + std::to_string(at) + " bytes";
+    return FLOCK_E_CAP;
+  }
+  for (uint32_t i = 0; i < n_items; ++i)
+    if (!proofs[i].empty()) memcpy(out + offsets[i], proofs[i].data(), proofs[i].size());
   return FLOCK_OK;
 }

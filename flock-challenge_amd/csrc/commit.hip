@@ -74,13 +74,11 @@ __device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__
 }
 
 template <int F>
-__global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *__restrict__ data, const f128 *__restrict__ tw, int k,
-                                                      int l) {
-  __shared__ __align__(16) char lds[2 * NIBTAB_BYTES];
+__device__ __forceinline__ void ntt_tail_body(f128 *__restrict__ data, const f128 *__restrict__ tw, int k, int l,
+                                              uint64_t g, char *lds) {
   const uint32_t lane = threadIdx.x;
   const NibtabLane NL(lane & 15u);
   const int s = k - l - F;
-  const uint64_t g = blockIdx.x;
   const uint64_t r = g & ((1ull << s) - 1ull);
   const uint64_t B = g >> s;
   const uint64_t base = (B << (k - l)) | r;
@@ -90,6 +88,30 @@ __global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *__restrict__ data, c
   tail_pass_step<F, 0, 0>(v, tw, l, B, lds, NL, lane);
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
+}
+
+template <int F>
+__global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *__restrict__ data, const f128 *__restrict__ tw, int k,
+                                                      int l) {
+  __shared__ __align__(16) char lds[2 * NIBTAB_BYTES];
+  ntt_tail_body<F>(data, tw, k, l, blockIdx.x, lds);
+}
+
+// The job a workgroup of a fused launch belongs to: the last one whose first
+// workgroup is at most blockIdx.x (jobs sorted by wg_begin).
+template <class Job>
+__device__ __forceinline__ const Job &job_of(const Job *__restrict__ jobs, uint32_t n_jobs) {
+  uint32_t j = 0;
+  for (uint32_t i = 1; i < n_jobs; ++i)
+    if (jobs[i].wg_begin <= blockIdx.x) j = i;
+  return jobs[j];
+}
+
+template <int F>
+__global__ __launch_bounds__(64) void ntt_tail_batch_kernel(const NttJob *__restrict__ jobs, uint32_t n_jobs) {
+  __shared__ __align__(16) char lds[2 * NIBTAB_BYTES];
+  const NttJob &J = job_of(jobs, n_jobs);
+  ntt_tail_body<F>(J.data, J.tw, J.k, J.l, blockIdx.x - J.wg_begin, lds);
 }
 
 // Shared-table pass.  When 2^s >= WAVES groups share the block B of the pass's
@@ -145,21 +167,21 @@ __device__ __forceinline__ void shared_pass_step(f128 (&v)[G][1 << F], f128 *tab
 
 // G groups per wavefront: a workgroup takes WAVES * G groups that share
 // block B (so 2^s >= WAVES * G), and each table serves all of them.
+constexpr int SHARED_LDS_F128 = 2 * 4096 + SHARED_TWIDDLES * 128;
+
 template <int F, int WAVES, int G>
-__global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__restrict__ data,
-                                                                     const f128 *__restrict__ tw, int k, int l,
-                                                                     const f128 *__restrict__ src,
-                                                                     uint64_t src_pos_mask) {
+__device__ __forceinline__ void ntt_pass_shared_body(f128 *__restrict__ data, const f128 *__restrict__ tw, int k,
+                                                     int l, const f128 *__restrict__ src, uint64_t src_pos_mask,
+                                                     uint64_t wg, f128 *lds) {
   constexpr uint32_t THREADS = 64 * WAVES;
   static_assert((1 << F) - 1 <= SHARED_TWIDDLES, "twiddle basis area");
-  __shared__ f128 lds[2 * 4096 + SHARED_TWIDDLES * 128];
   f128 *tabs = lds;                // 2 x 16 x 256
   f128 *basis = lds + 2 * 4096;    // SHARED_TWIDDLES x 128
   const uint32_t tid = threadIdx.x, lane = tid & 63, rot = lane & 15;
   const FulltabLane FL(rot);
   const int s = k - l - F;
   // wavefront-uniform group indices (scalar registers for the addresses)
-  const uint64_t g0 = ((uint64_t)blockIdx.x * WAVES + __builtin_amdgcn_readfirstlane(tid >> 6)) * G;
+  const uint64_t g0 = (wg * WAVES + __builtin_amdgcn_readfirstlane(tid >> 6)) * G;
   const uint64_t B = g0 >> s;  // the same for the whole workgroup
   // basis of every twiddle: entry n * 128 + i * 16 + kb = t_n x^(8 kb + i),
   // t_n = twiddle n of the pass in (layer offset j, block c) order,
@@ -190,6 +212,24 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__res
     for (int e = 0; e < (1 << F); ++e) data[((base[gi] | ((uint64_t)e << s)) << 6) + lane] = v[gi][e];
 }
 
+template <int F, int WAVES, int G>
+__global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *__restrict__ data,
+                                                                     const f128 *__restrict__ tw, int k, int l,
+                                                                     const f128 *__restrict__ src,
+                                                                     uint64_t src_pos_mask) {
+  __shared__ f128 lds[SHARED_LDS_F128];
+  ntt_pass_shared_body<F, WAVES, G>(data, tw, k, l, src, src_pos_mask, blockIdx.x, lds);
+}
+
+template <int F, int WAVES, int G>
+__global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_batch_kernel(const NttJob *__restrict__ jobs,
+                                                                           uint32_t n_jobs) {
+  __shared__ f128 lds[SHARED_LDS_F128];
+  const NttJob &J = job_of(jobs, n_jobs);
+  ntt_pass_shared_body<F, WAVES, G>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data, J.src ? J.src_pos_mask : ~0ull,
+                                    blockIdx.x - J.wg_begin, lds);
+}
+
 // Replicate the message into every 2^log_inv_rate sub-block: the exact state
 // after the first log_inv_rate layers (pcs/commit.rs:451-467).
 __global__ void replicate_kernel(const uint4 *__restrict__ msg, uint4 *__restrict__ cw, uint64_t msg_len,
@@ -202,9 +242,9 @@ __global__ void replicate_kernel(const uint4 *__restrict__ msg, uint4 *__restric
 // Leaf = non-root BLAKE3 chunk chaining value of leaf_bytes (<= 1024) bytes
 // (merkle.rs:200-205): counter 0, CHUNK_START on the first block, CHUNK_END on
 // the last, final block zero-padded with its true length.
-__global__ __launch_bounds__(256) void merkle_leaf_kernel(const uint4 *__restrict__ data, uint32_t leaf_bytes,
-                                                          uint64_t n_leaves, uint4 *__restrict__ out) {
-  uint64_t leaf = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void merkle_leaf_body(const uint4 *__restrict__ data, uint32_t leaf_bytes,
+                                                 uint64_t n_leaves, uint4 *__restrict__ out, uint64_t wg) {
+  uint64_t leaf = wg * blockDim.x + threadIdx.x;
   if (leaf >= n_leaves) return;
   const uint32_t leaf_u4 = leaf_bytes >> 4;
   const uint4 *p = data + leaf * leaf_u4;
@@ -229,14 +269,22 @@ __global__ __launch_bounds__(256) void merkle_leaf_kernel(const uint4 *__restric
   out[2 * leaf + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
+__global__ __launch_bounds__(256) void merkle_leaf_kernel(const uint4 *__restrict__ data, uint32_t leaf_bytes,
+                                                          uint64_t n_leaves, uint4 *__restrict__ out) {
+  merkle_leaf_body(data, leaf_bytes, n_leaves, out, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void merkle_leaf_batch_kernel(const MerkleJob *__restrict__ jobs, uint32_t n_jobs) {
+  const MerkleJob &J = job_of(jobs, n_jobs);
+  merkle_leaf_body((const uint4 *)J.data, J.leaf_bytes, J.n, (uint4 *)J.tree, blockIdx.x - J.wg_begin);
+}
+
 // Up to 9 parent levels per launch: workgroup w takes input nodes
 // [w * 2^L, (w + 1) * 2^L) of one level and writes the L levels above them
 // (every level is kept: multiproofs read them), staging CVs in LDS.
-__global__ __launch_bounds__(256) void merkle_subtree_kernel(uint4 *__restrict__ tree, uint64_t in_off,
-                                                             uint64_t n_in, int L) {
-  __shared__ uint32_t cvs[256][8];
+__device__ __forceinline__ void merkle_subtree_body(uint4 *__restrict__ tree, uint64_t in_off, uint64_t n_in, int L,
+                                                    uint64_t w, uint32_t (&cvs)[256][8]) {
   const uint32_t t = threadIdx.x;
-  const uint64_t w = blockIdx.x;
   uint64_t off = in_off, n = n_in;  // current level's offset and node count
   uint32_t width = 1u << L;         // nodes of the current level in this workgroup
   for (int lev = 0; lev < L; ++lev) {
@@ -277,6 +325,28 @@ __global__ __launch_bounds__(256) void merkle_subtree_kernel(uint4 *__restrict__
     n >>= 1;
     width = outs;
   }
+}
+
+__global__ __launch_bounds__(256) void merkle_subtree_kernel(uint4 *__restrict__ tree, uint64_t in_off,
+                                                             uint64_t n_in, int L) {
+  __shared__ uint32_t cvs[256][8];
+  merkle_subtree_body(tree, in_off, n_in, L, blockIdx.x, cvs);
+}
+
+__global__ __launch_bounds__(256) void merkle_subtree_batch_kernel(const MerkleJob *__restrict__ jobs,
+                                                                   uint32_t n_jobs) {
+  __shared__ uint32_t cvs[256][8];
+  const MerkleJob &J = job_of(jobs, n_jobs);
+  merkle_subtree_body((uint4 *)J.tree, J.in_off, J.n, J.levels, blockIdx.x - J.wg_begin, cvs);
+}
+
+// roots[i] = the last node of job i's tree (n = its node count).
+__global__ void gather_roots_kernel(const MerkleJob *__restrict__ jobs, uint32_t n_jobs, uint4 *__restrict__ roots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_jobs) return;
+  const uint4 *t = (const uint4 *)jobs[i].tree + 2 * (jobs[i].n - 1);
+  roots[2 * i] = t[0];
+  roots[2 * i + 1] = t[1];
 }
 
 int replicate_device(Ctx *ctx, hipStream_t st, const void *msg, uint64_t msg_len_f128, void *out,
@@ -376,6 +446,61 @@ int merkle_tree_device(Ctx *ctx, hipStream_t st, const void *d_data, uint32_t le
       rl >>= 1;
     }
   }
+  return FLOCK_OK;
+}
+
+NttVariant ntt_pass_variant(int k, int l, int F, uint64_t *wgs) {
+  const uint64_t groups = 1ull << (k - F);
+  if (ntt_pass_is_shared(k, l, F)) {
+    const bool g2 = k - l - F >= 4;
+    if (wgs) *wgs = groups / (g2 ? 16 : 8);
+    return g2 ? NTT_SHARED_G2 : NTT_SHARED_G1;
+  }
+  if (wgs) *wgs = groups;
+  return (NttVariant)(NTT_TAIL1 + (F - 1));
+}
+
+int ntt_batch_device(Ctx *ctx, hipStream_t st, NttVariant v, const NttJob *d_jobs, uint32_t n_jobs, uint64_t wgs) {
+  const dim3 grid((unsigned)wgs);
+  switch (v) {
+    case NTT_SHARED_G2: {
+      ScopedPhase ph(ctx, "ntt_pass_shared_kernel", st);
+      hipLaunchKernelGGL((ntt_pass_shared_batch_kernel<4, 8, 2>), grid, dim3(512), 0, st, d_jobs, n_jobs);
+      break;
+    }
+    case NTT_SHARED_G1: {
+      ScopedPhase ph(ctx, "ntt_pass_shared_kernel", st);
+      hipLaunchKernelGGL((ntt_pass_shared_batch_kernel<4, 8, 1>), grid, dim3(512), 0, st, d_jobs, n_jobs);
+      break;
+    }
+    case NTT_TAIL1: hipLaunchKernelGGL(ntt_tail_batch_kernel<1>, grid, dim3(64), 0, st, d_jobs, n_jobs); break;
+    case NTT_TAIL2: hipLaunchKernelGGL(ntt_tail_batch_kernel<2>, grid, dim3(64), 0, st, d_jobs, n_jobs); break;
+    case NTT_TAIL3: hipLaunchKernelGGL(ntt_tail_batch_kernel<3>, grid, dim3(64), 0, st, d_jobs, n_jobs); break;
+    case NTT_TAIL4: hipLaunchKernelGGL(ntt_tail_batch_kernel<4>, grid, dim3(64), 0, st, d_jobs, n_jobs); break;
+    default: ctx->last_error = "ntt batch: bad variant"; return FLOCK_E_ARG;
+  }
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+int merkle_leaf_batch_device(Ctx *ctx, hipStream_t st, const MerkleJob *d_jobs, uint32_t n_jobs, uint64_t wgs) {
+  ScopedPhase ph(ctx, "merkle_leaf_kernel", st);
+  hipLaunchKernelGGL(merkle_leaf_batch_kernel, dim3((unsigned)wgs), dim3(256), 0, st, d_jobs, n_jobs);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+int merkle_subtree_batch_device(Ctx *ctx, hipStream_t st, const MerkleJob *d_jobs, uint32_t n_jobs, uint64_t wgs) {
+  ScopedPhase ph(ctx, "merkle_parents", st);
+  hipLaunchKernelGGL(merkle_subtree_batch_kernel, dim3((unsigned)wgs), dim3(256), 0, st, d_jobs, n_jobs);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+int gather_roots_device(Ctx *ctx, hipStream_t st, const MerkleJob *d_jobs, uint32_t n_jobs, void *d_roots) {
+  hipLaunchKernelGGL(gather_roots_kernel, dim3((n_jobs + 63) / 64), dim3(64), 0, st, d_jobs, n_jobs,
+                     (uint4 *)d_roots);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
 

@@ -18,6 +18,8 @@
 
 namespace flock {
 
+struct BatchEngine;
+
 struct DeviceBuffer {
   void *ptr = nullptr;
   size_t bytes = 0;
@@ -113,6 +115,8 @@ struct Ctx {
   const F128 *device_twiddles(int dim);
   // Host threads for the prover's host-side loops, started on first use.
   std::unique_ptr<HostPool> pool;
+  // Batched proving state (worker contexts, captured graphs), on first use.
+  std::shared_ptr<BatchEngine> batch;
   HostPool &host_pool();
   ~Ctx();
 };

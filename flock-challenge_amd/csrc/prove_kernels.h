@@ -10,6 +10,18 @@ namespace flock {
 
 int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_real, int n_log, void *d_z,
                    void *d_a, void *d_b);
+// One instance of a fused witness launch: its compressions, their count and
+// the padded count 2^n_log, its packed z, a, b, and its first workgroup
+// (64 compressions per workgroup) in the launch.
+struct WitnessJob {
+  const flock_compression *blocks;
+  uint64_t n_real, n_total;
+  uint64_t *z, *a, *b;
+  uint64_t wg_begin;
+};
+// Every job's witness in one launch of total_wgs workgroups (d_jobs in
+// device memory, sorted by wg_begin).
+int witness_batch_device(Ctx *ctx, hipStream_t st, const WitnessJob *d_jobs, uint32_t n_jobs, uint64_t total_wgs);
 
 }  // namespace flock
 

@@ -423,24 +423,19 @@ int commitment_root_device(Ctx *ctx, hipStream_t st, const flock_compression *d_
   return ctx->wait_stream(st);
 }
 
-int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
-                        const uint8_t *domain, size_t domain_len, std::vector<uint8_t> &out) {
-  if (n_blocks == 0) {
-    ctx->last_error = "prove: need at least one compression";
-    return FLOCK_E_SHAPE;
-  }
-  int n_log = 0;
-  while ((1ull << n_log) < std::max<uint64_t>(n_blocks, 8)) ++n_log;
-  const int m = r1cs::K_LOG + n_log;
-  const flock_lig_config *cfg = nullptr;
+namespace {
+
+const flock_lig_config *lig_config(Ctx *ctx, int m) {
   for (int i = 0; i < FLOCK_LIG_N_CONFIGS; ++i)
-    if (FLOCK_LIG_CONFIGS[i].m == m && FLOCK_LIG_CONFIGS[i].profile == 0) cfg = &FLOCK_LIG_CONFIGS[i];
-  if (!cfg) {
-    ctx->last_error = "prove: no Ligerito configuration for m = " + std::to_string(m) + " (supported: 22..35)";
-    return FLOCK_E_SHAPE;
-  }
-  Prover P{ctx, st, m, m - 7, cfg, {}};
-  // The stream is idle between proves, so the upload ring can be recycled.
+    if (FLOCK_LIG_CONFIGS[i].m == m && FLOCK_LIG_CONFIGS[i].profile == 0) return &FLOCK_LIG_CONFIGS[i];
+  ctx->last_error = "prove: no Ligerito configuration for m = " + std::to_string(m) + " (supported: 22..35)";
+  return nullptr;
+}
+
+// Host staging areas of the context (allocated once) and the per-proof
+// resets; the stream is idle between proves, so the upload ring can be
+// recycled.
+int prepare_ctx(Ctx *ctx, hipStream_t st) {
   if (int e = ctx->wait_stream(st)) return e;
   ctx->pinned_reset();
   if (!ctx->landing) {
@@ -469,32 +464,88 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     ctx->readback = (uint8_t *)p;
     ctx->readback_dev = dp;
   }
+  if (!ctx->landing) {
+    ctx->last_error = "prove: landing area unavailable";
+    return FLOCK_E_DEVICE;
+  }
+  return FLOCK_OK;
+}
+
+// Everything after the commitment, over its buffers: zerocheck round 1
+// onwards.  The root is read from root_src once root_ready (may be null:
+// already there) has completed, so round 1's row products, queued first,
+// overlap the wait.  ph_all (may be null) is ended before the final wait.
+int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, const ProofFront &fb,
+               hipEvent_t root_ready, const uint8_t *root_src, const uint8_t *domain, size_t domain_len,
+               std::vector<uint8_t> &out, ScopedPhase *ph_all);
+
+}  // namespace
+
+int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_blocks,
+                        const uint8_t *domain, size_t domain_len, std::vector<uint8_t> &out) {
+  if (n_blocks == 0) {
+    ctx->last_error = "prove: need at least one compression";
+    return FLOCK_E_SHAPE;
+  }
+  int n_log = 0;
+  while ((1ull << n_log) < std::max<uint64_t>(n_blocks, 8)) ++n_log;
+  const int m = r1cs::K_LOG + n_log;
+  const flock_lig_config *cfg = lig_config(ctx, m);
+  if (!cfg) return FLOCK_E_SHAPE;
+  if (int e = prepare_ctx(ctx, st)) return e;
   ScopedPhase ph_all(ctx, "prove", st);
   HostPhase hp_all(ctx, "prove_wall");
   int rc = 0;
-  const int L = m - 7, n = m - 6;
+  const int L = m - 7;
   const size_t words = (size_t)1 << (m - 6);  // uint64 words per bit-vector
 
   // ---- witness + commit
-  void *z = P.dev("w.z", words * 8), *a = P.dev("w.a", words * 8), *b = P.dev("w.b", words * 8);
-  if (!z || !a || !b) return FLOCK_E_OOM;
-  if ((rc = witness_device(ctx, st, d_blocks, n_blocks, n_log, z, a, b))) return rc;
+  ProofFront fb;
+  fb.z = ctx->get_scratch("w.z", words * 8);
+  fb.a = ctx->get_scratch("w.a", words * 8);
+  fb.b = ctx->get_scratch("w.b", words * 8);
+  if (!fb.z || !fb.a || !fb.b) return FLOCK_E_OOM;
+  if ((rc = witness_device(ctx, st, d_blocks, n_blocks, n_log, fb.z, fb.a, fb.b))) return rc;
+  flock_pcs_params pp{(uint32_t)m, 1, 6, 0, 1};
+  const size_t npos0 = (size_t)1 << (L - 6 + 1);
+  fb.codeword = ctx->get_scratch("c.codeword", npos0 * 64 * 16);
+  fb.tree = ctx->get_scratch("c.tree", (2 * npos0 - 1) * 32);
+  if (!fb.codeword || !fb.tree) return FLOCK_E_OOM;
+  // The root comes back through the landing area under an event.
+  hipEvent_t root_ready = ctx->take_event();
+  if (!root_ready) {
+    ctx->last_error = "prove: no event for the root";
+    return FLOCK_E_DEVICE;
+  }
+  if ((rc = commit_device(ctx, &pp, fb.z, fb.codeword, fb.tree, ctx->landing, st, root_ready))) return rc;
+  return prove_back(ctx, st, m, cfg, fb, root_ready, ctx->landing, domain, domain_len, out, &ph_all);
+}
+
+int prove_after_commit_device(Ctx *ctx, hipStream_t st, int m, const ProofFront &fb, const uint8_t root[32],
+                              const uint8_t *domain, size_t domain_len, std::vector<uint8_t> &out) {
+  const flock_lig_config *cfg = lig_config(ctx, m);
+  if (!cfg) return FLOCK_E_SHAPE;
+  if (int e = prepare_ctx(ctx, st)) return e;
+  ScopedPhase ph_all(ctx, "prove", st);
+  return prove_back(ctx, st, m, cfg, fb, nullptr, root, domain, domain_len, out, &ph_all);
+}
+
+namespace {
+
+int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, const ProofFront &fb,
+               hipEvent_t root_ready, const uint8_t *root_src, const uint8_t *domain, size_t domain_len,
+               std::vector<uint8_t> &out, ScopedPhase *ph_all) {
+  Prover P{ctx, st, m, m - 7, cfg, {}};
+  int rc = 0;
+  const int L = m - 7, n = m - 6;
+  const size_t words = (size_t)1 << (m - 6);  // uint64 words per bit-vector
+  void *z = fb.z, *a = fb.a, *b = fb.b, *cw0 = fb.codeword, *tree0 = fb.tree;
   flock_pcs_params pp{(uint32_t)m, 1, 6, 0, 1};
   const int k0 = L - 6 + 1;
   const size_t npos0 = (size_t)1 << k0;
-  void *cw0 = P.dev("c.codeword", npos0 * 64 * 16), *tree0 = P.dev("c.tree", (2 * npos0 - 1) * 32);
-  if (!cw0 || !tree0) return FLOCK_E_OOM;
-  // The root comes back through the landing area under an event; zerocheck
-  // round 1's products (which depend on the rows only) are queued behind it,
-  // so the device works while the host builds the transcript and derives the
-  // challenges.
-  uint8_t root[32];
-  hipEvent_t root_ready = ctx->take_event();
-  if (!ctx->landing || !root_ready) {
-    ctx->last_error = "prove: landing area or event unavailable";
-    return FLOCK_E_DEVICE;
-  }
-  if ((rc = commit_device(ctx, &pp, z, cw0, tree0, ctx->landing, st, root_ready))) return rc;
+  // zerocheck round 1's products depend on the rows only: queued before the
+  // wait for the root, so the device works while the host builds the
+  // transcript and derives the challenges
   void *dzt = P.dev("zc.tables", sizeof(ZcTables));
   if (!dzt) return FLOCK_E_OOM;
   if (!ctx->zc_tables_uploaded) {
@@ -502,12 +553,13 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     ctx->zc_tables_uploaded = true;
   }
   if ((rc = zc_y_device(ctx, st, a, b, m, dzt))) return rc;
-  {
+  uint8_t root[32];
+  if (root_ready) {
     HostPhase hp(ctx, "wait_d2h");
     if (int e = ctx->wait_event(root_ready)) return e;
     ctx->event_pool.push_back(root_ready);
   }
-  memcpy(root, ctx->landing, 32);
+  memcpy(root, root_src, 32);
 
   Transcript t(domain, domain_len);
   {
@@ -1146,10 +1198,12 @@ int prove_blake3_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blo
     B.u64(fold_nonces.size());
     B.put(fold_nonces.data(), fold_nonces.size() * 8);
   }
-  ph_all.end();
+  if (ph_all) ph_all->end();
   if ((rc = ctx->wait_stream(st))) return rc;
   out.swap(B.v);
   return FLOCK_OK;
 }
+
+}  // namespace
 
 }  // namespace flock

@@ -100,12 +100,13 @@ constexpr uint8_t G_LANES_D[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13}, {2, 6, 10, 14
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void witness_kernel(const flock_compression *__restrict__ blocks, uint64_t n_real,
-                                                     uint64_t n_total, uint64_t *__restrict__ z,
-                                                     uint64_t *__restrict__ a, uint64_t *__restrict__ b) {
-  __shared__ uint64_t ring[3][RING * RING_STRIDE];
+// The 64 compressions from wg * 64 of one instance, one per lane.
+__device__ __forceinline__ void witness_body(const flock_compression *__restrict__ blocks, uint64_t n_real,
+                                             uint64_t n_total, uint64_t *__restrict__ z, uint64_t *__restrict__ a,
+                                             uint64_t *__restrict__ b, uint64_t wg,
+                                             uint64_t (&ring)[3][RING * RING_STRIDE]) {
   const uint32_t lane = threadIdx.x;
-  const uint64_t blk0 = (uint64_t)blockIdx.x * 64;
+  const uint64_t blk0 = wg * 64;
   const uint64_t blk = blk0 + lane;
   uint32_t cv[8], m[16], tlo = 0, thi = 0, blen = 0, flags = 0;
   if (blk < n_real) {
@@ -188,6 +189,31 @@ __global__ __launch_bounds__(64) void witness_kernel(const flock_compression *__
 #pragma unroll
   for (int k = 0; k < 256; ++k)
     if (k >= (int)w.word) w.emit(0, 0, 0);
+}
+
+__global__ __launch_bounds__(64) void witness_kernel(const flock_compression *__restrict__ blocks, uint64_t n_real,
+                                                     uint64_t n_total, uint64_t *__restrict__ z,
+                                                     uint64_t *__restrict__ a, uint64_t *__restrict__ b) {
+  __shared__ uint64_t ring[3][RING * RING_STRIDE];
+  witness_body(blocks, n_real, n_total, z, a, b, blockIdx.x, ring);
+}
+
+// Many instances in one launch: workgroup w belongs to the last job whose
+// first workgroup is at most w (jobs sorted by wg_begin).
+__global__ __launch_bounds__(64) void witness_batch_kernel(const WitnessJob *__restrict__ jobs, uint32_t n_jobs) {
+  __shared__ uint64_t ring[3][RING * RING_STRIDE];
+  uint32_t j = 0;
+  for (uint32_t i = 1; i < n_jobs; ++i)
+    if (jobs[i].wg_begin <= blockIdx.x) j = i;
+  const WitnessJob J = jobs[j];
+  witness_body(J.blocks, J.n_real, J.n_total, J.z, J.a, J.b, blockIdx.x - J.wg_begin, ring);
+}
+
+int witness_batch_device(Ctx *ctx, hipStream_t st, const WitnessJob *d_jobs, uint32_t n_jobs, uint64_t total_wgs) {
+  ScopedPhase ph(ctx, "witness", st);
+  hipLaunchKernelGGL(witness_batch_kernel, dim3((unsigned)total_wgs), dim3(64), 0, st, d_jobs, n_jobs);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
 }
 
 int witness_device(Ctx *ctx, hipStream_t st, const flock_compression *d_blocks, uint64_t n_real, int n_log, void *d_z,

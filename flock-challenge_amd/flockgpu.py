@@ -127,6 +127,7 @@ def lib():
         L.flock_server_pending.argtypes = [P, PU64]
         L.flock_ctx_set_wait_timeout.argtypes = [P, ctypes.c_double]
         L.flock_statement_digest.argtypes = [U32, P]
+        L.flock_prove_batch.argtypes = [P, P, U32, P, SZ, U32, ctypes.c_int, P, SZ, P, P, P, P]
         L.flock_statement_setup.argtypes = [P, ctypes.c_char_p, SZ]
         L.flock_server_set_wait_timeout.argtypes = [P, ctypes.c_double]
         _lib = L
@@ -145,7 +146,7 @@ def exported_symbols():
             "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
             "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending",
             "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout", "flock_commit_device_noalloc",
-            "flock_statement_digest", "flock_statement_setup"]
+            "flock_statement_digest", "flock_statement_setup", "flock_prove_batch"]
 
 
 def device_count():
@@ -378,6 +379,39 @@ def verify(proof, n_blocks, domain=BENCH_DOMAIN, expected_root=None):
         raise VerifyError(rc, err.value.decode(errors="replace"))
     if rc:
         raise FlockError(rc, err.value.decode(errors="replace"))
+
+
+class BatchItem(ctypes.Structure):
+    """flock_batch_item: device compressions, or generate_compressions(log2,
+    seed) when d_blocks is 0."""
+    _fields_ = [("d_blocks", ctypes.c_void_p), ("n_blocks", ctypes.c_uint64), ("log2", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64)]
+
+
+def prove_batch(ctx, items, n_workers=3, use_graph=True, domain=None):
+    """flock_prove_batch over `items`, each ("seeded", log2, seed) or
+    ("device", device_ptr, n_blocks).  Returns (proofs, done_s, front_s,
+    replayed)."""
+    d = bytes(domain if domain is not None else BENCH_DOMAIN)
+    arr = (BatchItem * len(items))()
+    for i, it in enumerate(items):
+        if it[0] == "seeded":
+            arr[i].d_blocks, arr[i].n_blocks, arr[i].log2, arr[i].seed = None, 0, it[1], it[2]
+        elif it[0] == "device":
+            arr[i].d_blocks, arr[i].n_blocks, arr[i].log2, arr[i].seed = it[1], it[2], 0, 0
+        else:
+            raise ValueError(f"batch item kind {it[0]!r}")
+    n = len(items)
+    offs = (ctypes.c_uint64 * (n + 1))()
+    done = (ctypes.c_double * max(1, n))()
+    front = ctypes.c_double()
+    rep = ctypes.c_int()
+    buf = np.empty(max(1, n) << 20, np.uint8)  # proofs are below 500 kB
+    rc = lib().flock_prove_batch(ctx.handle, arr, n, d, len(d), n_workers, 1 if use_graph else 0, buf.ctypes.data,
+                                 buf.nbytes, offs, done, ctypes.byref(front), ctypes.byref(rep))
+    ctx._check(rc)
+    proofs = [buf[offs[i]:offs[i + 1]].tobytes() for i in range(n)]
+    return proofs, [done[i] for i in range(n)], front.value, bool(rep.value)
 
 
 def statement_digest(m):

@@ -210,6 +210,36 @@ int flock_commitment_root_seeded(flock_setup *setup, uint32_t log2, uint64_t see
  * here for fingerprinting proof bytes. */
 void flock_blake3(const void *data, size_t len, uint8_t out[32]);
 
+/* ------------------------------------------------------------------ batching */
+
+/* One proof of a batch: d_blocks (device, n_blocks compressions), or, when
+ * d_blocks is NULL, generate_compressions(log2, seed) expanded on the
+ * device. */
+typedef struct {
+  const flock_compression *d_blocks;
+  uint64_t n_blocks;
+  uint32_t log2;
+  uint64_t seed;
+} flock_batch_item;
+
+/* Many independent proofs of uneven sizes (256 .. 2^21 compressions each).
+ * No counterpart in the reference (its worker proves one seed per process,
+ * worker/src/main.rs:36-45); each proof is the same bundle bytes as
+ * flock_prove_blake3 for the same input.  The fronts (witness, codeword NTT,
+ * Merkle tree) of all items run fused: one launch per step for every item,
+ * the launch sequence captured in a HIP graph per batch shape (the list of
+ * sizes) and replayed when the same shape comes again, if use_graph.  The
+ * rest of each proof runs on n_workers (1..16) worker streams, largest first.
+ * out receives the proofs back to back: proof i is out[offsets[i] ..
+ * offsets[i + 1]) (offsets has n_items + 1 entries); if cap is too small,
+ * FLOCK_E_CAP with offsets filled and nothing copied.  done_s (n_items, may
+ * be NULL) receives each proof's completion time from the call's start;
+ * front_s (may be NULL) the fused fronts' time; replayed (may be NULL) 1
+ * when the fronts replayed a captured graph. */
+int flock_prove_batch(flock_ctx *ctx, const flock_batch_item *items, uint32_t n_items, const uint8_t *domain,
+                      size_t domain_len, uint32_t n_workers, int use_graph, uint8_t *out, size_t cap,
+                      uint64_t *offsets, double *done_s, double *front_s, int *replayed);
+
 /* ------------------------------------------------------------------ serving */
 
 /* A proof server on GPU `device`: continuous batching of independent proofs.

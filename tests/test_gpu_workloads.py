@@ -206,3 +206,50 @@ def test_every_size_matches_fresh_context_and_verifies(solo_ctx, log2):
         want = O.prove(O.generate_compressions(log2, seed))
         diff = PF.first_difference(got, want)
         assert diff is None, f"2^{log2}: differs from the oracle's proof: {diff}"
+
+
+def _batch_items(ctx, salt, keep):
+    """The batched workload as flock_prove_batch items (64 seeded + one
+    ragged device item of 3000 compressions); the device buffer goes into
+    `keep` so it outlives the call."""
+    g = splitmix64(BATCH_SEED)
+    items, reqs = [], []
+    for _ in range(64):
+        log2 = 10 + next(g) % 9
+        seed = next(g) ^ salt
+        items.append(("seeded", log2, seed))
+        reqs.append((0.0, "seeded", (log2, seed)))
+    ragged = O.generate_compressions(12, 0xBA7C0000000000FF ^ salt)[:3000].copy()
+    d = ctx.upload(ragged)
+    keep.append(d)
+    items.append(("device", d.ptr, 3000))
+    reqs.append((0.0, "blocks", ragged))
+    return items, reqs
+
+
+def test_prove_batch_fused_fronts_match_solo_and_replay_the_graph(solo_ctx):
+    ctx = flockgpu.Context(0)
+    keep = []
+    try:
+        items, reqs = _batch_items(ctx, 0, keep)
+        _say(f"prove_batch: {len(items)} items, fronts fused, graph captured")
+        proofs, done, front_s, replayed = flockgpu.prove_batch(ctx, items, n_workers=3)
+        assert not replayed and len(proofs) == len(items)
+        _say(f"prove_batch: fronts {1000 * front_s:.2f} ms, last proof at {1000 * max(done):.1f} ms")
+        n_oracle = _check_all(solo_ctx, reqs, dict(enumerate(proofs)))
+        assert n_oracle >= 10
+        # the same shape with other inputs replays the captured graph
+        items2, reqs2 = _batch_items(ctx, 0x5A17, keep)
+        assert [it[1] for it in items2[:64]] == [it[1] for it in items[:64]]
+        _say("prove_batch: same shape, new seeds, graph replayed")
+        proofs2, _, _, replayed2 = flockgpu.prove_batch(ctx, items2, n_workers=3)
+        assert replayed2
+        _check_all(solo_ctx, reqs2, dict(enumerate(proofs2)), oracle_max_log2=0)
+        # and without the graph, on one worker, the same bytes
+        _say("prove_batch: no graph, one worker")
+        proofs3, _, _, replayed3 = flockgpu.prove_batch(ctx, items2[:5] + items2[-1:], n_workers=1, use_graph=False)
+        assert not replayed3 and proofs3 == proofs2[:5] + proofs2[-1:]
+    finally:
+        for d in keep:
+            d.free()
+        ctx.close()
