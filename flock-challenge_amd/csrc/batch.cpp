@@ -1,10 +1,11 @@
-// Batched proving: many independent proofs of uneven sizes, their fronts
-// (witness and commitment) fused across instances.
+// Batched proving: many independent proofs of uneven sizes, the fronts
+// (witness and commitment) of the small ones fused across instances.
 //
 // The front is the part of a proof that needs no challenge: the witness of
 // every compression, the interleaved NTT of the rate-1/2 codeword and its
-// Merkle tree.  For a batch it runs as one launch per step for all
-// instances at once: one witness launch, one launch per NTT pass stage and
+// Merkle tree.  For small instances (at most 2^FUSE_MAX_LOG2 compressions,
+// whose launches are short enough for launch overhead to matter) it runs
+// as one launch per step for all of them at once: one witness launch, one launch per NTT pass stage and
 // pass variant (pass p of every instance that has one; instances of
 // different sizes have different pass lists), one Merkle-leaf launch, one
 // launch per Merkle subtree stage, one root gather.  Each launch reads a job
@@ -14,10 +15,12 @@
 // is captured once per shape in a HIP graph and replayed: later batches of
 // the same shape rewrite the job tables and launch the graph.
 //
-// The rest of each proof (zerocheck onwards, which waits on Fiat-Shamir
-// challenges) runs on worker threads, each with its own context and stream,
-// largest instances first; the proofs overlap on the device as in the proof
-// server.  Every proof is the same bytes as a solo proof of its input.
+// Larger instances run as whole proofs on worker threads, each with its own
+// context and stream, largest first, while the fused fronts run; then the
+// workers take the rest of the small instances' proofs (zerocheck onwards,
+// which waits on Fiat-Shamir challenges).  The proofs overlap on the device
+// as in the proof server.  Every proof is the same bytes as a solo proof of
+// its input.
 #include "batch.h"
 
 #include <string.h>
@@ -25,6 +28,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -47,13 +52,7 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct Inst {
-  int n_log = 0, m = 0, k = 0;
-  uint64_t n_real = 0;
-  const flock_compression *blocks = nullptr;
-  ProofFront fb;
-  uint8_t root[32];
-};
+using Inst = FrontJob;
 
 // The launch plan of a batch: job tables (host copies, concatenated into one
 // upload) and the launches that read them.
@@ -81,31 +80,19 @@ struct Plan {
 
 }  // namespace
 
-struct BatchEngine {
-  Ctx *ctx;  // the front's context: arena, job area, stream
-  int n_workers = 0;
-  std::vector<std::unique_ptr<Ctx>> workers;
+struct FrontGraphs {
   std::map<std::vector<uint64_t>, hipGraphExec_t> graphs;
-  std::mutex mu;
-
-  explicit BatchEngine(Ctx *c) : ctx(c) {}
-  ~BatchEngine() {
+  ~FrontGraphs() {
     for (auto &kv : graphs) hipGraphExecDestroy(kv.second);
   }
+};
 
-  int ensure_workers(int n) {
-    while ((int)workers.size() < n) {
-      std::unique_ptr<Ctx> w(new Ctx());
-      w->device = ctx->device;
-      w->wait_timeout_s = ctx->wait_timeout_s;
-      if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
-        ctx->last_error = "batch: cannot create a worker stream";
-        return FLOCK_E_DEVICE;
-      }
-      workers.push_back(std::move(w));
-    }
-    return FLOCK_OK;
-  }
+namespace {
+
+// The fused fronts on one context: its stream and a fixed job area.
+struct FrontRunner {
+  Ctx *ctx;
+  FrontGraphs *cache;  // null: launch directly, no graph
 
   // The plan for the instances (whose buffers are already assigned).
   Plan plan(std::vector<Inst> &in) {
@@ -223,11 +210,11 @@ struct BatchEngine {
     if (!d_jobs) return FLOCK_E_OOM;
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(d_jobs, p.jobs.data(), p.roots_offset, hipMemcpyHostToDevice, st));
     *replayed = false;
-    if (!use_graph) {
+    if (!use_graph || !cache) {
       if (int rc = issue(p, d_jobs, st)) return rc;
     } else {
-      auto it = graphs.find(p.shape);
-      if (it == graphs.end()) {
+      auto it = cache->graphs.find(p.shape);
+      if (it == cache->graphs.end()) {
         // capture the launch sequence of this shape once (phase timing off:
         // its events are not part of the graph)
         const bool timing = ctx->timing;
@@ -246,7 +233,7 @@ struct BatchEngine {
         const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         FLOCK_HIP_CHECK(ctx, ie);
-        it = graphs.emplace(p.shape, ex).first;
+        it = cache->graphs.emplace(p.shape, ex).first;
       } else {
         *replayed = true;
       }
@@ -256,6 +243,47 @@ struct BatchEngine {
     FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(roots.data(), d_jobs + p.roots_offset, roots.size(), hipMemcpyDeviceToHost, st));
     if (int rc = ctx->wait_stream(st)) return rc;
     for (size_t i = 0; i < in.size(); ++i) memcpy(in[i].root, roots.data() + 32 * i, 32);
+    return FLOCK_OK;
+  }
+};
+
+}  // namespace
+
+int run_fronts(Ctx *ctx, std::vector<FrontJob> &jobs, FrontGraphs *graphs, bool *replayed) {
+  for (auto &j : jobs) {
+    j.m = r1cs::K_LOG + j.n_log;
+    j.k = j.m - 12;
+    if (j.m < 22 || j.n_real == 0 || j.n_real > (1ull << j.n_log)) {
+      ctx->last_error = "fronts: a job needs 256 .. 2^21 compressions";
+      return FLOCK_E_SHAPE;
+    }
+  }
+  bool rep = false;
+  FrontRunner r{ctx, graphs};
+  const int rc = r.front(jobs, graphs != nullptr, &rep);
+  if (replayed) *replayed = rep;
+  return rc;
+}
+
+struct BatchEngine {
+  Ctx *ctx;  // the fronts' context: arena, job area, stream
+  std::vector<std::unique_ptr<Ctx>> workers;
+  FrontGraphs graphs;
+  std::mutex mu;
+
+  explicit BatchEngine(Ctx *c) : ctx(c) {}
+
+  int ensure_workers(int n) {
+    while ((int)workers.size() < n) {
+      std::unique_ptr<Ctx> w(new Ctx());
+      w->device = ctx->device;
+      w->wait_timeout_s = ctx->wait_timeout_s;
+      if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->last_error = "batch: cannot create a worker stream";
+        return FLOCK_E_DEVICE;
+      }
+      workers.push_back(std::move(w));
+    }
     return FLOCK_OK;
   }
 };
@@ -279,11 +307,11 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
   if (int rc = E->ensure_workers(n_workers)) return rc;
   hipStream_t st = ctx->stream;
   if (int rc = ctx->wait_stream(st)) return rc;
-  // instance shapes and their buffers in one arena: seeded inputs, then per
-  // instance z, a, b, codeword and tree
+  // shapes; small items get their front buffers in one arena (seeded inputs,
+  // then z, a, b, codeword and tree)
   std::vector<Inst> in(items.size());
+  std::vector<size_t> small, large;
   size_t total = 0;
-  std::vector<size_t> blocks_at(items.size(), 0);
   for (size_t i = 0; i < items.size(); ++i) {
     const BatchItem &it = items[i];
     const uint64_t n = it.blocks ? it.n_blocks : (1ull << it.log2);
@@ -298,69 +326,130 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
     x.m = r1cs::K_LOG + n_log;
     x.k = x.m - 12;
     x.n_real = n;
+    x.blocks = it.blocks;
     if (x.m < 22) {
       ctx->last_error = "batch: item " + std::to_string(i) + " is below 256 compressions (no Ligerito configuration)";
       return FLOCK_E_SHAPE;
     }
-    if (!it.blocks) {
-      blocks_at[i] = total;
-      total = align256(total + n * sizeof(flock_compression));
+    if (n_log > FUSE_MAX_LOG2) {
+      large.push_back(i);
+      continue;
     }
-    total = align256(total + 3 * ((size_t)8 << (x.m - 6)));     // z, a, b
-    total = align256(total + ((size_t)1024 << (x.m - 12)));     // codeword
-    total = align256(total + (((size_t)2 << (x.m - 12)) - 1) * 32);  // tree
+    small.push_back(i);
   }
-  uint8_t *arena = (uint8_t *)ctx->get_scratch("batch.arena", total);
-  if (!arena) return FLOCK_E_OOM;
-  size_t off = 0;
-  for (size_t i = 0; i < items.size(); ++i) {
-    Inst &x = in[i];
-    if (!items[i].blocks) {
-      x.blocks = (const flock_compression *)(arena + blocks_at[i]);
-      if (int rc = generate_compressions_device(ctx, st, items[i].log2, items[i].seed,
-                                                (flock_compression *)(arena + blocks_at[i])))
-        return rc;
-      off = align256(blocks_at[i] + x.n_real * sizeof(flock_compression));
-    } else {
-      x.blocks = items[i].blocks;
-    }
-    const size_t words = (size_t)1 << (x.m - 6);
-    x.fb.z = arena + off;
-    x.fb.a = arena + off + 8 * words;
-    x.fb.b = arena + off + 16 * words;
-    off = align256(off + 24 * words);
-    x.fb.codeword = arena + off;
-    off = align256(off + ((size_t)1024 << (x.m - 12)));
-    x.fb.tree = arena + off;
-    off = align256(off + (((size_t)2 << (x.m - 12)) - 1) * 32);
+  // The work list: large items first (whole proofs, largest first), then
+  // the small ones once their fused fronts are done.
+  std::stable_sort(large.begin(), large.end(), [&](size_t a, size_t b) { return in[a].n_log > in[b].n_log; });
+  std::stable_sort(small.begin(), small.end(), [&](size_t a, size_t b) { return in[a].n_log > in[b].n_log; });
+  // The small items' regions in the arena, in the order the fronts take
+  // them: [compressions when seeded] z a b codeword tree, each 256-aligned.
+  struct Region {
+    size_t blocks, z, cw, tree, end;
+  };
+  std::vector<Region> reg(items.size());
+  for (size_t i : small) {
+    const Inst &x = in[i];
+    Region &g = reg[i];
+    g.blocks = total;
+    if (!items[i].blocks) total = align256(total + x.n_real * sizeof(flock_compression));
+    g.z = total;
+    total = align256(total + 3 * ((size_t)8 << (x.m - 6)));  // z, a, b
+    g.cw = total;
+    total = align256(total + ((size_t)1024 << (x.m - 12)));
+    g.tree = total;
+    total = align256(total + (((size_t)2 << (x.m - 12)) - 1) * 32);
+    g.end = total;
   }
-  bool replayed = false;
-  const double tf = now_s();
-  if (int rc = E->front(in, use_graph, &replayed)) return rc;
-  const double front_s = now_s() - tf;
-  // the rest of every proof on the workers, largest first
-  std::vector<size_t> order(in.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return in[a].n_log > in[b].n_log; });
+  std::mutex qmu;
+  std::condition_variable qcv;
+  std::deque<size_t> work(large.begin(), large.end());
+  bool fronts_done = small.empty();
   proofs.assign(in.size(), {});
   done_s.assign(in.size(), 0.0);
   std::vector<int> rcs(in.size(), FLOCK_OK);
   std::vector<std::string> errs(in.size());
-  std::atomic<size_t> next{0};
   std::vector<std::thread> ts;
   for (int w = 0; w < n_workers; ++w)
     ts.emplace_back([&, w] {
       Ctx *wc = E->workers[w].get();
       hipSetDevice(wc->device);
-      for (size_t q; (q = next.fetch_add(1)) < order.size();) {
-        const size_t i = order[q];
-        rcs[i] = prove_after_commit_device(wc, wc->stream, in[i].m, in[i].fb, in[i].root, domain, domain_len,
-                                           proofs[i]);
+      for (;;) {
+        size_t i;
+        {
+          std::unique_lock<std::mutex> ql(qmu);
+          qcv.wait(ql, [&] { return !work.empty() || fronts_done; });
+          if (work.empty()) return;
+          i = work.front();
+          work.pop_front();
+        }
+        const Inst &x = in[i];
+        if (x.n_log > FUSE_MAX_LOG2) {
+          // a whole proof on the worker's stream
+          const flock_compression *blocks = x.blocks;
+          if (!blocks) {
+            void *d = wc->get_scratch("batch.blocks", x.n_real * sizeof(flock_compression));
+            rcs[i] = d ? generate_compressions_device(wc, wc->stream, items[i].log2, items[i].seed,
+                                                      (flock_compression *)d)
+                       : FLOCK_E_OOM;
+            blocks = (const flock_compression *)d;
+          }
+          if (!rcs[i]) rcs[i] = prove_blake3_device(wc, wc->stream, blocks, x.n_real, domain, domain_len, proofs[i]);
+        } else {
+          rcs[i] = prove_after_commit_device(wc, wc->stream, x.m, x.fb, x.root, domain, domain_len, proofs[i]);
+        }
         if (rcs[i]) errs[i] = wc->last_error;
         done_s[i] = now_s() - t0;
       }
     });
+  // meanwhile, the small items' fronts, fused (graph per shape)
+  bool replayed = false;
+  double front_s = 0;
+  int front_rc = FLOCK_OK;
+  if (!small.empty()) {
+    const double tf = now_s();
+    std::vector<Inst> fused;
+    uint8_t *arena = (uint8_t *)ctx->get_scratch("batch.arena", total);
+    if (!arena) front_rc = FLOCK_E_OOM;
+    size_t prev_end = 0;
+    for (size_t q = 0; q < small.size() && !front_rc; ++q) {
+      const size_t i = small[q];
+      const Region &g = reg[i];
+      // the regions follow each other inside the arena
+      if (g.blocks != prev_end || g.end > total) {
+        ctx->last_error = "batch: arena layout inconsistent";
+        front_rc = FLOCK_E_SHAPE;
+        break;
+      }
+      prev_end = g.end;
+      Inst x = in[i];
+      if (!items[i].blocks) {
+        x.blocks = (const flock_compression *)(arena + g.blocks);
+        front_rc = generate_compressions_device(ctx, st, items[i].log2, items[i].seed,
+                                                (flock_compression *)(arena + g.blocks));
+      }
+      const size_t words = (size_t)1 << (x.m - 6);
+      x.fb.z = arena + g.z;
+      x.fb.a = arena + g.z + 8 * words;
+      x.fb.b = arena + g.z + 16 * words;
+      x.fb.codeword = arena + g.cw;
+      x.fb.tree = arena + g.tree;
+      fused.push_back(x);
+    }
+    if (!front_rc) front_rc = run_fronts(ctx, fused, use_graph ? &E->graphs : nullptr, &replayed);
+    front_s = now_s() - tf;
+    {
+      std::lock_guard<std::mutex> ql(qmu);
+      if (!front_rc)
+        for (size_t q = 0; q < small.size(); ++q) {
+          in[small[q]] = fused[q];
+          work.push_back(small[q]);
+        }
+      fronts_done = true;
+    }
+    qcv.notify_all();
+  }
   for (auto &t : ts) t.join();
+  if (front_rc) return front_rc;
   for (size_t i = 0; i < in.size(); ++i)
     if (rcs[i]) {
       ctx->last_error = "batch: item " + std::to_string(i) + ": " + errs[i];
@@ -371,6 +460,7 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
   if (stats) {
     stats->front_s = front_s;
     stats->graph_replayed = replayed;
+    stats->fused = small.size();
   }
   return FLOCK_OK;
 }

@@ -126,6 +126,16 @@ extern "C" int flock_ctx_create(int device, flock_ctx **out) {
 
 extern "C" void flock_ctx_destroy(flock_ctx *ctx) { delete ctx; }
 
+extern "C" int flock_ctx_set_option(flock_ctx *ctx, const char *name, int64_t value) {
+  if (!ctx || !name) return FLOCK_E_ARG;
+  if (!strcmp(name, "device_transcript")) {
+    ctx->c.device_transcript = value != 0;
+    return FLOCK_OK;
+  }
+  ctx->c.last_error = std::string("unknown option ") + name;
+  return FLOCK_E_ARG;
+}
+
 extern "C" int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds) {
   if (!ctx || !(seconds == seconds)) return FLOCK_E_ARG;
   ctx->c.wait_timeout_s = seconds;

@@ -161,6 +161,14 @@ int Ctx::release_scratch() {
   return FLOCK_OK;
 }
 
+void Ctx::free_scratch(const std::string &name) {
+  auto it = scratch.find(name);
+  if (it == scratch.end()) return;
+  if (wait_stream(stream)) return;
+  if (it->second.ptr) hipFree(it->second.ptr);
+  scratch.erase(it);
+}
+
 hipEvent_t Ctx::take_event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();

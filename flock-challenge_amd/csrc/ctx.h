@@ -53,6 +53,9 @@ struct Ctx {
   // destroying it leaks its device memory instead of waiting on that work.
   double wait_timeout_s = 120.0;
   bool timed_out = false;
+  // Run the zerocheck rounds' Fiat-Shamir steps on the device (fs_dev.hip)
+  // instead of a host round trip per round; the proof bytes are the same.
+  bool device_transcript = true;
   // Bounded waits: until a sequence number appears at `flag` (published by
   // the device in the mapped readback area), until the stream has drained,
   // until the event has completed.  FLOCK_OK, or FLOCK_E_DEVICE with
@@ -111,6 +114,8 @@ struct Ctx {
   // Free every named scratch buffer (after the stream has drained); the
   // next prove allocates again.  FLOCK_OK or the wait's error.
   int release_scratch();
+  // Free one named scratch buffer (after the stream has drained), if any.
+  void free_scratch(const std::string &name);
   const std::vector<F128> &host_twiddles(int dim);
   const F128 *device_twiddles(int dim);
   // Host threads for the prover's host-side loops, started on first use.

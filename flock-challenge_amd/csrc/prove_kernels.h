@@ -67,7 +67,9 @@ int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *
 // d_lw8 of L and d_lwr8 of rho L.
 int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw8, const void *d_lwr8,
                     uint64_t half, void *A2, void *B2);
-int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2);
+// d_r (may be null): read the challenge from device memory instead of r.
+int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2,
+                     const void *d_r = nullptr);
 // A round fused with the fold before it: fold the previous round's tables
 // (4 half entries) by its challenge into A2, B2 (2 half), compute the new
 // round's message over them and publish it (seq) as xor_rows_device does.
@@ -78,7 +80,8 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
 // evaluation.
 constexpr uint64_t SMALL_PAIRS = 256;
 int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
-                         uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq);
+                         uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq,
+                         const void *d_rho = nullptr);
 int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half, void *f2,
                         void *b2, void *out3, uint32_t seq);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,

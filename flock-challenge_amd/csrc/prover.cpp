@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "commit.h"
+#include "fs_dev.h"
 #include "host_algebra.h"
 #include "ligerito_configs.h"
 #include "prove_kernels.h"
@@ -23,6 +24,8 @@
 #include "zc_tables.h"
 
 namespace flock {
+
+static_assert(Transcript::STATE_BYTES == FS_STATE_BYTES, "host and device transcript states differ");
 
 namespace {
 
@@ -730,61 +733,118 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
     // the first whole-table round (n_split + 1) starts from eq itself; it
     // depends on the point only, so it is queued now
     if (n_split + 1 < n && (rc = P.eq_device(&r[8 + n_split], n - 2 - n_split, E))) return rc;
+    // With the device transcript, the rounds' messages go to device memory,
+    // fs_round_kernel absorbs each and squeezes the round's challenge into
+    // device memory, and the folds read it from there: the rounds run with
+    // no host round trip, and the host reads everything back once.
+    const bool dfs = ctx->device_transcript;
+    uint8_t *d_fs = nullptr, *d_msgs = nullptr, *d_rhos = nullptr, *d_ws = nullptr;
+    if (dfs) {
+      d_fs = (uint8_t *)P.dev("fs.state", FS_STATE_BYTES);
+      d_msgs = (uint8_t *)P.dev("zc.msgs", 32 * (size_t)(n + 1));
+      d_rhos = (uint8_t *)P.dev("zc.rhos", 16 * (size_t)n);
+      d_ws = (uint8_t *)P.dev("zc.ws", 64 * 16);
+      if (!d_fs || !d_msgs || !d_rhos || !d_ws) return FLOCK_E_OOM;
+      uint8_t st_bytes[FS_STATE_BYTES];
+      t.export_state(st_bytes);
+      if ((rc = P.h2d(d_fs, st_bytes, FS_STATE_BYTES)) || (rc = P.h2d(d_ws, ws, sizeof ws))) return rc;
+    }
     size_t len = rows;
     // A whole-table round folds the previous round's tables itself (one
     // launch fewer per round): then the fold by prev_rho is pending when the
     // round starts.
     bool fold_pending = false;
     F128 prev_rho;
+    const uint8_t *prev_rho_dev = nullptr;
     for (int i = 0; i < n; ++i) {
       const size_t half = len / 2;
       const bool split = i <= n_split;
-      const uint32_t seq = ctx->next_readback_seq();
+      const uint32_t seq = dfs ? 0 : ctx->next_readback_seq();
+      void *out2 = dfs ? (void *)(d_msgs + 32 * (size_t)i) : ctx->readback_dev;
       if (fold_pending) {
-        rc = zc_fold_round_device(ctx, st, A, Bv, prev_rho, E, half, A2, B2, i + 1 < n ? E2 : nullptr,
-                                  ctx->readback_dev, seq);
+        rc = zc_fold_round_device(ctx, st, A, Bv, prev_rho, E, half, A2, B2, i + 1 < n ? E2 : nullptr, out2, seq,
+                                  prev_rho_dev);
         std::swap(A, A2);
         std::swap(Bv, B2);
         fold_pending = false;
       } else if (i == 0) {
-        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, ctx->readback_dev, seq);
+        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, out2, seq);
       } else {  // 1 <= i <= n_split: every later round is fused with its fold
         rc = zc_round_split_device(ctx, st, A, Bv,
                                    (const uint8_t *)dElo + (elo0_len + ((size_t)(i - 1) << ZS_LOG)) * 16, Ehi, half,
-                                   ctx->readback_dev, i < n_split ? Ehi2 : nullptr, seq);
+                                   out2, i < n_split ? Ehi2 : nullptr, seq);
         std::swap(Ehi, Ehi2);
       }
       if (rc) return rc;
       if (!split) std::swap(E, E2);
-      uint4 h[2];
-      if ((rc = P.fetch(h, 32, seq))) return rc;
-      F128 g1 = fv(h[0]), gi = fv(h[1]);
-      zc_rounds.push_back({g1, gi});
-      t.observe(g1);
-      t.observe(gi);
-      F128 rho = t.sample();
-      rhos.push_back(rho);
+      F128 rho;
+      const uint8_t *rho_dev = nullptr;
+      if (dfs) {
+        rho_dev = d_rhos + 16 * (size_t)i;
+        if ((rc = fs_round_device(ctx, st, d_fs, out2, 2, (void *)rho_dev))) return rc;
+      } else {
+        uint4 h[2];
+        if ((rc = P.fetch(h, 32, seq))) return rc;
+        F128 g1 = fv(h[0]), gi = fv(h[1]);
+        zc_rounds.push_back({g1, gi});
+        t.observe(g1);
+        t.observe(gi);
+        rho = t.sample();
+        rhos.push_back(rho);
+      }
       if (i == 0) {
-        F128 wr[64];  // rho L_s(z)
-        for (int q = 0; q < 64; ++q) wr[q] = rho * ws[q];
-        std::vector<F128> lwr8(2048);
-        byte_tables(wr, lwr8.data());
-        if ((rc = P.h2d(dlwr8, lwr8.data(), lwr8.size() * 16))) return rc;
+        if (dfs) {
+          if ((rc = rho_bytetab_device(ctx, st, d_ws, rho_dev, dlwr8))) return rc;
+        } else {
+          F128 wr[64];  // rho L_s(z)
+          for (int q = 0; q < 64; ++q) wr[q] = rho * ws[q];
+          std::vector<F128> lwr8(2048);
+          byte_tables(wr, lwr8.data());
+          if ((rc = P.h2d(dlwr8, lwr8.data(), lwr8.size() * 16))) return rc;
+        }
         if ((rc = zc_fold0_device(ctx, st, a, b, dlw8, dlwr8, half, A, Bv))) return rc;
       } else if (i + 1 < n && i + 1 > n_split) {
         prev_rho = rho;  // folded by round i + 1's launch
+        prev_rho_dev = rho_dev;
         fold_pending = true;
       } else {
-        if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2))) return rc;
+        if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2, rho_dev))) return rc;
         std::swap(A, A2);
         std::swap(Bv, B2);
       }
       len = half;
     }
-    uint4 h[2];
-    if ((rc = P.d2h2(&h[0], A, 16, &h[1], Bv, 16))) return rc;
-    final_a = fv(h[0]);
-    final_b = fv(h[1]);
+    if (dfs) {
+      // one readback: messages, challenges, the final evaluations and the
+      // transcript state, which the host takes over
+      const size_t nm = 32 * (size_t)n, nr = 16 * (size_t)n;
+      uint8_t *land = ctx->landing;
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land, d_msgs, nm, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm, d_rhos, nr, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm + nr, A, 16, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm + nr + 16, Bv, 16, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm + nr + 32, d_fs, FS_STATE_BYTES, hipMemcpyDeviceToHost, st));
+      {
+        HostPhase hp(ctx, "wait_d2h");
+        if ((rc = P.wait_signal())) return rc;
+      }
+      for (int i = 0; i < n; ++i) {
+        F128 g1, gi, rh;
+        memcpy(&g1, land + 32 * i, 16);
+        memcpy(&gi, land + 32 * i + 16, 16);
+        memcpy(&rh, land + nm + 16 * i, 16);
+        zc_rounds.push_back({g1, gi});
+        rhos.push_back(rh);
+      }
+      memcpy(&final_a, land + nm + nr, 16);
+      memcpy(&final_b, land + nm + nr + 16, 16);
+      t.import_state(land + nm + nr + 32);
+    } else {
+      uint4 h[2];
+      if ((rc = P.d2h2(&h[0], A, 16, &h[1], Bv, 16))) return rc;
+      final_a = fv(h[0]);
+      final_b = fv(h[1]);
+    }
     t.observe(final_a);
     t.observe(final_b);
   }

@@ -2,20 +2,26 @@
 //
 // Requests (compressions, or a (log2, seed) pair expanded on the device by
 // generate_compressions, benchmark-tools/common/src/lib.rs:20-33) enter a FIFO
-// queue.  A pool of worker threads, each with its own context (device
-// buffers, HIP stream), admits the request at the head of the queue when a
-// worker is free and the HBM footprint of all admitted work stays within the
-// budget.  Proofs in different streams overlap on the device: one proof's
-// kernels run while another waits on its Fiat-Shamir round trip, and small
-// late-round kernels of one proof share the chip with large kernels of
-// another.  Each proof is computed by the same code as a solo prove, on
-// private buffers, so its bytes do not depend on what else is running.
+// queue.  A pool of worker threads, each with its own context and HIP
+// stream, proves them, admitted FIFO by the HBM footprint of their buffers
+// (Admission); one proof's kernels fill the GPU while another waits on its
+// Fiat-Shamir round trip.  Small requests (at most 2^FUSE_MAX_LOG2
+// compressions, whose short launches make launch overhead matter) are
+// batched phase by phase across requests: one thread takes the small
+// requests queued at the head, as many as fit the fronts' share of the
+// budget, and runs their fronts (input upload or expansion, witness,
+// codeword NTT, Merkle tree) fused, one launch per step for all of them
+// (run_fronts, batch.cpp); a worker then runs the rest of each.  Larger
+// requests go to the workers whole.
+// Each proof is computed by the same kernels as a solo prove on private
+// buffers, so its bytes do not depend on what else is running.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -23,6 +29,7 @@
 
 #include "../../include/flock_gpu.h"
 #include "admission.h"
+#include "batch.h"
 #include "ctx.h"
 #include "ligerito_configs.h"
 #include "prover.h"
@@ -45,14 +52,20 @@ int blocks_log2(uint64_t n) {
 
 }  // namespace
 
-// Device bytes a proof of 2^log2 compressions keeps in its context: at most
-// 16 packed bit-vectors of 2^(m-3) bytes (witness 3, codeword 2, zerocheck
-// round-1 sums and tables 4, ring-switch and Ligerito tables 3.5, with room
-// to spare) plus fixed tables.
-uint64_t prove_footprint_bytes(int n_log) {
+// Device bytes of one proof: its front (compressions, witness z, a, b, the
+// rate-1/2 codeword and its tree: about 5 packed bit-vectors of 2^(m-3)
+// bytes), and the rest of the proof in a worker's context (zerocheck
+// round-1 sums and tables, ring-switch and Ligerito tables: at most 11
+// bit-vectors with room to spare, plus fixed tables).
+uint64_t front_footprint_bytes(int n_log) {
   const int m = r1cs::K_LOG + n_log;
-  return 16ull * (1ull << (m - 3)) + (256ull << 20);
+  return (1ull << n_log) * sizeof(flock_compression) + 5ull * (1ull << (m - 3)) + (64ull << (m - 12)) + (1u << 20);
 }
+uint64_t back_footprint_bytes(int n_log) {
+  const int m = r1cs::K_LOG + n_log;
+  return 11ull * (1ull << (m - 3)) + (256ull << 20);
+}
+uint64_t prove_footprint_bytes(int n_log) { return front_footprint_bytes(n_log) + back_footprint_bytes(n_log); }
 
 struct Request {
   uint64_t id = 0;
@@ -73,10 +86,31 @@ struct Result {
   double t_submit = 0, t_start = 0, t_done = 0;
 };
 
+// A request whose front has run: its buffers (one device allocation) and root.
+struct Fronted {
+  Request req;
+  void *mem = nullptr;
+  uint64_t bytes = 0;
+  FrontJob job;
+  double t_start = 0;
+  int rc = 0;
+  std::string error;
+};
+
 class Server {
  public:
+  // Of the budget, fronts take FRONT_SHARE (at least one front of the
+  // largest supported size when the budget allows), workers the rest.
+  static constexpr double FRONT_SHARE = 0.35;
+  static constexpr size_t MAX_FUSED = 32;  // requests per fused front
+
   Server(int device, int n_workers, uint64_t budget, int reserve_log2)
-      : device_(device), reserve_log2_(reserve_log2), adm_(budget, n_workers), live_(n_workers) {
+      : device_(device),
+        reserve_log2_(reserve_log2),
+        front_budget_((uint64_t)(budget * FRONT_SHARE)),
+        adm_(budget - (uint64_t)(budget * FRONT_SHARE), n_workers),
+        live_(n_workers) {
+    front_thread_ = std::thread([this] { front_main(); });
     for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this, i] { worker_main(i); });
   }
   ~Server() {
@@ -85,17 +119,24 @@ class Server {
       stop_ = true;
     }
     cv_.notify_all();
+    front_thread_.join();
     for (auto &t : workers_) t.join();
+    for (auto &f : ready_)
+      if (f.mem) (void)hipFree(f.mem);
+    for (auto &kv : pool_) (void)hipFree(kv.second);
   }
 
   int submit(Request &&r, uint64_t *id, std::string &why) {
     std::lock_guard<std::mutex> lk(mu_);
-    if (!adm_.ever_fits(prove_footprint_bytes(r.log2))) {
+    const bool fits = r.log2 <= FUSE_MAX_LOG2
+                          ? front_footprint_bytes(r.log2) <= front_budget_ && adm_.ever_fits(back_footprint_bytes(r.log2))
+                          : adm_.ever_fits(prove_footprint_bytes(r.log2));
+    if (!fits) {
       why = "request exceeds the HBM budget";
       return FLOCK_E_OOM;
     }
-    if (live_ == 0) {
-      why = "no worker left: every worker's context timed out";
+    if (live_ == 0 || front_dead_) {
+      why = front_dead_ ? "the fronts' context timed out" : "no worker left: every worker's context timed out";
       return FLOCK_E_DEVICE;
     }
     r.id = next_id_++;
@@ -124,7 +165,7 @@ class Server {
 
   size_t pending() {
     std::lock_guard<std::mutex> lk(mu_);
-    return queue_.size() + running_;
+    return queue_.size() + fronting_ + ready_.size() + running_;
   }
 
   void set_wait_timeout(double s) {
@@ -132,64 +173,243 @@ class Server {
     wait_timeout_s_ = s;
   }
 
+  // Requests whose fronts ran fused with others', and fused front launches.
+  void fused_stats(uint64_t *requests, uint64_t *launches) {
+    std::lock_guard<std::mutex> lk(mu_);
+    *requests = fused_requests_;
+    *launches = front_batches_;
+  }
+
  private:
   // Fail a request without running it (under mu_).
-  void fail_locked(Request &&r, int rc, const std::string &why) {
+  void fail_locked(const Request &r, int rc, const std::string &why, double t_start) {
     Result res;
     res.id = r.id;
     res.rc = rc;
     res.error = why;
-    res.t_submit = res.t_start = r.t_submit;
+    res.t_submit = r.t_submit;
+    res.t_start = t_start ? t_start : r.t_submit;
     res.t_done = now_s();
     done_.push_back(std::move(res));
   }
 
+  // ---- stage 1: fused fronts
+  void front_main() {
+    Ctx ctx;
+    ctx.device = device_;
+    (void)hipSetDevice(device_);
+    if (hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking) != hipSuccess) {
+      std::lock_guard<std::mutex> lk(mu_);
+      front_dead_ = true;
+      fail_all_locked("cannot create the fronts' stream");
+      return;
+    }
+    for (;;) {
+      std::vector<Fronted> batch;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        auto fusable = [&] {
+          return !queue_.empty() && queue_.front().log2 <= FUSE_MAX_LOG2 &&
+                 front_live_ + front_footprint_bytes(queue_.front().log2) <= front_budget_;
+        };
+        cv_.wait(lk, [&] { return stop_ || fusable(); });
+        if (stop_) return;
+        ctx.wait_timeout_s = wait_timeout_s_;
+        const double t = now_s();
+        while (batch.size() < MAX_FUSED && fusable()) {
+          Fronted f;
+          f.req = std::move(queue_.front());
+          queue_.pop_front();
+          f.bytes = front_footprint_bytes(f.req.log2);
+          f.t_start = t;
+          front_live_ += f.bytes;
+          batch.push_back(std::move(f));
+        }
+        fronting_ += batch.size();
+      }
+      run_front_batch(ctx, batch);
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        fronting_ -= batch.size();
+        ++front_batches_;
+        if (batch.size() > 1) fused_requests_ += batch.size();
+        for (auto &f : batch) {
+          if (f.rc) {
+            fail_locked(f.req, f.rc, f.error, f.t_start);
+            give_front_memory_locked(f.mem, f.bytes);
+            front_live_ -= f.bytes;
+          } else {
+            ready_.push_back(std::move(f));
+          }
+        }
+        if (ctx.timed_out) {
+          front_dead_ = true;
+          fail_all_locked("the fronts' context timed out");
+        }
+      }
+      cv_.notify_all();
+      done_cv_.notify_all();
+      if (ctx.timed_out) return;
+    }
+  }
+
+  // Allocate each request's front buffers, bring its compressions to the
+  // device, then run every front fused.
+  void run_front_batch(Ctx &ctx, std::vector<Fronted> &batch) {
+    std::vector<FrontJob> jobs;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < batch.size(); ++i) {
+      Fronted &f = batch[i];
+      f.mem = take_front_memory(f.bytes);
+      if (!f.mem) {
+        f.rc = FLOCK_E_OOM;
+        f.error = "cannot allocate the front buffers";
+        continue;
+      }
+      const int m = r1cs::K_LOG + f.req.log2;
+      uint8_t *p = (uint8_t *)f.mem;
+      auto take = [&p](size_t n) {
+        uint8_t *q = p;
+        p += (n + 255) & ~(size_t)255;
+        return q;
+      };
+      flock_compression *blocks = (flock_compression *)take((1ull << f.req.log2) * sizeof(flock_compression));
+      FrontJob j;
+      j.blocks = blocks;
+      j.n_real = f.req.n;
+      j.n_log = f.req.log2;
+      const size_t words = (size_t)1 << (m - 6);
+      j.fb.z = take(8 * words);
+      j.fb.a = take(8 * words);
+      j.fb.b = take(8 * words);
+      j.fb.codeword = take((size_t)1024 << (m - 12));
+      j.fb.tree = take((((size_t)2 << (m - 12)) - 1) * 32);
+      int rc = FLOCK_OK;
+      if (f.req.seeded) {
+        rc = generate_compressions_device(&ctx, ctx.stream, (uint32_t)f.req.log2, f.req.seed, blocks);
+      } else if (hipMemcpyAsync(blocks, f.req.blocks.data(), f.req.n * sizeof(flock_compression),
+                                hipMemcpyHostToDevice, ctx.stream) != hipSuccess) {
+        rc = FLOCK_E_DEVICE;
+        ctx.last_error = "cannot upload the compressions";
+      }
+      if (rc) {
+        f.rc = rc;
+        f.error = ctx.last_error;
+        continue;
+      }
+      jobs.push_back(j);
+      idx.push_back(i);
+    }
+    if (jobs.empty()) return;
+    const int rc = run_fronts(&ctx, jobs, nullptr, nullptr);
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      Fronted &f = batch[idx[q]];
+      f.job = jobs[q];
+      if (rc) {
+        f.rc = ctx.timed_out ? FLOCK_E_DEVICE : rc;
+        f.error = ctx.last_error;
+      }
+    }
+  }
+
+  // Front buffers come from a pool of earlier ones of the same size (a
+  // request's front is one allocation whose size depends only on its
+  // log2), so steady serving neither allocates nor frees: hipFree waits for
+  // the device, which would stall the workers' streams.  Pooled bytes count
+  // against the fronts' budget; a new size frees pooled buffers of other
+  // sizes when needed.
+  void *take_front_memory(uint64_t bytes) {
+    std::vector<void *> drop;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = pool_.find(bytes);
+      if (it != pool_.end()) {
+        void *p = it->second;
+        pool_.erase(it);
+        pooled_bytes_ -= bytes;
+        return p;
+      }
+      // front_live_ already counts this request
+      while (front_live_ + pooled_bytes_ > front_budget_ && !pool_.empty()) {
+        auto last = std::prev(pool_.end());
+        pooled_bytes_ -= last->first;
+        drop.push_back(last->second);
+        pool_.erase(last);
+      }
+    }
+    for (void *p : drop) (void)hipFree(p);
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    return p;
+  }
+  void give_front_memory_locked(void *p, uint64_t bytes) {
+    if (!p) return;
+    pool_.emplace(bytes, p);
+    pooled_bytes_ += bytes;
+  }
+
+  // Everything queued or fronted fails (under mu_).
+  void fail_all_locked(const std::string &why) {
+    while (!queue_.empty()) {
+      fail_locked(queue_.front(), FLOCK_E_DEVICE, why, 0);
+      queue_.pop_front();
+    }
+    while (!ready_.empty()) {
+      fail_locked(ready_.front().req, FLOCK_E_DEVICE, why, ready_.front().t_start);
+      give_front_memory_locked(ready_.front().mem, ready_.front().bytes);
+      front_live_ -= ready_.front().bytes;
+      ready_.pop_front();
+    }
+  }
+
+  // ---- stage 2: the rest of each proof
   void worker_main(int w) {
     Ctx ctx;
     ctx.device = device_;
-    hipSetDevice(device_);
+    (void)hipSetDevice(device_);
     const bool stream_ok = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking) == hipSuccess;
     {
       std::lock_guard<std::mutex> lk(mu_);
       ctx.wait_timeout_s = wait_timeout_s_;
     }
     if (stream_ok && reserve_log2_ > 0) {
-      // Size every device buffer once for the largest request (one proof of
-      // generated compressions), so serving never reallocates.
+      // Size the context's buffers once for the largest request (one whole
+      // proof of generated compressions), so serving never reallocates.
       Request warm;
       warm.log2 = reserve_log2_;
       warm.n = 1ull << reserve_log2_;
       warm.seeded = true;
       {
         std::lock_guard<std::mutex> lk(mu_);
-        adm_.reserve(w, prove_footprint_bytes(warm.log2));
+        adm_.reserve(w, (reserve_log2_ > FUSE_MAX_LOG2 ? prove_footprint_bytes : back_footprint_bytes)(reserve_log2_));
       }
       std::vector<uint8_t> scratch_proof;
-      run(ctx, warm, scratch_proof);
+      run_whole(ctx, warm, scratch_proof);
     }
     for (;;) {
-      Request req;
+      Fronted f;
+      bool whole = false;
       {
         std::unique_lock<std::mutex> lk(mu_);
-        if (ctx.timed_out) {
+        if (ctx.timed_out || !stream_ok) {
           // This worker's context may still have work on the device: it
-          // takes no more requests.  With no worker left, the queue can
-          // never drain, so its requests fail now.
+          // takes no more requests.  With no worker left, nothing can drain.
           --live_;
-          if (live_ == 0)
-            while (!queue_.empty()) {
-              fail_locked(std::move(queue_.front()), FLOCK_E_DEVICE, "no worker left: every worker's context timed out");
-              queue_.pop_front();
-            }
+          if (live_ == 0) fail_all_locked("no worker left: every worker's context timed out");
           done_cv_.notify_all();
           cv_.notify_all();
           return;
         }
+        // the earliest submitted of: the head of the fronted requests, the
+        // head of the queue when it is large (a whole proof here)
         Admission::Action act = Admission::Wait;
         cv_.wait(lk, [&] {
           if (stop_) return true;
-          if (queue_.empty()) return false;
-          act = adm_.decide(w, prove_footprint_bytes(queue_.front().log2));
+          const bool q_large = !queue_.empty() && queue_.front().log2 > FUSE_MAX_LOG2;
+          if (ready_.empty() && !q_large) return false;
+          whole = q_large && (ready_.empty() || queue_.front().t_submit < ready_.front().req.t_submit);
+          act = whole ? adm_.decide(w, prove_footprint_bytes(queue_.front().log2))
+                      : adm_.decide(w, back_footprint_bytes(ready_.front().req.log2));
           return act != Admission::Wait;
         });
         if (stop_) return;
@@ -202,17 +422,27 @@ class Server {
           cv_.notify_all();
           continue;
         }
-        req = std::move(queue_.front());
-        queue_.pop_front();
-        adm_.take(w, prove_footprint_bytes(req.log2));
+        if (whole) {
+          f.req = std::move(queue_.front());
+          queue_.pop_front();
+          f.t_start = now_s();
+          adm_.take(w, prove_footprint_bytes(f.req.log2));
+        } else {
+          f = std::move(ready_.front());
+          ready_.pop_front();
+          adm_.take(w, back_footprint_bytes(f.req.log2));
+        }
         ctx.wait_timeout_s = wait_timeout_s_;
         ++running_;
       }
       Result res;
-      res.id = req.id;
-      res.t_submit = req.t_submit;
-      res.t_start = now_s();
-      res.rc = stream_ok ? run(ctx, req, res.proof) : FLOCK_E_DEVICE;
+      res.id = f.req.id;
+      res.t_submit = f.req.t_submit;
+      res.t_start = f.t_start;
+      (void)hipSetDevice(device_);
+      res.rc = whole ? run_whole(ctx, f.req, res.proof)
+                     : prove_after_commit_device(&ctx, ctx.stream, f.job.m, f.job.fb, f.job.root,
+                                                 f.req.domain.data(), f.req.domain.size(), res.proof);
       if (res.rc) {
         if (ctx.timed_out) res.rc = FLOCK_E_DEVICE;
         res.error = ctx.last_error;
@@ -221,6 +451,10 @@ class Server {
       {
         std::lock_guard<std::mutex> lk(mu_);
         --running_;
+        // the worker's stream has drained, so nothing reads the front any
+        // more; after a timeout it may, so those buffers are not reused
+        if (!ctx.timed_out) give_front_memory_locked(f.mem, f.bytes);
+        front_live_ -= f.bytes;
         adm_.finish(w);
         done_.push_back(std::move(res));
       }
@@ -229,8 +463,8 @@ class Server {
     }
   }
 
-  int run(Ctx &ctx, const Request &req, std::vector<uint8_t> &proof) {
-    hipSetDevice(device_);
+  // A whole proof on the worker's context and stream.
+  int run_whole(Ctx &ctx, const Request &req, std::vector<uint8_t> &proof) {
     const size_t bytes = req.n * sizeof(flock_compression);
     void *d = ctx.get_scratch("server.blocks", bytes);
     if (!d) return FLOCK_E_OOM;
@@ -247,15 +481,23 @@ class Server {
 
   int device_;
   int reserve_log2_;
+  uint64_t front_budget_;
+  std::thread front_thread_;
   std::vector<std::thread> workers_;
-  Admission adm_;  // guarded by mu_
+  Admission adm_;  // guarded by mu_ (workers' buffers)
   int live_;       // workers whose context has not timed out
+  bool front_dead_ = false;
+  uint64_t front_live_ = 0;  // bytes of front buffers allocated or about to be
+  std::multimap<uint64_t, void *> pool_;  // free front buffers by size
+  uint64_t pooled_bytes_ = 0;
   double wait_timeout_s_ = 120.0;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
-  std::deque<Request> queue_;
+  std::deque<Request> queue_;    // waiting for a front
+  std::deque<Fronted> ready_;    // front done, waiting for a worker
   std::deque<Result> done_;
-  size_t running_ = 0;
+  size_t fronting_ = 0, running_ = 0;
+  uint64_t fused_requests_ = 0, front_batches_ = 0;
   uint64_t next_id_ = 1;
   bool stop_ = false;
 };
@@ -300,8 +542,11 @@ extern "C" int flock_server_create(int device, int n_workers, uint64_t hbm_budge
   flock::r1cs::precompute_digests(flock::r1cs::K_LOG + 8, flock::r1cs::K_LOG + 21);
   flock_server *s = new (std::nothrow) flock_server();
   if (!s) return FLOCK_E_OOM;
-  if (reserve_log2 && (uint64_t)n_workers * flock::prove_footprint_bytes(flock::blocks_log2(1ull << reserve_log2)) >
-                          hbm_budget_bytes) {
+  const int rl = flock::blocks_log2(1ull << reserve_log2);
+  const uint64_t per_worker =
+      rl > flock::FUSE_MAX_LOG2 ? flock::prove_footprint_bytes(rl) : flock::back_footprint_bytes(rl);
+  if (reserve_log2 && (uint64_t)n_workers * per_worker >
+                          hbm_budget_bytes - (uint64_t)(hbm_budget_bytes * flock::Server::FRONT_SHARE)) {
     delete s;
     return FLOCK_E_OOM;
   }
@@ -383,6 +628,12 @@ extern "C" int flock_server_poll(flock_server *s, int wait_ms, uint64_t *request
 extern "C" int flock_server_set_wait_timeout(flock_server *s, double seconds) {
   if (!s || !(seconds == seconds)) return FLOCK_E_ARG;
   s->s->set_wait_timeout(seconds);
+  return FLOCK_OK;
+}
+
+extern "C" int flock_server_stats(flock_server *s, uint64_t *fused_requests, uint64_t *front_launches) {
+  if (!s || !fused_requests || !front_launches) return FLOCK_E_ARG;
+  s->s->fused_stats(fused_requests, front_launches);
   return FLOCK_OK;
 }
 

@@ -283,12 +283,12 @@ int zc_fold0_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, cons
 constexpr int FOLD_UNROLL = 2;
 
 __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__restrict__ A, const f128 *__restrict__ B,
-                                                               uint64_t half, f128 r, f128 *__restrict__ A2,
-                                                               f128 *__restrict__ B2) {
+                                                               uint64_t half, f128 r, const f128 *__restrict__ r_dev,
+                                                               f128 *__restrict__ A2, f128 *__restrict__ B2) {
   __shared__ f128 tab[4096];
   __shared__ f128 basis[128];
   const uint32_t tid = threadIdx.x, rot = tid & 15;
-  fulltab_basis(r, basis, tid);
+  fulltab_basis(r_dev ? *r_dev : r, basis, tid);
   __syncthreads();
   build_fulltab<FT_THREADS>(basis, tab, tid);
   __syncthreads();
@@ -326,13 +326,14 @@ __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__res
   }
 }
 
-int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2) {
+int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uint64_t half, F128 r, void *A2, void *B2,
+                     const void *d_r) {
   ScopedPhase ph(ctx, "fold_pair", st);
   f128 rr = make_uint4((uint32_t)r.lo, (uint32_t)(r.lo >> 32), (uint32_t)r.hi, (uint32_t)(r.hi >> 32));
   // FOLD_UNROLL elements per thread (one memory wait) until 512 workgroups
   unsigned g = grid_for(half, FT_THREADS * FOLD_UNROLL, 512);
   hipLaunchKernelGGL(fold_pair_kernel, dim3(g), dim3(FT_THREADS), 0, st, (const f128 *)A, (const f128 *)B, half, rr,
-                     (f128 *)A2, (f128 *)B2);
+                     (const f128 *)d_r, (f128 *)A2, (f128 *)B2);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
@@ -377,8 +378,10 @@ __device__ __forceinline__ void small_publish(f128 s0, f128 s1, f128 *red, f128 
     out[0] = red[0];
     out[1] = red[SMALL_THREADS];
     if (n_out > 2) out[2] = f128_zero();
-    __threadfence_system();
-    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (seq) {
+      __threadfence_system();
+      __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -389,9 +392,10 @@ __device__ __forceinline__ void small_publish(f128 s0, f128 s1, f128 *red, f128 
 // eq table E_next[y] = E[2y] + E[2y+1] (the two eq factors of the dropped
 // variable sum to one) unless E_next is null.
 __global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
-    const f128 *__restrict__ A, const f128 *__restrict__ B, f128 rho, const f128 *__restrict__ E, uint32_t half,
-    f128 *__restrict__ A2, f128 *__restrict__ B2, f128 *__restrict__ E_next, f128 *__restrict__ out, uint32_t *flag,
-    uint32_t seq) {
+    const f128 *__restrict__ A, const f128 *__restrict__ B, f128 rho_arg, const f128 *__restrict__ rho_dev,
+    const f128 *__restrict__ E, uint32_t half, f128 *__restrict__ A2, f128 *__restrict__ B2,
+    f128 *__restrict__ E_next, f128 *__restrict__ out, uint32_t *flag, uint32_t seq) {
+  const f128 rho = rho_dev ? *rho_dev : rho_arg;
   __shared__ f128 mt[SMALL_THREADS / 64][1024];
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
@@ -450,11 +454,13 @@ __global__ __launch_bounds__(SMALL_THREADS) void lig_fold_msg_small_kernel(const
 // and published by xor_rows_kernel.  The fold's products use comb tables of
 // the challenge (rebuilt per pair: the message's products overwrite them).
 __global__ __launch_bounds__(MUL_THREADS) void zc_fold_round_kernel(const f128 *__restrict__ A,
-                                                                    const f128 *__restrict__ B, f128 rho,
+                                                                    const f128 *__restrict__ B, f128 rho_arg,
+                                                                    const f128 *__restrict__ rho_dev,
                                                                     const f128 *__restrict__ E, uint64_t half,
                                                                     f128 *__restrict__ A2, f128 *__restrict__ B2,
                                                                     f128 *__restrict__ E_next,
                                                                     f128 *__restrict__ partial) {
+  const f128 rho = rho_dev ? *rho_dev : rho_arg;
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
   f128 *tab = &mt[0][0];
@@ -512,7 +518,8 @@ static f128 to_f128(F128 v) {
 }
 
 int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
-                               uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq) {
+                               uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq,
+                               const void *d_rho) {
   ScopedPhase ph(ctx, "zc_round", st);
   if (half == 0) {
     ctx->last_error = "zerocheck: fused round needs at least one pair";
@@ -521,8 +528,8 @@ int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B,
   if (half < 2) E_next = nullptr;
   if (half <= SMALL_PAIRS) {
     hipLaunchKernelGGL(zc_fold_round_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)A,
-                       (const f128 *)B, to_f128(rho), (const f128 *)E, (uint32_t)half, (f128 *)A2, (f128 *)B2,
-                       (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq);
+                       (const f128 *)B, to_f128(rho), (const f128 *)d_rho, (const f128 *)E, (uint32_t)half,
+                       (f128 *)A2, (f128 *)B2, (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
@@ -530,7 +537,8 @@ int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B,
   f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
   hipLaunchKernelGGL(zc_fold_round_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)A, (const f128 *)B,
-                     to_f128(rho), (const f128 *)E, half, (f128 *)A2, (f128 *)B2, (f128 *)E_next, part);
+                     to_f128(rho), (const f128 *)d_rho, (const f128 *)E, half, (f128 *)A2, (f128 *)B2,
+                     (f128 *)E_next, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return xor_rows_device(ctx, st, part, g, 2, out2, seq);
 }

@@ -62,6 +62,11 @@ class Transcript {
   }
   // PoW base: 32-byte root output of the current state (challenger.rs:440).
   void state_digest(uint8_t d[32]) const { h_.finalize_xof(d, 32); }
+  // The hasher state as bytes (the device transcript, fs_dev.hip, keeps the
+  // same layout), to move the transcript between host and device.
+  static constexpr size_t STATE_BYTES = sizeof(hb3::Hasher);
+  void export_state(void *dst) const { memcpy(dst, &h_, sizeof h_); }
+  void import_state(const void *src) { memcpy(&h_, src, sizeof h_); }
 
  private:
   void u8(uint8_t v) { h_.update(&v, 1); }

@@ -125,8 +125,10 @@ def lib():
         L.flock_server_submit_seeded.argtypes = [P, U32, U64, P, SZ, PU64]
         L.flock_server_poll.argtypes = [P, ctypes.c_int, PU64, P, SZ, PSZ, PD, PD]
         L.flock_server_pending.argtypes = [P, PU64]
+        L.flock_server_stats.argtypes = [P, PU64, PU64]
         L.flock_ctx_set_wait_timeout.argtypes = [P, ctypes.c_double]
         L.flock_statement_digest.argtypes = [U32, P]
+        L.flock_ctx_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_int64]
         L.flock_prove_batch.argtypes = [P, P, U32, P, SZ, U32, ctypes.c_int, P, SZ, P, P, P, P]
         L.flock_statement_setup.argtypes = [P, ctypes.c_char_p, SZ]
         L.flock_server_set_wait_timeout.argtypes = [P, ctypes.c_double]
@@ -146,7 +148,8 @@ def exported_symbols():
             "flock_server_create", "flock_server_destroy", "flock_server_last_error", "flock_server_submit",
             "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending",
             "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout", "flock_commit_device_noalloc",
-            "flock_statement_digest", "flock_statement_setup", "flock_prove_batch"]
+            "flock_statement_digest", "flock_statement_setup", "flock_prove_batch", "flock_server_stats",
+            "flock_ctx_set_option"]
 
 
 def device_count():
@@ -254,6 +257,10 @@ class Context:
         return bytes(root)
 
     # ---------------------------------------------------------------- timing
+    def set_option(self, name, value):
+        """flock_ctx_set_option (e.g. "device_transcript", 0 or 1)."""
+        self._check(lib().flock_ctx_set_option(self.handle, name.encode(), int(value)))
+
     def statement_setup(self):
         """Prepare the statement digest's spine on this GPU (persisted cache
         first); returns where it came from."""
@@ -489,6 +496,12 @@ class ProofServer:
         n = ctypes.c_uint64()
         lib().flock_server_pending(self.handle, ctypes.byref(n))
         return n.value
+
+    def stats(self):
+        """(requests whose fronts ran fused with others', fused front launch sets)."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        lib().flock_server_stats(self.handle, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def close(self):
         if self.handle:

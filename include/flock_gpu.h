@@ -70,6 +70,14 @@ const char *flock_last_error(const flock_ctx *ctx);
  * trial from outside (benchmark-tools/harness/src/main.rs:20-21). */
 int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds);
 
+/* Context options (the proof bytes never depend on them):
+ *  "device_transcript" (default 1): the zerocheck rounds' Fiat-Shamir steps
+ *    run on the device (the BLAKE3 transcript state moves to device memory
+ *    and back) instead of one host round trip per round; 0 keeps them on
+ *    the host.
+ * Returns FLOCK_E_ARG for an unknown name. */
+int flock_ctx_set_option(flock_ctx *ctx, const char *name, int64_t value);
+
 /* PCS commit of an F128-packed witness: replicate, interleaved additive NTT
  * from layer log_inv_rate, BLAKE3 Merkle tree over positions.  Replaces
  * `pcs::commit` (crates/flock-core/src/pcs/commit.rs:250) returning
@@ -226,10 +234,13 @@ typedef struct {
  * No counterpart in the reference (its worker proves one seed per process,
  * worker/src/main.rs:36-45); each proof is the same bundle bytes as
  * flock_prove_blake3 for the same input.  The fronts (witness, codeword NTT,
- * Merkle tree) of all items run fused: one launch per step for every item,
- * the launch sequence captured in a HIP graph per batch shape (the list of
- * sizes) and replayed when the same shape comes again, if use_graph.  The
- * rest of each proof runs on n_workers (1..16) worker streams, largest first.
+ * Merkle tree) of the items of at most 2^14 compressions, whose short
+ * launches make launch overhead matter, run fused: one launch per step for
+ * all of them, the launch sequence captured in a HIP graph per shape (the
+ * list of their sizes) and replayed when the same shape comes again, if
+ * use_graph; the rest of their proofs follows on the workers.  Larger items
+ * run as whole proofs on the n_workers (1..16) worker streams from the
+ * start, largest first, in parallel with the fused fronts.
  * out receives the proofs back to back: proof i is out[offsets[i] ..
  * offsets[i + 1]) (offsets has n_items + 1 entries); if cap is too small,
  * FLOCK_E_CAP with offsets filled and nothing copied.  done_s (n_items, may
@@ -243,6 +254,10 @@ int flock_prove_batch(flock_ctx *ctx, const flock_batch_item *items, uint32_t n_
 /* ------------------------------------------------------------------ serving */
 
 /* A proof server on GPU `device`: continuous batching of independent proofs.
+ * Small requests (at most 2^14 compressions) queued together have their
+ * fronts run fused (one launch per step for all of them, flock_prove_batch's
+ * kernels) and the rest of each proof on a worker; larger requests run
+ * whole on one of n_workers worker streams.
  * No counterpart in the reference (its worker proves one seed per process,
  * worker/src/main.rs:36-45); each proof is the same bundle bytes as
  * flock_prove_blake3 for the same input, whatever else is in flight.
@@ -273,6 +288,10 @@ int flock_server_poll(flock_server *server, int wait_ms, uint64_t *request_id, u
                       size_t *out_len, double *latency_s, double *queue_s);
 /* Requests queued or running. */
 int flock_server_pending(flock_server *server, uint64_t *n);
+/* Requests whose fronts (witness, codeword NTT, Merkle tree) ran fused with
+ * other requests' in one set of launches, and the number of front launches
+ * sets so far. */
+int flock_server_stats(flock_server *server, uint64_t *fused_requests, uint64_t *front_launches);
 /* The wait deadline of every worker's context (flock_ctx_set_wait_timeout),
  * applied from each worker's next request on.  A request whose wait times
  * out fails with FLOCK_E_DEVICE and its worker retires; when no worker is

@@ -66,3 +66,31 @@ def test_oversized_request_is_rejected():
             server.submit_seeded(18, 1)  # needs ~9 GB
     finally:
         server.close()
+
+
+def test_queued_requests_share_fused_fronts(ctx):
+    """Requests queued together have their fronts (witness, codeword NTT,
+    Merkle tree) run fused; every result still equals the solo proof."""
+    print("server: 12 requests at once, 1 worker", flush=True)
+    server = flockgpu.ProofServer(0, n_workers=1)
+    try:
+        want = {}
+        for i in range(12):
+            log2, seed = 8 + i % 4, 0x7700 + i
+            want[server.submit_seeded(log2, seed)] = (log2, seed)
+        got = {}
+        while len(got) < len(want):
+            r = server.poll(120000)
+            assert r is not None, "server stalled"
+            got[r[0]] = r[1]
+        fused, launches = server.stats()
+    finally:
+        server.close()
+    print(f"server: {fused} requests fused over {launches} front launch sets", flush=True)
+    assert fused >= 2 and launches < 12
+    for rid, (log2, seed) in want.items():
+        s = flockgpu.Blake3Setup(ctx, 1 << log2)
+        try:
+            assert got[rid] == s.prove_seeded(log2, seed)
+        finally:
+            s.close()
