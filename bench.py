@@ -272,6 +272,37 @@ def bench_prove(args):
     return out
 
 
+def bench_sizes(args):
+    """Single-proof latency by size (device-resident inputs, 1 warm-up and 5
+    timed proofs each, median), with the zerocheck transcript on the device
+    (default) and, for comparison, on the host."""
+    import flockgpu
+    ctx = flockgpu.Context(0)
+    out = {}
+    try:
+        for log2 in (10, 12, 14, 16, 18):
+            s = flockgpu.Blake3Setup(ctx, 1 << log2)
+            buf = flockgpu.generate_compressions_device(ctx, log2, 0x512E + log2)
+            row = {}
+            for mode in (1, 0):
+                ctx.set_option("device_transcript", mode)
+                s.prove_fast_device(buf.ptr, 1 << log2)
+                ts = []
+                for _ in range(5):
+                    t0 = time.perf_counter()
+                    s.prove_fast_device(buf.ptr, 1 << log2)
+                    ts.append(time.perf_counter() - t0)
+                row["device_transcript_ms" if mode else "host_transcript_ms"] = 1000 * percentile(ts, 50)
+            ctx.set_option("device_transcript", 1)
+            row["compressions_per_s"] = (1 << log2) / (row["device_transcript_ms"] / 1000)
+            out[str(log2)] = row
+            buf.free()
+            s.close()
+    finally:
+        ctx.close()
+    return out
+
+
 def splitmix64(seed):
     """SplitMix64 stream (benchmark-tools/common/src/lib.rs) as a generator."""
     state = seed & 0xFFFFFFFFFFFFFFFF
@@ -525,6 +556,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true", help="skip the verifier check (profiling runs only)")
     ap.add_argument("--no-serve", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--no-sizes", action="store_true", help="skip the single-proof latency by size")
     ap.add_argument("--workers", type=int, default=3, help="concurrent proofs in the batched/served runs")
     ap.add_argument("--rate", type=float, default=100.0, help="served: offered load, requests/s")
     ap.add_argument("--sweep", default="150,200,250",
@@ -544,6 +576,8 @@ def main():
         out = bench_commit(args)
     else:
         out = bench_prove(args)
+        if not args.no_sizes:
+            out["by_size"] = bench_sizes(args)
         if not args.no_serve:
             out["batched"] = bench_batched(args)
             out["batched_server"] = bench_batched_server(args)

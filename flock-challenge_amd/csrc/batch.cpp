@@ -220,7 +220,7 @@ struct FrontRunner {
         const bool timing = ctx->timing;
         ctx->timing = false;
         hipGraph_t g = nullptr;
-        FLOCK_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        FLOCK_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));  // other threads keep using their streams
         const int rc = issue(p, d_jobs, st);
         const hipError_t e = hipStreamEndCapture(st, &g);
         ctx->timing = timing;
@@ -278,6 +278,9 @@ struct BatchEngine {
       std::unique_ptr<Ctx> w(new Ctx());
       w->device = ctx->device;
       w->wait_timeout_s = ctx->wait_timeout_s;
+      // several workers run at once: no graph captures on their streams
+      // (a capture on one thread is disturbed by the others' runtime calls)
+      w->zc_graph = false;
       if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
         ctx->last_error = "batch: cannot create a worker stream";
         return FLOCK_E_DEVICE;
@@ -368,40 +371,8 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
   done_s.assign(in.size(), 0.0);
   std::vector<int> rcs(in.size(), FLOCK_OK);
   std::vector<std::string> errs(in.size());
-  std::vector<std::thread> ts;
-  for (int w = 0; w < n_workers; ++w)
-    ts.emplace_back([&, w] {
-      Ctx *wc = E->workers[w].get();
-      hipSetDevice(wc->device);
-      for (;;) {
-        size_t i;
-        {
-          std::unique_lock<std::mutex> ql(qmu);
-          qcv.wait(ql, [&] { return !work.empty() || fronts_done; });
-          if (work.empty()) return;
-          i = work.front();
-          work.pop_front();
-        }
-        const Inst &x = in[i];
-        if (x.n_log > FUSE_MAX_LOG2) {
-          // a whole proof on the worker's stream
-          const flock_compression *blocks = x.blocks;
-          if (!blocks) {
-            void *d = wc->get_scratch("batch.blocks", x.n_real * sizeof(flock_compression));
-            rcs[i] = d ? generate_compressions_device(wc, wc->stream, items[i].log2, items[i].seed,
-                                                      (flock_compression *)d)
-                       : FLOCK_E_OOM;
-            blocks = (const flock_compression *)d;
-          }
-          if (!rcs[i]) rcs[i] = prove_blake3_device(wc, wc->stream, blocks, x.n_real, domain, domain_len, proofs[i]);
-        } else {
-          rcs[i] = prove_after_commit_device(wc, wc->stream, x.m, x.fb, x.root, domain, domain_len, proofs[i]);
-        }
-        if (rcs[i]) errs[i] = wc->last_error;
-        done_s[i] = now_s() - t0;
-      }
-    });
-  // meanwhile, the small items' fronts, fused (graph per shape)
+  // first the small items' fronts, fused (a graph per shape), on this
+  // thread alone: a graph capture must not race the workers' runtime calls
   bool replayed = false;
   double front_s = 0;
   int front_rc = FLOCK_OK;
@@ -448,6 +419,39 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
     }
     qcv.notify_all();
   }
+  std::vector<std::thread> ts;
+  for (int w = 0; w < n_workers; ++w)
+    ts.emplace_back([&, w] {
+      Ctx *wc = E->workers[w].get();
+      hipSetDevice(wc->device);
+      for (;;) {
+        size_t i;
+        {
+          std::unique_lock<std::mutex> ql(qmu);
+          qcv.wait(ql, [&] { return !work.empty() || fronts_done; });
+          if (work.empty()) return;
+          i = work.front();
+          work.pop_front();
+        }
+        const Inst &x = in[i];
+        if (x.n_log > FUSE_MAX_LOG2) {
+          // a whole proof on the worker's stream
+          const flock_compression *blocks = x.blocks;
+          if (!blocks) {
+            void *d = wc->get_scratch("batch.blocks", x.n_real * sizeof(flock_compression));
+            rcs[i] = d ? generate_compressions_device(wc, wc->stream, items[i].log2, items[i].seed,
+                                                      (flock_compression *)d)
+                       : FLOCK_E_OOM;
+            blocks = (const flock_compression *)d;
+          }
+          if (!rcs[i]) rcs[i] = prove_blake3_device(wc, wc->stream, blocks, x.n_real, domain, domain_len, proofs[i]);
+        } else {
+          rcs[i] = prove_after_commit_device(wc, wc->stream, x.m, x.fb, x.root, domain, domain_len, proofs[i]);
+        }
+        if (rcs[i]) errs[i] = wc->last_error;
+        done_s[i] = now_s() - t0;
+      }
+    });
   for (auto &t : ts) t.join();
   if (front_rc) return front_rc;
   for (size_t i = 0; i < in.size(); ++i)

@@ -132,6 +132,10 @@ extern "C" int flock_ctx_set_option(flock_ctx *ctx, const char *name, int64_t va
     ctx->c.device_transcript = value != 0;
     return FLOCK_OK;
   }
+  if (!strcmp(name, "zc_graph")) {
+    ctx->c.zc_graph = value != 0;
+    return FLOCK_OK;
+  }
   ctx->c.last_error = std::string("unknown option ") + name;
   return FLOCK_E_ARG;
 }

@@ -244,6 +244,7 @@ Ctx::~Ctx() {
     hipEventDestroy(e.end);
   }
   for (hipEvent_t e : event_pool) hipEventDestroy(e);
+  for (auto &kv : zc_graphs) (void)hipGraphExecDestroy(kv.second);
   if (pinned) hipHostFree(pinned);
   if (landing) hipHostFree(landing);
   if (open_area) hipHostFree(open_area);

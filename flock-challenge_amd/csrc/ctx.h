@@ -56,6 +56,10 @@ struct Ctx {
   // Run the zerocheck rounds' Fiat-Shamir steps on the device (fs_dev.hip)
   // instead of a host round trip per round; the proof bytes are the same.
   bool device_transcript = true;
+  // Capture the zerocheck rounds (device transcript only) in a HIP graph per
+  // m and buffer set, replayed by later proofs.
+  bool zc_graph = true;
+  std::map<std::vector<const void *>, hipGraphExec_t> zc_graphs;
   // Bounded waits: until a sequence number appears at `flag` (published by
   // the device in the mapped readback area), until the stream has drained,
   // until the event has completed.  FLOCK_OK, or FLOCK_E_DEVICE with

@@ -749,6 +749,32 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
       t.export_state(st_bytes);
       if ((rc = P.h2d(d_fs, st_bytes, FS_STATE_BYTES)) || (rc = P.h2d(d_ws, ws, sizeof ws))) return rc;
     }
+    // The reductions' partials: sized once for the largest grid (4096
+    // workgroups x 2 values), so no launcher regrows it mid-sequence and a
+    // captured graph keeps pointing at it.
+    void *d_partial = P.dev("zc.round.partial", (size_t)4096 * 2 * 16);
+    if (!d_partial) return FLOCK_E_OOM;
+    // With the device transcript the rounds are a fixed sequence of launches
+    // for this m and these buffers: captured once in a HIP graph and
+    // replayed by later proofs.
+    std::vector<const void *> graph_key;
+    if (dfs && ctx->zc_graph)
+      graph_key = {(const void *)(uintptr_t)m, A, Bv, A2, B2, E, E2, dElo, Ehi, Ehi2, d_fs, d_msgs, d_rhos,
+                   d_ws, dlw8, dlwr8, dlw, a, b, d_partial};
+    hipGraphExec_t *graph = nullptr;
+    bool capturing = false;
+    if (!graph_key.empty()) {
+      auto it = ctx->zc_graphs.find(graph_key);
+      if (it != ctx->zc_graphs.end()) {
+        graph = &it->second;
+      } else {
+        FLOCK_HIP_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));  // other threads keep using their streams
+        capturing = true;
+      }
+    }
+    const bool replay = graph != nullptr;  // the loop below does not run
+    const bool timing_was = ctx->timing;
+    if (capturing) ctx->timing = false;  // phase events are not part of the graph
     size_t len = rows;
     // A whole-table round folds the previous round's tables itself (one
     // launch fewer per round): then the fold by prev_rho is pending when the
@@ -756,7 +782,7 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
     bool fold_pending = false;
     F128 prev_rho;
     const uint8_t *prev_rho_dev = nullptr;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n && !graph; ++i) {
       const size_t half = len / 2;
       const bool split = i <= n_split;
       const uint32_t seq = dfs ? 0 : ctx->next_readback_seq();
@@ -775,13 +801,13 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
                                    out2, i < n_split ? Ehi2 : nullptr, seq);
         std::swap(Ehi, Ehi2);
       }
-      if (rc) return rc;
+      if (rc) break;
       if (!split) std::swap(E, E2);
       F128 rho;
       const uint8_t *rho_dev = nullptr;
       if (dfs) {
         rho_dev = d_rhos + 16 * (size_t)i;
-        if ((rc = fs_round_device(ctx, st, d_fs, out2, 2, (void *)rho_dev))) return rc;
+        if ((rc = fs_round_device(ctx, st, d_fs, out2, 2, (void *)rho_dev))) break;
       } else {
         uint4 h[2];
         if ((rc = P.fetch(h, 32, seq))) return rc;
@@ -794,7 +820,7 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
       }
       if (i == 0) {
         if (dfs) {
-          if ((rc = rho_bytetab_device(ctx, st, d_ws, rho_dev, dlwr8))) return rc;
+          if ((rc = rho_bytetab_device(ctx, st, d_ws, rho_dev, dlwr8))) break;
         } else {
           F128 wr[64];  // rho L_s(z)
           for (int q = 0; q < 64; ++q) wr[q] = rho * ws[q];
@@ -802,17 +828,42 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
           byte_tables(wr, lwr8.data());
           if ((rc = P.h2d(dlwr8, lwr8.data(), lwr8.size() * 16))) return rc;
         }
-        if ((rc = zc_fold0_device(ctx, st, a, b, dlw8, dlwr8, half, A, Bv))) return rc;
+        if ((rc = zc_fold0_device(ctx, st, a, b, dlw8, dlwr8, half, A, Bv))) break;
       } else if (i + 1 < n && i + 1 > n_split) {
         prev_rho = rho;  // folded by round i + 1's launch
         prev_rho_dev = rho_dev;
         fold_pending = true;
       } else {
-        if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2, rho_dev))) return rc;
+        if ((rc = fold_pair_device(ctx, st, A, Bv, half, rho, A2, B2, rho_dev))) break;
         std::swap(A, A2);
         std::swap(Bv, B2);
       }
       len = half;
+    }
+    if (capturing) {
+      ctx->timing = timing_was;
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      FLOCK_HIP_CHECK(ctx, e);
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      FLOCK_HIP_CHECK(ctx, ie);
+      graph = &ctx->zc_graphs.emplace(graph_key, ex).first->second;
+    }
+    if (rc) return rc;
+    if (graph) {
+      FLOCK_HIP_CHECK(ctx, hipGraphLaunch(*graph, st));
+      // where the final tables are when the loop did not run: every round
+      // after round 0 folds once into the other buffer pair (n - 1 swaps)
+      if (replay && ((n - 1) & 1)) {
+        std::swap(A, A2);
+        std::swap(Bv, B2);
+      }
     }
     if (dfs) {
       // one readback: messages, challenges, the final evaluations and the

@@ -366,6 +366,9 @@ class Server {
   void worker_main(int w) {
     Ctx ctx;
     ctx.device = device_;
+    // several workers run at once: no graph captures on their streams (a
+    // capture on one thread is disturbed by the others' runtime calls)
+    ctx.zc_graph = false;
     (void)hipSetDevice(device_);
     const bool stream_ok = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking) == hipSuccess;
     {

@@ -75,6 +75,9 @@ int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds);
  *    run on the device (the BLAKE3 transcript state moves to device memory
  *    and back) instead of one host round trip per round; 0 keeps them on
  *    the host.
+ *  "zc_graph" (default 1): with the device transcript, the zerocheck rounds'
+ *    launches are captured once per size and buffer set in a HIP graph and
+ *    replayed by later proofs.
  * Returns FLOCK_E_ARG for an unknown name. */
 int flock_ctx_set_option(flock_ctx *ctx, const char *name, int64_t value);
 

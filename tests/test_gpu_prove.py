@@ -147,20 +147,28 @@ def test_shape_errors_are_reported(ctx):
 
 @pytest.mark.parametrize("log2", [8, 12, 16])
 def test_device_and_host_transcripts_give_the_same_bytes(ctx, log2):
-    """The zerocheck rounds' Fiat-Shamir steps on the device (the default)
-    and on the host give identical proofs; at 2^8 and 2^12 both equal the
-    oracle's."""
+    """The zerocheck rounds' Fiat-Shamir steps on the device (the default;
+    the first proof of a size captures the rounds in a HIP graph, later
+    ones replay it) and on the host give identical proofs, for three inputs
+    in a row; at 2^8 and 2^12 they also equal the oracle's."""
     _say(f"device vs host transcript log2={log2}")
-    seed = 0xD7F5_0000 | log2
     s = flockgpu.Blake3Setup(ctx, 1 << log2)
     try:
-        ctx.set_option("device_transcript", 1)
-        dev = s.prove_seeded(log2, seed)
-        ctx.set_option("device_transcript", 0)
-        host = s.prove_seeded(log2, seed)
+        for k in range(3):
+            seed = 0xD7F5_0000 | (k << 8) | log2
+            ctx.set_option("device_transcript", 1)
+            dev = s.prove_seeded(log2, seed)
+            ctx.set_option("device_transcript", 0)
+            host = s.prove_seeded(log2, seed)
+            ctx.set_option("device_transcript", 1)
+            ctx.set_option("zc_graph", 0)
+            direct = s.prove_seeded(log2, seed)
+            ctx.set_option("zc_graph", 1)
+            _check_same(dev, host)
+            _check_same(direct, host)
+            if log2 <= 12 and k == 0:
+                _check_same(dev, O.prove(O.generate_compressions(log2, seed)))
     finally:
         ctx.set_option("device_transcript", 1)
+        ctx.set_option("zc_graph", 1)
         s.close()
-    _check_same(dev, host)
-    if log2 <= 12:
-        _check_same(dev, O.prove(O.generate_compressions(log2, seed)))
