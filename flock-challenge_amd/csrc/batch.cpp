@@ -58,7 +58,8 @@ using Inst = FrontJob;
 // upload) and the launches that read them.
 struct Launch {
   enum Kind { WITNESS, NTT, LEAF, SUBTREE, ROOTS } kind;
-  NttVariant variant = NTT_SHARED_G2;
+  int F = 0;  // NTT: the pass's layer count and kernel
+  NttKernel kernel = NTT_SHARED_W8G2;
   size_t offset = 0;  // byte offset of its jobs in the job area
   uint32_t n_jobs = 0;
   uint64_t wgs = 0;
@@ -105,40 +106,38 @@ struct FrontRunner {
       wj.push_back({x.blocks, x.n_real, n_total, (uint64_t *)x.fb.z, (uint64_t *)x.fb.a, (uint64_t *)x.fb.b, wg});
       wg += (n_total + 63) / 64;
     }
-    p.launches.push_back({Launch::WITNESS, NTT_SHARED_G2, p.add(wj), (uint32_t)wj.size(), wg});
-    // NTT pass stages: pass s of every instance, one launch per variant; the
-    // first pass of each reads the message (positions masked to the
-    // un-replicated half), as commit_device does
-    std::vector<std::vector<std::pair<NttVariant, NttJob>>> stages;
-    std::vector<uint64_t> stage_wgs;
+    p.launches.push_back({Launch::WITNESS, 0, NTT_SHARED_W8G2, p.add(wj), (uint32_t)wj.size(), wg});
+    // NTT pass stages: pass s of every instance (its plan's), one launch per
+    // (F, kernel); the first pass of each reads the message (positions
+    // masked to the un-replicated half), as commit_device does
+    std::vector<std::vector<std::pair<int, NttJob>>> stages;  // (F * NTT_KERNELS + kernel, job)
     for (auto &x : in) {
       const uint4 *tw = (const uint4 *)ctx->device_twiddles(x.k);
-      int l = 1, s = 0;
-      while (l < x.k) {
-        const int F = std::min(4, x.k - l);
-        uint64_t w = 0;
-        const NttVariant v = ntt_pass_variant(x.k, l, F, &w);
+      const std::vector<NttPass> plan = ntt_plan(x.k, 1);
+      for (size_t s = 0; s < plan.size(); ++s) {
+        const NttPass &q = plan[s];
         NttJob j{(uint4 *)x.fb.codeword, tw, s == 0 ? (const uint4 *)x.fb.z : nullptr,
-                 s == 0 ? (1ull << (x.k - 1)) - 1 : ~0ull, x.k, l, w};  // wg_begin holds the count for now
-        if ((int)stages.size() <= s) stages.emplace_back();
-        stages[s].push_back({v, j});
-        l += F;
-        ++s;
+                 s == 0 ? (1ull << (x.k - 1)) - 1 : ~0ull, x.k, q.l,
+                 ntt_pass_wgs(x.k, q.F, q.kernel)};  // wg_begin holds the count for now
+        if (stages.size() <= s) stages.emplace_back();
+        stages[s].push_back({q.F * NTT_KERNELS + (int)q.kernel, j});
       }
     }
     for (auto &st : stages)
-      for (int v = 0; v < NTT_VARIANTS; ++v) {
+      for (int key = 0; key < 5 * NTT_KERNELS; ++key) {
         std::vector<NttJob> jobs;
         uint64_t w = 0;
         for (auto &pj : st)
-          if (pj.first == v) {
+          if (pj.first == key) {
             NttJob j = pj.second;
             const uint64_t cnt = j.wg_begin;
             j.wg_begin = w;
             w += cnt;
             jobs.push_back(j);
           }
-        if (!jobs.empty()) p.launches.push_back({Launch::NTT, (NttVariant)v, p.add(jobs), (uint32_t)jobs.size(), w});
+        if (!jobs.empty())
+          p.launches.push_back({Launch::NTT, key / NTT_KERNELS, (NttKernel)(key % NTT_KERNELS), p.add(jobs),
+                                (uint32_t)jobs.size(), w});
       }
     // Merkle leaves, then subtree stages of up to 9 levels
     std::vector<MerkleJob> lj;
@@ -148,7 +147,7 @@ struct FrontRunner {
       lj.push_back({x.fb.codeword, x.fb.tree, npos, 0, 1024u, 0, wg});
       wg += (npos + 255) / 256;
     }
-    p.launches.push_back({Launch::LEAF, NTT_SHARED_G2, p.add(lj), (uint32_t)lj.size(), wg});
+    p.launches.push_back({Launch::LEAF, 0, NTT_SHARED_W8G2, p.add(lj), (uint32_t)lj.size(), wg});
     std::vector<std::vector<MerkleJob>> sub;
     for (auto &x : in) {
       uint64_t rs = 0, rl = 1ull << x.k;
@@ -170,11 +169,11 @@ struct FrontRunner {
         j.wg_begin = w;
         w += cnt;
       }
-      p.launches.push_back({Launch::SUBTREE, NTT_SHARED_G2, p.add(st), (uint32_t)st.size(), w});
+      p.launches.push_back({Launch::SUBTREE, 0, NTT_SHARED_W8G2, p.add(st), (uint32_t)st.size(), w});
     }
     std::vector<MerkleJob> rj;
     for (auto &x : in) rj.push_back({nullptr, x.fb.tree, (2ull << x.k) - 1, 0, 0u, 0, 0});
-    p.launches.push_back({Launch::ROOTS, NTT_SHARED_G2, p.add(rj), (uint32_t)rj.size(), 0});
+    p.launches.push_back({Launch::ROOTS, 0, NTT_SHARED_W8G2, p.add(rj), (uint32_t)rj.size(), 0});
     p.roots_offset = align256(p.jobs.size());
     p.jobs.resize(p.roots_offset + 32 * in.size());
     return p;
@@ -186,7 +185,7 @@ struct FrontRunner {
       int rc = FLOCK_OK;
       switch (L.kind) {
         case Launch::WITNESS: rc = witness_batch_device(ctx, st, (const WitnessJob *)j, L.n_jobs, L.wgs); break;
-        case Launch::NTT: rc = ntt_batch_device(ctx, st, L.variant, (const NttJob *)j, L.n_jobs, L.wgs); break;
+        case Launch::NTT: rc = ntt_batch_device(ctx, st, L.F, L.kernel, (const NttJob *)j, L.n_jobs, L.wgs); break;
         case Launch::LEAF: rc = merkle_leaf_batch_device(ctx, st, (const MerkleJob *)j, L.n_jobs, L.wgs); break;
         case Launch::SUBTREE: rc = merkle_subtree_batch_device(ctx, st, (const MerkleJob *)j, L.n_jobs, L.wgs); break;
         case Launch::ROOTS:

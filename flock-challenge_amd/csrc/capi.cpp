@@ -180,6 +180,70 @@ extern "C" int flock_commit_device_noalloc(flock_ctx *ctx, const flock_pcs_param
                                                          pick_stream(ctx, stream)));
 }
 
+// ------------------------------------------------------------------ NTT plans
+extern "C" int flock_ntt_autotune(flock_ctx *ctx, const flock_pcs_params *params, int reps, char *json, size_t cap) {
+  if (int u = usable(ctx)) return u;
+  if (!params || !json || cap == 0) return FLOCK_E_ARG;
+  Ctx *c = &ctx->c;
+  if (const char *why = flock::commit_params_unsupported(params)) {
+    c->last_error = std::string("ntt autotune: ") + why;
+    return FLOCK_E_SHAPE;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) return FLOCK_E_DEVICE;
+  const int k = (int)params->m - 7 - (int)params->log_batch_size + (int)params->log_inv_rate;
+  std::string out;
+  const int rc = flock::ntt_autotune(c, k, (int)params->log_inv_rate, reps, out);
+  if (rc) return device_rc(c, rc);
+  if (out.size() + 1 > cap) {
+    c->last_error = "ntt autotune: json buffer too small (" + std::to_string(out.size() + 1) + " bytes needed)";
+    return FLOCK_E_CAP;
+  }
+  memcpy(json, out.c_str(), out.size() + 1);
+  return FLOCK_OK;
+}
+
+// The plan table is process-wide: these need no GPU, and ctx (may be NULL)
+// only receives the error text.
+extern "C" int flock_ntt_plan_set(flock_ctx *ctx, uint32_t k, uint32_t start, const int32_t *passes,
+                                  uint32_t n_passes) {
+  if ((n_passes && !passes) || n_passes > 64) return FLOCK_E_ARG;
+  std::string err;
+  std::vector<flock::NttPass> p;
+  for (uint32_t i = 0; i < n_passes; ++i) {
+    const int32_t kn = passes[3 * i + 2];
+    if (kn < 0 || kn >= flock::NTT_KERNELS) {
+      if (ctx) ctx->c.last_error = "ntt plan: unknown kernel " + std::to_string(kn);
+      return FLOCK_E_ARG;
+    }
+    p.push_back({passes[3 * i], passes[3 * i + 1], (flock::NttKernel)kn});
+  }
+  const int rc = flock::ntt_plan_install((int)k, (int)start, p, err);
+  if (rc && ctx) ctx->c.last_error = err;
+  return rc;
+}
+
+extern "C" int flock_ntt_plan_get(flock_ctx *ctx, uint32_t k, uint32_t start, int32_t *passes, uint32_t cap,
+                                  uint32_t *n_passes) {
+  (void)ctx;
+  if (!n_passes || (cap && !passes) || start >= k || k > 40) return FLOCK_E_ARG;
+  const std::vector<flock::NttPass> p = flock::ntt_plan((int)k, (int)start);
+  *n_passes = (uint32_t)p.size();
+  if (p.size() > cap) return FLOCK_E_CAP;
+  for (size_t i = 0; i < p.size(); ++i) {
+    passes[3 * i] = p[i].l;
+    passes[3 * i + 1] = p[i].F;
+    passes[3 * i + 2] = (int32_t)p[i].kernel;
+  }
+  return FLOCK_OK;
+}
+
+extern "C" int flock_ntt_plans_load(flock_ctx *ctx, const char *path) {
+  std::string err;
+  const int rc = flock::ntt_plans_load(path, err);
+  if (rc && ctx) ctx->c.last_error = err;
+  return rc;
+}
+
 extern "C" int flock_commit(flock_ctx *ctx, const flock_pcs_params *params, const void *z_packed, size_t z_bytes,
                             uint8_t root[32]) {
   if (int u = usable(ctx)) return u;

@@ -1,0 +1,722 @@
+// Interleaved additive NTT of the PCS commit on gfx950 (64 RS lanes).
+//
+// Layout in HBM (the reference's, crates/flock-core/src/pcs/commit.rs:235-245):
+// codeword[pos * 64 + lane] F128.
+//
+// A pass applies F consecutive layers [l, l+F) of the neighbors-last forward
+// transform (ntt/additive_ntt_f128.rs:1281-1320).  The 2^F positions that
+// exchange data in those layers share every position bit except F contiguous
+// ones, so they form a "group"
+//     pos(e) = (B << (k-l)) | (e << s) | r,   s = k - l - F,  e in [0, 2^F).
+// One wavefront owns one group: lane = RS lane, and each lane keeps its 2^F
+// elements in VGPRs, so every butterfly is lane-local and the global loads
+// and stores are 1 KiB-contiguous per position (fully coalesced).  The
+// twiddle of a butterfly depends only on (layer, block), i.e. it is
+// wavefront-uniform: its products run through lookup tables of that twiddle
+// in LDS, full byte tables shared by a workgroup (fulltab.h) where the
+// twiddle meets many products, per-wavefront nibble tables (nibtab.h) in the
+// last layers.
+//
+// Several kernels can run a pass (NttKernel); which ones, and how the layers
+// are split into passes, is a plan per transform shape (k, start layer).
+// ntt_autotune times every feasible (layer, F, kernel) on the device and
+// picks the fastest partition by dynamic programming; tuned plans are kept
+// in a process-wide table, loaded at first use from a JSON file beside the
+// library (tune/ntt_plans_gfx950.json), and any valid plan gives the same
+// codeword.
+#include <hip/hip_runtime.h>
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/flock_gpu.h"
+#include "commit.h"
+#include "ctx.h"
+#include "fulltab.h"
+#include "fused_job.h"
+#include "gf128_dev.h"
+#include "nibtab.h"
+
+namespace flock {
+
+// ---------------------------------------------------------------- tail passes
+// Layers whose twiddles are shared by too few groups to pay for full byte
+// tables (the last layers of the transform, where a twiddle meets 64 .. 512
+// products).  One wavefront per group; it builds the 8 KiB nibble tables of
+// its own twiddles, NB at a time, in NB * 8 KiB of LDS (NB = 2: two products
+// by different twiddles in flight; NB = 1: twice the wavefronts per CU).
+// Step (J, C) handles twiddles C .. C + NT - 1 of layer offset J; template
+// recursion keeps every register index constant.
+template <int F, int NB, int J, int C>
+__device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__restrict__ tw, int l, uint64_t B,
+                                               char *lds, const NibtabLane &NL, uint32_t lane) {
+  constexpr int NT = (NB == 2 && (1 << J) - C >= 2) ? 2 : 1;
+  constexpr int nb = 1 << (F - 1 - J);
+  const uint64_t tb = (1ull << (l + J)) - 1ull + (B << J);
+  __syncthreads();  // the previous step's tables are read
+  build_nibtab<0>(tw[tb + C], lds, lane);
+  if constexpr (NT == 2) build_nibtab<NIBTAB_BYTES>(tw[tb + C + 1], lds, lane);
+  __syncthreads();
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int el = 0; el < nb; ++el) {
+    const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+    const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<0>(v[e1], lds, NL));
+    v[e0] = u0;
+    f128_xor_into(v[e1], u0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (NT == 2) {
+#pragma unroll
+    for (int el = 0; el < nb; ++el) {
+      const int e0 = ((C + 1) << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<NIBTAB_BYTES>(v[e1], lds, NL));
+      v[e0] = u0;
+      f128_xor_into(v[e1], u0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if constexpr (C + NT < (1 << J))
+    tail_pass_step<F, NB, J, C + NT>(v, tw, l, B, lds, NL, lane);
+  else if constexpr (J + 1 < F)
+    tail_pass_step<F, NB, J + 1, 0>(v, tw, l, B, lds, NL, lane);
+}
+
+// src is the codeword itself, or (first pass) the message, read with
+// positions masked by src_mask: position p of the replicated codeword holds
+// message position p & src_mask, the exact state after the replication layers
+// (pcs/commit.rs:451-467).
+template <int F, int NB>
+__device__ __forceinline__ void ntt_tail_body(f128 *data, const f128 *__restrict__ tw, int k, int l, const f128 *src,
+                                              uint64_t src_mask, uint64_t g, char *lds) {
+  const uint32_t lane = threadIdx.x;
+  const NibtabLane NL(lane & 15u);
+  const int s = k - l - F;
+  const uint64_t r = g & ((1ull << s) - 1ull);
+  const uint64_t B = g >> s;
+  const uint64_t base = (B << (k - l)) | r;
+  f128 v[1 << F];
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_mask) << 6) + lane];
+  tail_pass_step<F, NB, 0, 0>(v, tw, l, B, lds, NL, lane);
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
+}
+
+template <int F, int NB>
+__global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *data, const f128 *__restrict__ tw, int k, int l,
+                                                      const f128 *src, uint64_t src_mask) {
+  __shared__ __align__(16) char lds[NB * NIBTAB_BYTES];
+  ntt_tail_body<F, NB>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+}
+
+template <int F, int NB>
+__global__ __launch_bounds__(64) void ntt_tail_batch_kernel(const NttJob *__restrict__ jobs, uint32_t n_jobs) {
+  __shared__ __align__(16) char lds[NB * NIBTAB_BYTES];
+  const NttJob &J = job_of(jobs, n_jobs);
+  ntt_tail_body<F, NB>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data, J.src ? J.src_pos_mask : ~0ull,
+                       blockIdx.x - J.wg_begin, lds);
+}
+
+// ---------------------------------------------------------------- shared-table passes
+// When 2^s >= WAVES * G groups share the block B of the pass's first layer,
+// they share all 2^F - 1 twiddles of the pass.  A workgroup of WAVES
+// wavefronts takes WAVES * G such groups (G per wavefront).  Up front its
+// threads compute the 128 values t x^i of every twiddle t of the pass; then
+// for each twiddle they build the full byte tables T_k[b] = (b x^(8k)) t,
+// k = 0..15 (64 KiB), and a product v t is 16 independent lookups, one per
+// byte of v, with no shifts.  Two table buffers alternate, so one barrier per
+// twiddle suffices: the buffer being rebuilt was last read before the
+// previous barrier.
+constexpr int SHARED_TWIDDLES = 15;  // 2^F - 1 for F <= 4
+
+// Twiddle n = 2^J - 1 + C of a shared-table pass (layer offset J, block C):
+// build its tables into buffer n & 1, then every butterfly of it in each of
+// the wave's G groups, two products in flight per lane (one with 16
+// wavefronts per workgroup, whose 128-VGPR budget holds no more).
+template <int F, int G, int THREADS, int J, int C>
+__device__ __forceinline__ void shared_pass_step(f128 (&v)[G][1 << F], f128 *tabs, const f128 *basis,
+                                                 const FulltabLane &FL, uint32_t tid) {
+  constexpr int N = (1 << J) - 1 + C;
+  constexpr bool HI = N & 1;
+  const char *tbase = (const char *)tabs;  // buffer 0 at +0, buffer 1 at +64 KiB
+  build_fulltab<THREADS>(basis + N * 128, tabs + (HI ? 4096 : 0), tid);
+  __syncthreads();
+  constexpr int nb = 1 << (F - 1 - J);
+  // product slots (group, butterfly) in pairs
+  constexpr int SLOTS = G * nb;
+  constexpr int STEP = THREADS > 512 ? 1 : 2;
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl += STEP) {
+    const int ga = sl / nb, la = sl % nb, gb = (sl + 1) / nb, lb = (sl + 1) % nb;
+    const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
+    if (STEP == 2 && sl + 1 < SLOTS) {
+      const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
+      f128 p0, p1;
+      f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
+      const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
+      v[ga][a0] = u0;
+      f128_xor_into(v[ga][a1], u0);
+      v[gb][b0] = u1;
+      f128_xor_into(v[gb][b1], u1);
+    } else {
+      const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
+      v[ga][a0] = u0;
+      f128_xor_into(v[ga][a1], u0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (C + 1 < (1 << J))
+    shared_pass_step<F, G, THREADS, J, C + 1>(v, tabs, basis, FL, tid);
+  else if constexpr (J + 1 < F)
+    shared_pass_step<F, G, THREADS, J + 1, 0>(v, tabs, basis, FL, tid);
+}
+
+constexpr int SHARED_LDS_F128 = 2 * 4096 + SHARED_TWIDDLES * 128;
+
+template <int F, int WAVES, int G>
+__device__ __forceinline__ void ntt_pass_shared_body(f128 *data, const f128 *__restrict__ tw, int k, int l,
+                                                     const f128 *src, uint64_t src_mask, uint64_t wg, f128 *lds) {
+  constexpr uint32_t THREADS = 64 * WAVES;
+  static_assert((1 << F) - 1 <= SHARED_TWIDDLES, "twiddle basis area");
+  f128 *tabs = lds;                // 2 x 16 x 256
+  f128 *basis = lds + 2 * 4096;    // SHARED_TWIDDLES x 128
+  const uint32_t tid = threadIdx.x, lane = tid & 63, rot = lane & 15;
+  const FulltabLane FL(rot);
+  const int s = k - l - F;
+  // wavefront-uniform group indices (scalar registers for the addresses)
+  const uint64_t g0 = (wg * WAVES + __builtin_amdgcn_readfirstlane(tid >> 6)) * G;
+  const uint64_t B = g0 >> s;  // the same for the whole workgroup
+  // basis of every twiddle: entry n * 128 + i * 16 + kb = t_n x^(8 kb + i),
+  // t_n = twiddle n of the pass in (layer offset j, block c) order,
+  // n = 2^j - 1 + c
+  for (uint32_t e = tid; e < ((1u << F) - 1u) * 128u; e += THREADS) {
+    const uint32_t n = e >> 7, i = (e >> 4) & 7, kb = e & 15;
+    const uint32_t j = 31 - __builtin_clz(n + 1);
+    const uint32_t c = n + 1 - (1u << j);
+    basis[e] = f128_mul_xpow(tw[(1ull << (l + j)) - 1ull + (B << j) + c], 8 * kb + i);
+  }
+  f128 v[G][1 << F];
+  uint64_t base[G];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const uint64_t g = g0 + gi;
+    base[gi] = (B << (k - l)) | (g & ((1ull << s) - 1ull));
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e) v[gi][e] = src[(((base[gi] | ((uint64_t)e << s)) & src_mask) << 6) + lane];
+  }
+  __syncthreads();  // basis visible
+  shared_pass_step<F, G, THREADS, 0, 0>(v, tabs, basis, FL, tid);
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e) data[((base[gi] | ((uint64_t)e << s)) << 6) + lane] = v[gi][e];
+}
+
+template <int F, int WAVES, int G>
+__global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *data, const f128 *__restrict__ tw, int k,
+                                                                     int l, const f128 *src, uint64_t src_mask) {
+  __shared__ f128 lds[SHARED_LDS_F128];
+  ntt_pass_shared_body<F, WAVES, G>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+}
+
+template <int F, int WAVES, int G>
+__global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_batch_kernel(const NttJob *__restrict__ jobs,
+                                                                           uint32_t n_jobs) {
+  __shared__ f128 lds[SHARED_LDS_F128];
+  const NttJob &J = job_of(jobs, n_jobs);
+  ntt_pass_shared_body<F, WAVES, G>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data, J.src ? J.src_pos_mask : ~0ull,
+                                    blockIdx.x - J.wg_begin, lds);
+}
+
+// ---------------------------------------------------------------- nibble-table shared passes
+// The 2^F - 1 twiddles of the pass are shared by the workgroup's WAVES
+// groups (2^s >= WAVES); their nibble tables (8 KiB each, 120 KiB for F = 4)
+// are built once, up front, wavefront n building twiddle n's, and then every
+// butterfly of the pass runs with no further barrier.  A product is 32
+// lookups (16 with full byte tables), but nothing is rebuilt per twiddle, and
+// a lane's 128-VGPR budget lets 16 wavefronts share a CU.
+template <int F, int J, int C>
+__device__ __forceinline__ void nibshared_step(f128 (&v)[1 << F], const char *lds, const NibtabLane &NL) {
+  constexpr int N = (1 << J) - 1 + C;
+  constexpr int nb = 1 << (F - 1 - J);
+#pragma unroll
+  for (int el = 0; el < nb; ++el) {
+    const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+    const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<(uint32_t)N * NIBTAB_BYTES>(v[e1], lds, NL));
+    v[e0] = u0;
+    f128_xor_into(v[e1], u0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (C + 1 < (1 << J))
+    nibshared_step<F, J, C + 1>(v, lds, NL);
+  else if constexpr (J + 1 < F)
+    nibshared_step<F, J + 1, 0>(v, lds, NL);
+}
+
+template <int F, int WAVES>
+__device__ __forceinline__ void ntt_pass_nibshared_body(f128 *data, const f128 *__restrict__ tw, int k, int l,
+                                                        const f128 *src, uint64_t src_mask, uint64_t wg, char *lds) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const NibtabLane NL(lane & 15u);
+  const int s = k - l - F;
+  const uint64_t g = wg * WAVES + wave;
+  const uint64_t B = g >> s;  // the same for the whole workgroup
+  const uint64_t base = (B << (k - l)) | (g & ((1ull << s) - 1ull));
+  f128 v[1 << F];
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_mask) << 6) + lane];
+  // twiddle n = 2^j - 1 + c of the pass (layer offset j, block c)
+  for (uint32_t n = wave; n < (1u << F) - 1u; n += WAVES) {
+    const uint32_t j = 31 - __builtin_clz(n + 1), c = n + 1 - (1u << j);
+    build_nibtab<0>(tw[(1ull << (l + j)) - 1ull + (B << j) + c], lds + n * NIBTAB_BYTES, lane);
+  }
+  __syncthreads();
+  nibshared_step<F, 0, 0>(v, lds, NL);
+#pragma unroll
+  for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
+}
+
+template <int F, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ntt_pass_nibshared_kernel(f128 *data, const f128 *__restrict__ tw,
+                                                                        int k, int l, const f128 *src,
+                                                                        uint64_t src_mask) {
+  __shared__ __align__(16) char lds[((1 << F) - 1) * NIBTAB_BYTES];
+  ntt_pass_nibshared_body<F, WAVES>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+}
+
+template <int F, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ntt_pass_nibshared_batch_kernel(const NttJob *__restrict__ jobs,
+                                                                              uint32_t n_jobs) {
+  __shared__ __align__(16) char lds[((1 << F) - 1) * NIBTAB_BYTES];
+  const NttJob &J = job_of(jobs, n_jobs);
+  ntt_pass_nibshared_body<F, WAVES>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data,
+                                    J.src ? J.src_pos_mask : ~0ull, blockIdx.x - J.wg_begin, lds);
+}
+
+// ---------------------------------------------------------------- launch
+namespace {
+
+struct PassArgs {
+  f128 *data;
+  const f128 *tw;
+  int k, l;
+  const f128 *src;
+  uint64_t mask;
+};
+
+template <int F, int W, int G>
+void launch_shared(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *jobs, uint32_t n_jobs) {
+  if (jobs)
+    hipLaunchKernelGGL((ntt_pass_shared_batch_kernel<F, W, G>), grid, dim3(64 * W), 0, st, jobs, n_jobs);
+  else
+    hipLaunchKernelGGL((ntt_pass_shared_kernel<F, W, G>), grid, dim3(64 * W), 0, st, a.data, a.tw, a.k, a.l, a.src,
+                       a.mask);
+}
+
+template <int F, int W>
+void launch_nibshared(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *jobs, uint32_t n_jobs) {
+  if (jobs)
+    hipLaunchKernelGGL((ntt_pass_nibshared_batch_kernel<F, W>), grid, dim3(64 * W), 0, st, jobs, n_jobs);
+  else
+    hipLaunchKernelGGL((ntt_pass_nibshared_kernel<F, W>), grid, dim3(64 * W), 0, st, a.data, a.tw, a.k, a.l, a.src,
+                       a.mask);
+}
+
+template <int F, int NB>
+void launch_tail(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *jobs, uint32_t n_jobs) {
+  if (jobs)
+    hipLaunchKernelGGL((ntt_tail_batch_kernel<F, NB>), grid, dim3(64), 0, st, jobs, n_jobs);
+  else
+    hipLaunchKernelGGL((ntt_tail_kernel<F, NB>), grid, dim3(64), 0, st, a.data, a.tw, a.k, a.l, a.src, a.mask);
+}
+
+const char *kernel_phase(NttKernel kn) {
+  return kn == NTT_TAIL_NB2 || kn == NTT_TAIL_NB1 ? "ntt_tail_kernel" : "ntt_pass_shared_kernel";
+}
+
+// One pass, directly (jobs null) or as a fused launch over jobs.
+int launch_pass(Ctx *ctx, hipStream_t st, int F, NttKernel kn, uint64_t wgs, const PassArgs &a, const NttJob *jobs,
+                uint32_t n_jobs) {
+  ScopedPhase ph(ctx, kernel_phase(kn), st);
+  const dim3 grid((unsigned)wgs);
+  switch ((int)kn * 8 + F) {
+#define FLOCK_SHARED(KN, F_, W, G) \
+  case (int)KN * 8 + F_: launch_shared<F_, W, G>(grid, st, a, jobs, n_jobs); break;
+#define FLOCK_TAIL(KN, F_, NB) \
+  case (int)KN * 8 + F_: launch_tail<F_, NB>(grid, st, a, jobs, n_jobs); break;
+    FLOCK_SHARED(NTT_SHARED_W8G2, 3, 8, 2)
+    FLOCK_SHARED(NTT_SHARED_W8G2, 4, 8, 2)
+    FLOCK_SHARED(NTT_SHARED_W8G1, 3, 8, 1)
+    FLOCK_SHARED(NTT_SHARED_W8G1, 4, 8, 1)
+    FLOCK_SHARED(NTT_SHARED_W16G1, 3, 16, 1)
+    FLOCK_SHARED(NTT_SHARED_W16G1, 4, 16, 1)
+    FLOCK_TAIL(NTT_TAIL_NB2, 1, 2)
+    FLOCK_TAIL(NTT_TAIL_NB2, 2, 2)
+    FLOCK_TAIL(NTT_TAIL_NB2, 3, 2)
+    FLOCK_TAIL(NTT_TAIL_NB2, 4, 2)
+    FLOCK_TAIL(NTT_TAIL_NB1, 1, 1)
+    FLOCK_TAIL(NTT_TAIL_NB1, 2, 1)
+    FLOCK_TAIL(NTT_TAIL_NB1, 3, 1)
+    FLOCK_TAIL(NTT_TAIL_NB1, 4, 1)
+#define FLOCK_NIBSHARED(KN, F_, W) \
+  case (int)KN * 8 + F_: launch_nibshared<F_, W>(grid, st, a, jobs, n_jobs); break;
+    FLOCK_NIBSHARED(NTT_NIBSHARED_W16, 3, 16)
+    FLOCK_NIBSHARED(NTT_NIBSHARED_W16, 4, 16)
+    FLOCK_NIBSHARED(NTT_NIBSHARED_W8, 3, 8)
+    FLOCK_NIBSHARED(NTT_NIBSHARED_W8, 4, 8)
+#undef FLOCK_SHARED
+#undef FLOCK_TAIL
+#undef FLOCK_NIBSHARED
+    default: ctx->last_error = "ntt: no kernel for this pass"; return FLOCK_E_ARG;
+  }
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- plans
+int ntt_kernel_groups_log(NttKernel kn) {
+  switch (kn) {
+    case NTT_SHARED_W8G2: return 4;
+    case NTT_SHARED_W8G1: return 3;
+    case NTT_SHARED_W16G1: return 4;
+    case NTT_NIBSHARED_W16: return 4;
+    case NTT_NIBSHARED_W8: return 3;
+    default: return 0;
+  }
+}
+
+bool ntt_pass_feasible(int k, int l, int F, int kn) {
+  if (l < 0 || F < 1 || F > 4 || l + F > k || kn < 0 || kn >= NTT_KERNELS) return false;
+  if (kn == NTT_TAIL_NB2 || kn == NTT_TAIL_NB1) return true;
+  return F >= 3 && k - l - F >= ntt_kernel_groups_log((NttKernel)kn);
+}
+
+uint64_t ntt_pass_wgs(int k, int F, NttKernel kn) { return (1ull << (k - F)) >> ntt_kernel_groups_log(kn); }
+
+std::vector<NttPass> ntt_default_plan(int k, int start) {
+  std::vector<NttPass> p;
+  for (int l = start; l < k;) {
+    const int F = std::min(4, k - l), s = k - l - F;
+    const NttKernel kn = F == 4 && s >= 4 ? NTT_SHARED_W8G2 : F == 4 && s == 3 ? NTT_SHARED_W8G1 : NTT_TAIL_NB2;
+    p.push_back({l, F, kn});
+    l += F;
+  }
+  return p;
+}
+
+bool ntt_plan_valid(int k, int start, const std::vector<NttPass> &p) {
+  if (start < 0 || k > 40 || start >= k) return p.empty();
+  int l = start;
+  for (const NttPass &q : p) {
+    if (q.l != l || !ntt_pass_feasible(k, q.l, q.F, q.kernel)) return false;
+    l += q.F;
+  }
+  return l == k;
+}
+
+namespace {
+
+std::mutex g_plan_mu;
+std::map<std::pair<int, int>, std::vector<NttPass>> g_plans;
+bool g_plans_loaded = false;
+
+std::string default_plan_path() {
+  if (const char *e = getenv("FLOCK_NTT_PLANS")) return e;  // empty: no file
+  Dl_info info;
+  if (!dladdr((void *)&ntt_default_plan, &info) || !info.dli_fname) return "";
+  std::string so = info.dli_fname;
+  const size_t slash = so.rfind('/');
+  const std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+  return dir + "/../tune/ntt_plans_gfx950.json";
+}
+
+bool parse_int(const std::string &t, size_t &i, long &out) {
+  while (i < t.size() && (t[i] == ' ' || t[i] == '\n' || t[i] == '\t' || t[i] == '\r' || t[i] == ':' || t[i] == ','))
+    ++i;
+  size_t j = i;
+  if (j < t.size() && t[j] == '-') ++j;
+  if (j >= t.size() || t[j] < '0' || t[j] > '9') return false;
+  char *end = nullptr;
+  out = strtol(t.c_str() + i, &end, 10);
+  i = (size_t)(end - t.c_str());
+  return true;
+}
+
+// The plans of a file {"plans": [{"k": K, "start": S, "passes": [[l, F,
+// kernel], ...], ...}, ...]} (other keys ignored).
+bool parse_plans(const std::string &t, std::map<std::pair<int, int>, std::vector<NttPass>> &out, std::string &err) {
+  size_t at = t.find("\"plans\"");
+  if (at == std::string::npos) {
+    err = "no \"plans\" key";
+    return false;
+  }
+  for (size_t ob = t.find('{', at); ob != std::string::npos; ob = t.find('{', ob + 1)) {
+    const size_t cb = t.find('}', ob);
+    if (cb == std::string::npos) {
+      err = "unterminated plan object";
+      return false;
+    }
+    const std::string o = t.substr(ob, cb - ob + 1);
+    long k = -1, start = -1;
+    size_t i = o.find("\"k\"");
+    if (i == std::string::npos || !parse_int(o, i += 3, k)) {
+      err = "plan without \"k\"";
+      return false;
+    }
+    i = o.find("\"start\"");
+    if (i == std::string::npos || !parse_int(o, i += 7, start)) {
+      err = "plan without \"start\"";
+      return false;
+    }
+    i = o.find("\"passes\"");
+    if (i == std::string::npos || (i = o.find('[', i)) == std::string::npos) {
+      err = "plan without \"passes\"";
+      return false;
+    }
+    ++i;
+    std::vector<NttPass> p;
+    while (true) {
+      while (i < o.size() && (o[i] == ' ' || o[i] == ',' || o[i] == '\n' || o[i] == '\t' || o[i] == '\r')) ++i;
+      if (i >= o.size()) {
+        err = "unterminated passes";
+        return false;
+      }
+      if (o[i] == ']') break;
+      if (o[i] != '[') {
+        err = "a pass is not [l, F, kernel]";
+        return false;
+      }
+      ++i;
+      long l, F, kn;
+      if (!parse_int(o, i, l) || !parse_int(o, i, F) || !parse_int(o, i, kn)) {
+        err = "a pass is not [l, F, kernel]";
+        return false;
+      }
+      while (i < o.size() && (o[i] == ' ' || o[i] == '\n' || o[i] == '\t' || o[i] == '\r')) ++i;
+      if (i >= o.size() || o[i] != ']') {
+        err = "a pass is not [l, F, kernel]";
+        return false;
+      }
+      ++i;
+      p.push_back({(int)l, (int)F, (NttKernel)kn});
+    }
+    if (!ntt_plan_valid((int)k, (int)start, p)) {
+      err = "invalid plan for k=" + std::to_string(k) + " start=" + std::to_string(start);
+      return false;
+    }
+    out[{(int)k, (int)start}] = p;
+    ob = cb;
+  }
+  return true;
+}
+
+int load_plans_locked(const std::string &path, std::string *err) {
+  if (path.empty()) return FLOCK_OK;
+  std::ifstream f(path);
+  if (!f) {
+    if (err) *err = "cannot open " + path;
+    return err ? FLOCK_E_ARG : FLOCK_OK;  // the default file is optional
+  }
+  std::stringstream ss;
+  ss << f.rdbuf();
+  std::map<std::pair<int, int>, std::vector<NttPass>> plans;
+  std::string e;
+  if (!parse_plans(ss.str(), plans, e)) {
+    if (err) *err = path + ": " + e;
+    return FLOCK_E_ARG;
+  }
+  for (auto &kv : plans) g_plans[kv.first] = kv.second;
+  return FLOCK_OK;
+}
+
+void ensure_loaded_locked() {
+  if (!g_plans_loaded) {
+    g_plans_loaded = true;
+    load_plans_locked(default_plan_path(), nullptr);
+  }
+}
+
+}  // namespace
+
+std::vector<NttPass> ntt_plan(int k, int start) {
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  ensure_loaded_locked();
+  auto it = g_plans.find({k, start});
+  return it != g_plans.end() ? it->second : ntt_default_plan(k, start);
+}
+
+int ntt_plan_install(int k, int start, const std::vector<NttPass> &p, std::string &err) {
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  ensure_loaded_locked();
+  if (p.empty()) {
+    g_plans.erase({k, start});
+    return FLOCK_OK;
+  }
+  if (!ntt_plan_valid(k, start, p)) {
+    err = "ntt plan: passes must cover layers start..k-1 in order, each with a kernel that can run it";
+    return FLOCK_E_ARG;
+  }
+  g_plans[{k, start}] = p;
+  return FLOCK_OK;
+}
+
+int ntt_plans_load(const char *path, std::string &err) {
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  g_plans_loaded = true;
+  std::string e;
+  const int rc = load_plans_locked(path ? std::string(path) : default_plan_path(), &e);
+  if (rc) err = "ntt plans: " + e;
+  return rc;
+}
+
+// ---------------------------------------------------------------- transform
+int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k, int num_lanes, int start_layer,
+                                   const void *d_msg, const void *d_twiddles) {
+  if (num_lanes != 64) {
+    ctx->last_error = "ntt: only 64 interleaved lanes are implemented";
+    return FLOCK_E_SHAPE;
+  }
+  const F128 *tw = d_twiddles ? (const F128 *)d_twiddles : ctx->device_twiddles(k);
+  if (!tw) return FLOCK_E_DEVICE;
+  const std::vector<NttPass> plan = ntt_plan(k, start_layer);
+  bool first = true;
+  for (const NttPass &q : plan) {
+    const bool from_msg = first && d_msg;
+    const PassArgs a{(f128 *)d_data, (const f128 *)tw, k, q.l, from_msg ? (const f128 *)d_msg : (const f128 *)d_data,
+                     from_msg ? (1ull << (k - start_layer)) - 1 : ~0ull};
+    if (int rc = launch_pass(ctx, st, q.F, q.kernel, ntt_pass_wgs(k, q.F, q.kernel), a, nullptr, 0)) return rc;
+    first = false;
+  }
+  return FLOCK_OK;
+}
+
+int ntt_batch_device(Ctx *ctx, hipStream_t st, int F, NttKernel kn, const NttJob *d_jobs, uint32_t n_jobs,
+                     uint64_t wgs) {
+  const PassArgs none{nullptr, nullptr, 0, 0, nullptr, 0};
+  return launch_pass(ctx, st, F, kn, wgs, none, d_jobs, n_jobs);
+}
+
+// ---------------------------------------------------------------- autotuning
+// Pseudo-random contents for the timed buffers (SplitMix64 of the word
+// index): table lookups address by data bytes, so constant data would time
+// a pattern no codeword has.
+__global__ void fill_random_kernel(uint64_t *p, uint64_t n_words, uint64_t seed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    p[i] = z ^ (z >> 31);
+  }
+}
+
+int ntt_autotune(Ctx *ctx, int k, int start, int reps, std::string &json) {
+  if (k < 2 || k > 30 || start < 1 || start >= k || reps < 1) {
+    ctx->last_error = "ntt autotune: needs 1 <= start < k <= 30 and reps >= 1";
+    return FLOCK_E_ARG;
+  }
+  const F128 *tw = ctx->device_twiddles(k);
+  if (!tw) return FLOCK_E_DEVICE;
+  hipStream_t st = ctx->stream;
+  const size_t cw_bytes = (size_t)1024 << k, msg_bytes = cw_bytes >> start;
+  void *cw = nullptr, *msg = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FLOCK_OK;
+  // t[l][F][kernel] in ms (< 0: infeasible)
+  std::vector<double> t((size_t)(k + 1) * 5 * NTT_KERNELS, -1.0);
+  auto T = [&](int l, int F, int kn) -> double & { return t[((size_t)l * 5 + F) * NTT_KERNELS + kn]; };
+  std::ostringstream timings;
+  bool first_timing = true;
+  const bool was_timing = ctx->timing;
+  ctx->timing = false;
+  do {
+    if (hipMalloc(&cw, cw_bytes) != hipSuccess || hipMalloc(&msg, msg_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->last_error = "ntt autotune: out of device memory";
+      rc = FLOCK_E_OOM;
+      break;
+    }
+    hipLaunchKernelGGL(fill_random_kernel, dim3(4096), dim3(256), 0, st, (uint64_t *)cw, (uint64_t)(cw_bytes / 8),
+                       0x5eed0001ull);
+    hipLaunchKernelGGL(fill_random_kernel, dim3(4096), dim3(256), 0, st, (uint64_t *)msg, (uint64_t)(msg_bytes / 8),
+                       0x5eed0002ull);
+    if (hipGetLastError() != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      ctx->last_error = "ntt autotune: setup failed";
+      rc = FLOCK_E_DEVICE;
+      break;
+    }
+    for (int l = start; l < k && !rc; ++l)
+      for (int F = 1; F <= 4 && !rc; ++F)
+        for (int kn = 0; kn < NTT_KERNELS && !rc; ++kn) {
+          if (!ntt_pass_feasible(k, l, F, kn)) continue;
+          const bool from_msg = l == start;
+          const PassArgs a{(f128 *)cw, (const f128 *)tw, k, l, from_msg ? (const f128 *)msg : (const f128 *)cw,
+                           from_msg ? (1ull << (k - start)) - 1 : ~0ull};
+          const uint64_t wgs = ntt_pass_wgs(k, F, (NttKernel)kn);
+          if ((rc = launch_pass(ctx, st, F, (NttKernel)kn, wgs, a, nullptr, 0))) break;  // warm-up
+          if (hipEventRecord(e0, st) != hipSuccess) rc = FLOCK_E_DEVICE;
+          for (int r = 0; r < reps && !rc; ++r) rc = launch_pass(ctx, st, F, (NttKernel)kn, wgs, a, nullptr, 0);
+          if (rc || hipEventRecord(e1, st) != hipSuccess) {
+            rc = rc ? rc : FLOCK_E_DEVICE;
+            break;
+          }
+          if ((rc = ctx->wait_event(e1))) break;
+          float ms = 0.f;
+          if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = FLOCK_E_DEVICE;
+            break;
+          }
+          T(l, F, kn) = ms / reps;
+          timings << (first_timing ? "" : ", ") << "[" << l << ", " << F << ", " << kn << ", " << ms / reps << "]";
+          first_timing = false;
+        }
+  } while (false);
+  ctx->timing = was_timing;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (cw || msg) (void)hipStreamSynchronize(st);
+  if (cw) (void)hipFree(cw);
+  if (msg) (void)hipFree(msg);
+  if (rc) return rc;
+  // fastest partition: best[l] = min over passes (l, F, kernel) of its time
+  // plus best[l + F]
+  std::vector<double> best(k + 1, 1e30);
+  std::vector<NttPass> choice(k + 1);
+  best[k] = 0.0;
+  for (int l = k - 1; l >= start; --l)
+    for (int F = 1; F <= 4 && l + F <= k; ++F)
+      for (int kn = 0; kn < NTT_KERNELS; ++kn) {
+        const double x = T(l, F, kn);
+        if (x < 0 || best[l + F] >= 1e30) continue;
+        if (x + best[l + F] < best[l]) {
+          best[l] = x + best[l + F];
+          choice[l] = {l, F, (NttKernel)kn};
+        }
+      }
+  std::vector<NttPass> plan;
+  for (int l = start; l < k; l += choice[l].F) plan.push_back(choice[l]);
+  double def_ms = 0.0;
+  for (const NttPass &q : ntt_default_plan(k, start)) def_ms += T(q.l, q.F, q.kernel);
+  if ((rc = ntt_plan_install(k, start, plan, ctx->last_error))) return rc;
+  std::ostringstream o;
+  o << "{\"k\": " << k << ", \"start\": " << start << ", \"passes\": [";
+  for (size_t i = 0; i < plan.size(); ++i)
+    o << (i ? ", " : "") << "[" << plan[i].l << ", " << plan[i].F << ", " << (int)plan[i].kernel << "]";
+  o << "], \"ms\": " << best[start] << ", \"default_ms\": " << def_ms << ", \"timings\": [" << timings.str() << "]}";
+  json = o.str();
+  return FLOCK_OK;
+}
+
+}  // namespace flock

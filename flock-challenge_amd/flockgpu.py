@@ -132,6 +132,11 @@ def lib():
         L.flock_prove_batch.argtypes = [P, P, U32, P, SZ, U32, ctypes.c_int, P, SZ, P, P, P, P]
         L.flock_statement_setup.argtypes = [P, ctypes.c_char_p, SZ]
         L.flock_server_set_wait_timeout.argtypes = [P, ctypes.c_double]
+        PI32 = ctypes.POINTER(ctypes.c_int32)
+        L.flock_ntt_autotune.argtypes = [P, ctypes.POINTER(PcsParams), ctypes.c_int, ctypes.c_char_p, SZ]
+        L.flock_ntt_plan_set.argtypes = [P, U32, U32, PI32, U32]
+        L.flock_ntt_plan_get.argtypes = [P, U32, U32, PI32, U32, ctypes.POINTER(U32)]
+        L.flock_ntt_plans_load.argtypes = [P, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -149,11 +154,45 @@ def exported_symbols():
             "flock_server_submit_seeded", "flock_server_poll", "flock_server_pending",
             "flock_ctx_set_wait_timeout", "flock_server_set_wait_timeout", "flock_commit_device_noalloc",
             "flock_statement_digest", "flock_statement_setup", "flock_prove_batch", "flock_server_stats",
-            "flock_ctx_set_option"]
+            "flock_ctx_set_option", "flock_ntt_autotune", "flock_ntt_plan_set", "flock_ntt_plan_get",
+            "flock_ntt_plans_load"]
 
 
 def device_count():
     return lib().flock_device_count()
+
+
+# Commit NTT plans (process-wide; no GPU needed).  Kernel ids: shared
+# full-table passes 8x2, 8x1, 16x1 (wavefronts x groups), tail passes with 2
+# or 1 nibble tables in flight, shared nibble-table passes of 16 or 8
+# wavefronts.
+(NTT_SHARED_W8G2, NTT_SHARED_W8G1, NTT_SHARED_W16G1, NTT_TAIL_NB2, NTT_TAIL_NB1, NTT_NIBSHARED_W16,
+ NTT_NIBSHARED_W8) = range(7)
+
+
+def ntt_plan_set(k, start, passes, ctx=None):
+    """Install a plan: passes = [(l, F, kernel), ...]; [] restores the default."""
+    flat = (ctypes.c_int32 * max(1, 3 * len(passes)))(*[int(x) for p in passes for x in p])
+    rc = lib().flock_ntt_plan_set(ctx.handle if ctx else None, k, start, flat, len(passes))
+    if rc:
+        raise FlockError(rc, lib().flock_last_error(ctx.handle).decode() if ctx else "invalid NTT plan")
+
+
+def ntt_plan_get(k, start):
+    """The plan in force for the transform (k, start): [(l, F, kernel), ...]."""
+    arr = (ctypes.c_int32 * 192)()
+    n = ctypes.c_uint32()
+    rc = lib().flock_ntt_plan_get(None, k, start, arr, 64, ctypes.byref(n))
+    if rc:
+        raise FlockError(rc, "flock_ntt_plan_get")
+    return [tuple(arr[3 * i:3 * i + 3]) for i in range(n.value)]
+
+
+def ntt_plans_load(path=None):
+    """Merge a plan file (None: the library's default file) into the table."""
+    rc = lib().flock_ntt_plans_load(None, path.encode() if path else None)
+    if rc:
+        raise FlockError(rc, f"cannot load NTT plans from {path or 'the default file'}")
 
 
 # The benchmark's Fiat-Shamir domain separator (the worker's FsChallenger).
@@ -267,6 +306,22 @@ class Context:
         buf = ctypes.create_string_buffer(16)
         self._check(lib().flock_statement_setup(self.handle, buf, len(buf)))
         return buf.value.decode()
+
+    # ---------------------------------------------------------------- NTT plans
+    def ntt_autotune(self, params, reps=3):
+        """Time every (layer, F, kernel) of the commit transform of `params`,
+        install the fastest plan; returns {"k", "start", "passes", "ms",
+        "default_ms", "timings"}."""
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(lib().flock_ntt_autotune(self.handle, ctypes.byref(params), int(reps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def ntt_plan_set(self, k, start, passes):
+        ntt_plan_set(k, start, passes, self)
+
+    def ntt_plan_get(self, k, start):
+        return ntt_plan_get(k, start)
 
     def timing(self, on=True, only=None):
         """Enable phase timing; `only` = iterable of phase names to time (None: all)."""

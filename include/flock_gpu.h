@@ -116,6 +116,35 @@ int flock_commit_device(flock_ctx *ctx, const flock_pcs_params *params, const vo
 int flock_commit_device_noalloc(flock_ctx *ctx, const flock_pcs_params *params, const void *d_z, void *d_codeword,
                                 void *d_tree, uint8_t root[32], void *stream);
 
+/* Schedules of the commit's forward transform.  No counterpart in the
+ * reference (its CPU transform has no launch shapes to choose).  The
+ * transform over 2^k positions from layer `start` (k = m - 13 +
+ * log_inv_rate, start = log_inv_rate) runs as passes, each applying F
+ * consecutive layers [l, l + F) with one of several kernels:
+ *   0, 1, 2: shared full-table passes (F = 3 or 4; 8 wavefronts x 2
+ *            groups, 8 x 1, 16 x 1; they need 2^(k - l - F) >= 16, 8, 16);
+ *   3, 4:    tail passes (F = 1..4; two or one nibble tables in flight);
+ *   5, 6:    shared nibble-table passes (F = 3 or 4; 16 or 8 wavefronts,
+ *            needing 2^(k - l - F) >= 16 or 8).
+ * Every valid plan gives the same codeword; only the time differs.  Plans
+ * live in a process-wide table, loaded at first use from $FLOCK_NTT_PLANS or
+ * else tune/ntt_plans_gfx950.json in the directory above the library's
+ * (JSON {"plans": [{"k": K, "start": S, "passes": [[l, F, kernel], ...]},
+ * ...]}); shapes without a plan use passes of 4 layers.
+ *
+ * flock_ntt_autotune times every feasible (l, F, kernel) of the transform
+ * `params` selects on the context's GPU (reps launches each), installs the
+ * fastest partition and writes it, with the timings, as a JSON object into
+ * json (FLOCK_E_CAP if cap is too small).  flock_ntt_plan_set installs a
+ * plan of n_passes (l, F, kernel) triples (n_passes = 0: back to the
+ * default), flock_ntt_plan_get reads the plan in force, flock_ntt_plans_load
+ * merges a plan file (NULL: the default path) into the table. */
+int flock_ntt_autotune(flock_ctx *ctx, const flock_pcs_params *params, int reps, char *json, size_t cap);
+int flock_ntt_plan_set(flock_ctx *ctx, uint32_t k, uint32_t start, const int32_t *passes, uint32_t n_passes);
+int flock_ntt_plan_get(flock_ctx *ctx, uint32_t k, uint32_t start, int32_t *passes, uint32_t cap,
+                       uint32_t *n_passes);
+int flock_ntt_plans_load(flock_ctx *ctx, const char *path);
+
 /* Device memory helpers for callers without their own HIP runtime handle
  * (tests, benchmarks, FFI hosts).  Pointers are device pointers on the
  * context's GPU; copies are synchronous with respect to the context stream. */

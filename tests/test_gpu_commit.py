@@ -55,3 +55,56 @@ def test_commit_rejects_bad_shapes(ctx):
         ctx.commit(p, np.zeros(10, np.uint64))
     with pytest.raises(flockgpu.FlockError):
         ctx.commit(flockgpu.PcsParams(20, 1, 6, 0, 0), np.zeros(2 << 13, np.uint64))
+
+
+# Plans for the transform of m = 24, rate 1/2 (k = 12 from layer 1) that
+# exercise every kernel and layer count: (l, F, kernel) passes.
+S82, S81, S161, T2, T1, N16, N8 = range(7)
+PLANS_K12 = {
+    "default": None,
+    "16-wave shared, one-table tail": [(1, 3, S161), (4, 4, S161), (8, 4, T1)],
+    "8x1 and 8x2 shared, mixed tails": [(1, 4, S81), (5, 3, S82), (8, 1, T1), (9, 2, T2), (11, 1, T1)],
+    "tail passes only, first reads the message": [(1, 2, T2), (3, 4, T1), (7, 4, T2), (11, 1, T2)],
+    "one layer per pass": [(l, 1, T1 if l % 2 else T2) for l in range(1, 12)],
+    "nibble-shared 16 and 8": [(1, 4, N16), (5, 4, N8), (9, 3, T1)],
+    "nibble-shared, three layers": [(1, 3, N8), (4, 3, N16), (7, 3, T2), (10, 2, T1)],
+}
+
+
+@pytest.mark.parametrize("name", list(PLANS_K12))
+def test_every_plan_gives_the_oracle_root(ctx, name):
+    m, rate, k = 24, 1, 12
+    print(f"commit plan '{name}' m={m}", flush=True)
+    z = gpu_util.commit_message(m - 7, seed=0x9147)
+    want, _, _ = O.pcs_commit(m, rate, 6, z)
+    prev = flockgpu.ntt_plan_get(k, rate)
+    try:
+        if PLANS_K12[name] is not None:
+            flockgpu.ntt_plan_set(k, rate, PLANS_K12[name], ctx)
+            assert flockgpu.ntt_plan_get(k, rate) == PLANS_K12[name]
+        assert ctx.commit(flockgpu.PcsParams(m, rate, 6, 0, 1), z) == want
+    finally:
+        flockgpu.ntt_plan_set(k, rate, prev)
+
+
+@pytest.mark.parametrize("m,rate", [(24, 1), (25, 2)])
+def test_autotuned_plan_is_installed_and_gives_the_oracle_root(ctx, m, rate):
+    k = m - 13 + rate
+    print(f"ntt autotune m={m} rate={rate}", flush=True)
+    z = gpu_util.commit_message(m - 7, seed=0x7111 + m)
+    want, _, _ = O.pcs_commit(m, rate, 6, z)
+    prev = flockgpu.ntt_plan_get(k, rate)
+    try:
+        r = ctx.ntt_autotune(flockgpu.PcsParams(m, rate, 6, 0, 1), reps=2)
+        assert (r["k"], r["start"]) == (k, rate)
+        passes = [tuple(p) for p in r["passes"]]
+        assert flockgpu.ntt_plan_get(k, rate) == passes
+        # the partition covers the layers in order and is no slower than the
+        # default's passes as measured in the same run
+        assert passes[0][0] == rate and sum(p[1] for p in passes) == k - rate
+        assert r["ms"] <= r["default_ms"] + 1e-9
+        measured = {(t[0], t[1], t[2]) for t in r["timings"]}
+        assert all(p in measured for p in passes)
+        assert ctx.commit(flockgpu.PcsParams(m, rate, 6, 0, 1), z) == want
+    finally:
+        flockgpu.ntt_plan_set(k, rate, prev)
