@@ -274,8 +274,8 @@ def bench_prove(args):
 
 def bench_sizes(args):
     """Single-proof latency by size (device-resident inputs, 1 warm-up and 5
-    timed proofs each, median), with the zerocheck transcript on the device
-    (default) and, for comparison, on the host."""
+    timed proofs each, median), with the zerocheck transcript on the host
+    (default) and, for comparison, on the device."""
     import flockgpu
     ctx = flockgpu.Context(0)
     out = {}
@@ -293,8 +293,8 @@ def bench_sizes(args):
                     s.prove_fast_device(buf.ptr, 1 << log2)
                     ts.append(time.perf_counter() - t0)
                 row["device_transcript_ms" if mode else "host_transcript_ms"] = 1000 * percentile(ts, 50)
-            ctx.set_option("device_transcript", 1)
-            row["compressions_per_s"] = (1 << log2) / (row["device_transcript_ms"] / 1000)
+            ctx.set_option("device_transcript", 0)
+            row["compressions_per_s"] = (1 << log2) / (row["host_transcript_ms"] / 1000)
             out[str(log2)] = row
             buf.free()
             s.close()

@@ -53,9 +53,10 @@ struct Ctx {
   // destroying it leaks its device memory instead of waiting on that work.
   double wait_timeout_s = 120.0;
   bool timed_out = false;
-  // Run the zerocheck rounds' Fiat-Shamir steps on the device (fs_dev.hip)
-  // instead of a host round trip per round; the proof bytes are the same.
-  bool device_transcript = true;
+  // Run the zerocheck rounds' Fiat-Shamir steps on the device
+  // (transcript_dev.h) instead of a host round trip per round; the proof
+  // bytes are the same.  Off by default: measured slower at every size.
+  bool device_transcript = false;
   // Capture the zerocheck rounds (device transcript only) in a HIP graph per
   // m and buffer set, replayed by later proofs.
   bool zc_graph = true;

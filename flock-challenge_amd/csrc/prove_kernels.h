@@ -29,11 +29,20 @@ namespace flock {
 size_t zc_tables_bytes();
 // Write seq to the mapped host word d_flag after the stream's earlier work.
 int signal_device(Ctx *ctx, hipStream_t st, void *d_flag, uint32_t seq);
+// A zerocheck round's transcript step (device transcript), run by the
+// kernel that reduces the round's message: absorb the message (G(1), G(inf))
+// into the transcript state and sample the round's challenge into rho (both
+// device memory, transcript_dev.h).  Null state: no step.
+struct FsStep {
+  void *state = nullptr;
+  void *rho = nullptr;
+};
 // out[w] = xor_r in[r * width + w] (width <= 4); a nonzero seq is then
 // published at the context's readback flag (see Ctx::READBACK_FLAG), which
 // the reductions ending in the readback area use in place of signal_device.
+// With fs.state (width 2) the kernel then runs the transcript step over out.
 int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out,
-                    uint32_t seq = 0);
+                    uint32_t seq = 0, const FsStep &fs = FsStep());
 // Zerocheck round 1, product part, in two steps: zc_y_device computes
 // Y_j(x_mid, x_hi) from the rows (no challenge involved), zc_round1_device
 // then d_q[4 x_mid + q4][8 j + t] (64 x 128 F128) = sum_{x_hi} Ehi(x_hi)
@@ -56,12 +65,13 @@ int eq_expand_device(Ctx *ctx, hipStream_t st, const void *lo, int k, const void
 // once the result (out2 / out3) is written (xor_rows_device).
 constexpr int ZS_LOG = 10;
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
-                          uint64_t half, void *out2, void *Ehi_next, uint32_t seq = 0);
+                          uint64_t half, void *out2, void *Ehi_next, uint32_t seq = 0, const FsStep &fs = FsStep());
 // Round 0 split at ZS_LOG + 1 over the packed rows a, b (A = L(row) through
 // the 256-entry nibble tables d_lw, entry v * 16 + p): Elo has 2^(ZS_LOG + 1)
 // entries, Ehi half >> (ZS_LOG + 1) (at least 2); Ehi is not folded.
 int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
-                           const void *Ehi, uint64_t half, void *out2, uint32_t seq = 0);
+                           const void *Ehi, uint64_t half, void *out2, uint32_t seq = 0,
+                           const FsStep &fs = FsStep());
 // The fold after zerocheck round 0, reading the packed rows a, b and
 // evaluating A = L(row) through 2048-entry byte tables (entry v * 8 + p):
 // d_lw8 of L and d_lwr8 of rho L.
@@ -81,7 +91,7 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
 constexpr uint64_t SMALL_PAIRS = 256;
 int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
                          uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq,
-                         const void *d_rho = nullptr);
+                         const void *d_rho = nullptr, const FsStep &fs = FsStep());
 int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half, void *f2,
                         void *b2, void *out3, uint32_t seq);
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,

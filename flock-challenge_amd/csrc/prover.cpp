@@ -734,9 +734,10 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
     // depends on the point only, so it is queued now
     if (n_split + 1 < n && (rc = P.eq_device(&r[8 + n_split], n - 2 - n_split, E))) return rc;
     // With the device transcript, the rounds' messages go to device memory,
-    // fs_round_kernel absorbs each and squeezes the round's challenge into
-    // device memory, and the folds read it from there: the rounds run with
-    // no host round trip, and the host reads everything back once.
+    // the kernel that reduces each absorbs it and squeezes the round's
+    // challenge into device memory, and the folds read it from there: the
+    // rounds run with no host round trip, and the host reads everything back
+    // once.
     const bool dfs = ctx->device_transcript;
     uint8_t *d_fs = nullptr, *d_msgs = nullptr, *d_rhos = nullptr, *d_ws = nullptr;
     if (dfs) {
@@ -787,28 +788,29 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
       const bool split = i <= n_split;
       const uint32_t seq = dfs ? 0 : ctx->next_readback_seq();
       void *out2 = dfs ? (void *)(d_msgs + 32 * (size_t)i) : ctx->readback_dev;
+      // device transcript: the round's reduction kernel absorbs the message
+      // and samples the challenge into rho_dev
+      const uint8_t *rho_dev = dfs ? d_rhos + 16 * (size_t)i : nullptr;
+      FsStep fs;
+      if (dfs) fs = FsStep{d_fs, (void *)rho_dev};
       if (fold_pending) {
         rc = zc_fold_round_device(ctx, st, A, Bv, prev_rho, E, half, A2, B2, i + 1 < n ? E2 : nullptr, out2, seq,
-                                  prev_rho_dev);
+                                  prev_rho_dev, fs);
         std::swap(A, A2);
         std::swap(Bv, B2);
         fold_pending = false;
       } else if (i == 0) {
-        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, out2, seq);
+        rc = zc_round0_split_device(ctx, st, a, b, dlw, dElo, Ehi, half, out2, seq, fs);
       } else {  // 1 <= i <= n_split: every later round is fused with its fold
         rc = zc_round_split_device(ctx, st, A, Bv,
                                    (const uint8_t *)dElo + (elo0_len + ((size_t)(i - 1) << ZS_LOG)) * 16, Ehi, half,
-                                   out2, i < n_split ? Ehi2 : nullptr, seq);
+                                   out2, i < n_split ? Ehi2 : nullptr, seq, fs);
         std::swap(Ehi, Ehi2);
       }
       if (rc) break;
       if (!split) std::swap(E, E2);
       F128 rho;
-      const uint8_t *rho_dev = nullptr;
-      if (dfs) {
-        rho_dev = d_rhos + 16 * (size_t)i;
-        if ((rc = fs_round_device(ctx, st, d_fs, out2, 2, (void *)rho_dev))) break;
-      } else {
+      if (!dfs) {
         uint4 h[2];
         if ((rc = P.fetch(h, 32, seq))) return rc;
         F128 g1 = fv(h[0]), gi = fv(h[1]);

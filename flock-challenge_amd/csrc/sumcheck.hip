@@ -19,6 +19,7 @@
 #include "gf128_dev.h"
 #include "nibtab.h"
 #include "prove_kernels.h"
+#include "transcript_dev.h"
 
 namespace flock {
 
@@ -109,12 +110,9 @@ __device__ __forceinline__ f128 lagrange_row(uint64_t row, const f128 *__restric
 // no per-product table build, and each thread keeps its 2^LOG / THREADS
 // x_lo's sums, weighted by Elo once at the end.  Also folds Ehi for the next
 // round, Ehi_next[y] = Ehi[2y] + Ehi[2y+1], unless Ehi_next is null.
-// ROWS: the tables are L(rows) of the packed rows a, b (round 0, nibble
-// tables of L in lw); otherwise F128 tables A, B.
-template <int THREADS, int LOG, bool ROWS>
-__global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const void *__restrict__ Ain,
-                                                                 const void *__restrict__ Bin,
-                                                                 const f128 *__restrict__ lw,
+template <int THREADS, int LOG>
+__global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const f128 *__restrict__ A,
+                                                                 const f128 *__restrict__ B,
                                                                  const f128 *__restrict__ Elo,
                                                                  const f128 *__restrict__ Ehi, uint64_t n_hi,
                                                                  f128 *__restrict__ partial,
@@ -123,11 +121,7 @@ __global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const void *__r
   static_assert(THREADS == 256 || THREADS == 512, "the table builds assume 256 or 512 threads");
   __shared__ f128 mt[WAVES][1024];  // per-lane product tables; the reduction reuses them
   __shared__ __align__(16) char nt[NIBTAB_BYTES];
-  __shared__ f128 T[ROWS ? 256 : 1];
   const uint32_t tid = threadIdx.x;
-  if constexpr (ROWS) {
-    if (tid < 256) T[tid] = lw[tid];
-  }
   f128 *tab = &mt[0][0];
   const CombLane CL(tid);
   const NibtabLane NL(tid & 15u);
@@ -137,27 +131,13 @@ __global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const void *__r
   for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
     const f128 eh = Ehi[xh];
     if (Ehi_next && !(xh & 1) && tid == 0) Ehi_next[xh >> 1] = f128_xor(eh, Ehi[xh + 1]);
-    __syncthreads();  // the previous x_hi's table reads are done (and T is visible)
+    __syncthreads();  // the previous x_hi's table reads are done
     if (tid < 256) build_nibtab256<0>(eh, nt, tid);
     __syncthreads();
 #pragma unroll 1
     for (int s = 0; s < PER; ++s) {
       const uint64_t x = (xh << LOG) + (uint64_t)s * THREADS + tid;
-      f128 a0, a1, b0, b1;
-      if constexpr (ROWS) {
-        const ulonglong2 ar = ((const ulonglong2 *)Ain)[x], br = ((const ulonglong2 *)Bin)[x];
-        const uint32_t rot = tid & 15u;
-        a0 = lagrange_row(ar.x, T, rot);
-        a1 = lagrange_row(ar.y, T, rot);
-        b0 = lagrange_row(br.x, T, rot);
-        b1 = lagrange_row(br.y, T, rot);
-      } else {
-        const f128 *A = (const f128 *)Ain, *B = (const f128 *)Bin;
-        a0 = A[2 * x];
-        a1 = A[2 * x + 1];
-        b0 = B[2 * x];
-        b1 = B[2 * x + 1];
-      }
+      const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1];
       const f128 p1 = f128_mul(a1, b1, tab, CL);
       const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
       const f128 w1 = f128_mul_nibtab<0>(p1, nt, NL), wi = f128_mul_nibtab<0>(pi, nt, NL);
@@ -182,11 +162,74 @@ __global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const void *__r
   wg_xor_reduce(gi, tab, &partial[2 * blockIdx.x + 1]);
 }
 
+// Round 0 over the packed rows a, b (row pair x: bits of (x, 0) and (x, 1)).
+// The tables are A = L(row) for the GF(2)-linear L of the nibble tables T
+// (lagrange_row), so with r1 = row (x, 1) and ri = row (x, 0) ^ row (x, 1)
+//   Ehi p1 = (Ehi L)(a_1) L(b_1),   Ehi p_inf = (Ehi L)(a_i) L(b_i):
+// the weight Ehi[x_hi] moves into the tables of one factor, T' = Ehi T,
+// rebuilt per x_hi from T by 256 products through Ehi's nibble tables, and a
+// row pair costs four table evaluations and two general products, with no
+// product by Ehi.  Same sums as zc_round_split_kernel over A = L(a), B = L(b).
+template <int THREADS, int LOG>
+__global__ __launch_bounds__(THREADS) void zc_round0_split_kernel(const ulonglong2 *__restrict__ a,
+                                                                  const ulonglong2 *__restrict__ b,
+                                                                  const f128 *__restrict__ lw,
+                                                                  const f128 *__restrict__ Elo,
+                                                                  const f128 *__restrict__ Ehi, uint64_t n_hi,
+                                                                  f128 *__restrict__ partial) {
+  constexpr int WAVES = THREADS / 64, PER = (1 << LOG) / THREADS;
+  static_assert(THREADS == 256 || THREADS == 512, "the table builds assume 256 or 512 threads");
+  __shared__ f128 mt[WAVES][1024];  // per-lane product tables; the reduction reuses them
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
+  __shared__ f128 T[256], Tw[256];  // L's nibble tables, and Ehi[x_hi] times them
+  const uint32_t tid = threadIdx.x, rot = tid & 15u;
+  if (tid < 256) T[tid] = lw[tid];
+  f128 *tab = &mt[0][0];
+  const CombLane CL(tid);
+  const NibtabLane NL(tid & 15u);
+  f128 s1[PER], si[PER];
+#pragma unroll
+  for (int s = 0; s < PER; ++s) s1[s] = si[s] = f128_zero();
+  for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
+    const f128 eh = Ehi[xh];
+    __syncthreads();  // the previous x_hi's reads of nt and Tw are done (and T is visible)
+    if (tid < 256) build_nibtab256<0>(eh, nt, tid);
+    __syncthreads();
+    if (tid < 256) Tw[tid] = f128_mul_nibtab<0>(T[tid], nt, NL);
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < PER; ++s) {
+      const uint64_t x = (xh << LOG) + (uint64_t)s * THREADS + tid;
+      const ulonglong2 ar = a[x], br = b[x];
+      const f128 a1 = lagrange_row(ar.y, Tw, rot), ai = lagrange_row(ar.x ^ ar.y, Tw, rot);
+      const f128 b1 = lagrange_row(br.y, T, rot), bi = lagrange_row(br.x ^ br.y, T, rot);
+      const f128 w1 = f128_mul(a1, b1, tab, CL);
+      const f128 wi = f128_mul(ai, bi, tab, CL);
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        if (q == s) {
+          f128_xor_into(s1[q], w1);
+          f128_xor_into(si[q], wi);
+        }
+    }
+  }
+  f128 g1 = f128_zero(), gi = f128_zero();
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    f128_mul_build(Elo[s * THREADS + tid], tab, CL);
+    f128_xor_into(g1, f128_mul_comb(s1[s], tab, CL));
+    f128_xor_into(gi, f128_mul_comb(si[s], tab, CL));
+  }
+  __syncthreads();  // every lane's table reads are done before the reduction reuses mt
+  wg_xor_reduce(g1, tab, &partial[2 * blockIdx.x]);
+  wg_xor_reduce(gi, tab, &partial[2 * blockIdx.x + 1]);
+}
+
 // Rounds after round 0: 256-thread workgroups (72 KiB of LDS, two per CU)
-// over F128 tables, split at ZS_LOG.  Round 0: 512-thread workgroups (140 KiB,
+// over F128 tables, split at ZS_LOG.  Round 0: 512-thread workgroups (144 KiB,
 // one per CU) over the rows, split at ZS_LOG + 1.
 int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, const void *Elo, const void *Ehi,
-                          uint64_t half, void *out2, void *Ehi_next, uint32_t seq) {
+                          uint64_t half, void *out2, void *Ehi_next, uint32_t seq, const FsStep &fs) {
   ScopedPhase ph(ctx, "zc_round", st);
   const uint64_t n_hi = half >> ZS_LOG;
   if ((half & ((1ull << ZS_LOG) - 1)) || n_hi < 2) {
@@ -196,14 +239,14 @@ int zc_round_split_device(Ctx *ctx, hipStream_t st, const void *A, const void *B
   const unsigned g = (unsigned)(n_hi < 512 ? n_hi : 512);
   f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
-  hipLaunchKernelGGL((zc_round_split_kernel<256, ZS_LOG, false>), dim3(g), dim3(256), 0, st, A, B, nullptr,
-                     (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, (f128 *)Ehi_next);
+  hipLaunchKernelGGL((zc_round_split_kernel<256, ZS_LOG>), dim3(g), dim3(256), 0, st, (const f128 *)A,
+                     (const f128 *)B, (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, (f128 *)Ehi_next);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq, fs);
 }
 
 int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *b, const void *d_lw, const void *Elo,
-                           const void *Ehi, uint64_t half, void *out2, uint32_t seq) {
+                           const void *Ehi, uint64_t half, void *out2, uint32_t seq, const FsStep &fs) {
   ScopedPhase ph(ctx, "zc_round", st);
   constexpr int LOG = ZS_LOG + 1;
   const uint64_t n_hi = half >> LOG;
@@ -214,16 +257,16 @@ int zc_round0_split_device(Ctx *ctx, hipStream_t st, const void *a, const void *
   const unsigned g = (unsigned)(n_hi < 256 ? n_hi : 256);
   f128 *part = (f128 *)ctx->get_scratch("zc.round.partial", (size_t)g * 2 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
-  hipLaunchKernelGGL((zc_round_split_kernel<512, LOG, true>), dim3(g), dim3(512), 0, st, a, b, (const f128 *)d_lw,
-                     (const f128 *)Elo, (const f128 *)Ehi, n_hi, part, nullptr);
+  hipLaunchKernelGGL((zc_round0_split_kernel<512, LOG>), dim3(g), dim3(512), 0, st, (const ulonglong2 *)a,
+                     (const ulonglong2 *)b, (const f128 *)d_lw, (const f128 *)Elo, (const f128 *)Ehi, n_hi, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq, fs);
 }
 
 // ---------------------------------------------------------------- zerocheck fold after round 0
 // After round 1 the multilinear tables are A[x] = sum_s bit_s(a[x]) L_s(z)
 // (fold_at_z_naive, zerocheck/multilinear.rs:598-625), a GF(2)-linear image
-// of the 64-bit row.  Round 0 (zc_round_split_kernel with ROWS) and its fold
+// of the 64-bit row.  Round 0 (zc_round0_split_kernel) and its fold
 // read the rows and evaluate that map on the fly instead of materializing A
 // and B.  The fold uses byte tables T[v * 8 + p] = sum_{t < 8, bit t of v}
 // L_{8p + t}(z) (32 KiB), read in a lane-dependent order p = step ^ (lane mod 8).
@@ -394,9 +437,10 @@ __device__ __forceinline__ void small_publish(f128 s0, f128 s1, f128 *red, f128 
 __global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
     const f128 *__restrict__ A, const f128 *__restrict__ B, f128 rho_arg, const f128 *__restrict__ rho_dev,
     const f128 *__restrict__ E, uint32_t half, f128 *__restrict__ A2, f128 *__restrict__ B2,
-    f128 *__restrict__ E_next, f128 *__restrict__ out, uint32_t *flag, uint32_t seq) {
+    f128 *__restrict__ E_next, f128 *out, uint32_t *flag, uint32_t seq, FsState *fs, f128 *rho_out) {
   const f128 rho = rho_dev ? *rho_dev : rho_arg;
   __shared__ f128 mt[SMALL_THREADS / 64][1024];
+  __shared__ FsState fss;
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
   const uint32_t x = threadIdx.x;
@@ -419,6 +463,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
     gi = f128_mul_comb(pi, tab, CL);
   }
   small_publish(g1, gi, tab, out, 2, flag, seq);
+  if (fs) fs_round_block(fs, fss, out, rho_out);  // thread 0 reads back what it wrote
 }
 
 // Ligerito round over `half` pairs after folding f, b (4 half entries each)
@@ -519,7 +564,7 @@ static f128 to_f128(F128 v) {
 
 int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
                                uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq,
-                               const void *d_rho) {
+                               const void *d_rho, const FsStep &fs) {
   ScopedPhase ph(ctx, "zc_round", st);
   if (half == 0) {
     ctx->last_error = "zerocheck: fused round needs at least one pair";
@@ -529,7 +574,8 @@ int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B,
   if (half <= SMALL_PAIRS) {
     hipLaunchKernelGGL(zc_fold_round_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)A,
                        (const f128 *)B, to_f128(rho), (const f128 *)d_rho, (const f128 *)E, (uint32_t)half,
-                       (f128 *)A2, (f128 *)B2, (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq);
+                       (f128 *)A2, (f128 *)B2, (f128 *)E_next, (f128 *)out2, ctx->readback_flag_dev(), seq,
+                       (FsState *)fs.state, (f128 *)fs.rho);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
@@ -540,7 +586,7 @@ int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B,
                      to_f128(rho), (const f128 *)d_rho, (const f128 *)E, half, (f128 *)A2, (f128 *)B2,
                      (f128 *)E_next, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 2, out2, seq);
+  return xor_rows_device(ctx, st, part, g, 2, out2, seq, fs);
 }
 
 int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half,

@@ -28,6 +28,7 @@
 
 #include "../../include/flock_gpu.h"
 #include "ctx.h"
+#include "transcript_dev.h"
 #include "gf128_dev.h"
 #include "prove_kernels.h"
 #include "zc_tables.h"
@@ -195,10 +196,11 @@ __global__ __launch_bounds__(ZCY_THREADS) void zc_y_kernel(const uint64_t *__res
 // XOR-sum `count` rows of W F128 each, out[w] = xor_r in[r * W + w], in one
 // workgroup (a 256-thread tree per column; four rows per thread in flight).  With seq != 0 it then publishes
 // seq at *flag after out is written (as signal_kernel does), so a host
-// waiting for out needs no further launch.
+// waiting for out needs no further launch.  With fs (W = 2) the workgroup
+// then absorbs out into the transcript state and samples *rho.
 template <int W>
-__global__ __launch_bounds__(256) void xor_rows_kernel(const f128 *__restrict__ in, uint64_t count,
-                                                       f128 *__restrict__ out, uint32_t *flag, uint32_t seq) {
+__global__ __launch_bounds__(256) void xor_rows_kernel(const f128 *__restrict__ in, uint64_t count, f128 *out,
+                                                       uint32_t *flag, uint32_t seq, FsState *fs, f128 *rho) {
   __shared__ f128 red[W][256];
   const uint32_t tid = threadIdx.x;
   f128 s[W];
@@ -234,18 +236,28 @@ __global__ __launch_bounds__(256) void xor_rows_kernel(const f128 *__restrict__ 
       __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
+  if constexpr (W == 2) {
+    __shared__ FsState fss;
+    if (fs) fs_round_block(fs, fss, out, rho);  // thread 0 reads back what it wrote
+  }
 }
 
 int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out,
-                    uint32_t seq) {
+                    uint32_t seq, const FsStep &fs) {
   const f128 *i = (const f128 *)in;
   f128 *o = (f128 *)out;
   uint32_t *flag = ctx->readback_flag_dev();
+  if (fs.state && width != 2) {
+    ctx->last_error = "xor_rows: the transcript step takes two values";
+    return FLOCK_E_ARG;
+  }
+  FsState *S = (FsState *)fs.state;
+  f128 *R = (f128 *)fs.rho;
   switch (width) {
-    case 1: hipLaunchKernelGGL(xor_rows_kernel<1>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
-    case 2: hipLaunchKernelGGL(xor_rows_kernel<2>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
-    case 3: hipLaunchKernelGGL(xor_rows_kernel<3>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
-    case 4: hipLaunchKernelGGL(xor_rows_kernel<4>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq); break;
+    case 1: hipLaunchKernelGGL(xor_rows_kernel<1>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
+    case 2: hipLaunchKernelGGL(xor_rows_kernel<2>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
+    case 3: hipLaunchKernelGGL(xor_rows_kernel<3>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
+    case 4: hipLaunchKernelGGL(xor_rows_kernel<4>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
     default:
       ctx->last_error = "xor_rows: width out of range";
       return FLOCK_E_SHAPE;

@@ -71,10 +71,14 @@ const char *flock_last_error(const flock_ctx *ctx);
 int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds);
 
 /* Context options (the proof bytes never depend on them):
- *  "device_transcript" (default 1): the zerocheck rounds' Fiat-Shamir steps
- *    run on the device (the BLAKE3 transcript state moves to device memory
- *    and back) instead of one host round trip per round; 0 keeps them on
- *    the host.
+ *  "device_transcript" (default 0): 1 runs the zerocheck rounds' Fiat-Shamir
+ *    steps on the device (the BLAKE3 transcript state moves to device memory
+ *    and back; each round's reduction kernel absorbs the round's message and
+ *    samples its challenge) instead of one host round trip per round.  The
+ *    one-lane hashing costs about what the round trip does and the host
+ *    learns the challenges only at the end, so measured proofs are slower
+ *    with it at every size; it is for callers that capture or replay a
+ *    proof's rounds without the host.
  *  "zc_graph" (default 1): with the device transcript, the zerocheck rounds'
  *    launches are captured once per size and buffer set in a HIP graph and
  *    replayed by later proofs.

@@ -147,10 +147,10 @@ def test_shape_errors_are_reported(ctx):
 
 @pytest.mark.parametrize("log2", [8, 12, 16])
 def test_device_and_host_transcripts_give_the_same_bytes(ctx, log2):
-    """The zerocheck rounds' Fiat-Shamir steps on the device (the default;
-    the first proof of a size captures the rounds in a HIP graph, later
-    ones replay it) and on the host give identical proofs, for three inputs
-    in a row; at 2^8 and 2^12 they also equal the oracle's."""
+    """The zerocheck rounds' Fiat-Shamir steps on the device (the first
+    proof of a size captures the rounds in a HIP graph, later ones replay
+    it) and on the host (the default) give identical proofs, for three
+    inputs in a row; at 2^8 and 2^12 they also equal the oracle's."""
     _say(f"device vs host transcript log2={log2}")
     s = flockgpu.Blake3Setup(ctx, 1 << log2)
     try:
@@ -169,6 +169,6 @@ def test_device_and_host_transcripts_give_the_same_bytes(ctx, log2):
             if log2 <= 12 and k == 0:
                 _check_same(dev, O.prove(O.generate_compressions(log2, seed)))
     finally:
-        ctx.set_option("device_transcript", 1)
+        ctx.set_option("device_transcript", 0)
         ctx.set_option("zc_graph", 1)
         s.close()
