@@ -52,7 +52,8 @@ struct MerkleJob {
 // (F = 3 or 4; W wavefronts per workgroup with G groups each, 2^s >= W G
 // groups sharing the pass's twiddles; full byte tables rebuilt per twiddle,
 // or all the pass's nibble tables built once), and per-wavefront tail passes
-// (F = 1..4, NB nibble tables in flight).
+// (F = 1..4, NB nibble tables in flight, or no tables: products by shifted
+// XORs, for the short twiddles of the deepest layers).
 enum NttKernel : uint8_t {
   NTT_SHARED_W8G2 = 0,
   NTT_SHARED_W8G1 = 1,
@@ -61,7 +62,8 @@ enum NttKernel : uint8_t {
   NTT_TAIL_NB1 = 4,
   NTT_NIBSHARED_W16 = 5,
   NTT_NIBSHARED_W8 = 6,
-  NTT_KERNELS = 7
+  NTT_TAIL_SHIFT = 7,
+  NTT_KERNELS = 8
 };
 // One pass: layers [l, l + F) with kernel.
 struct NttPass {

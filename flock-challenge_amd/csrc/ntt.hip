@@ -44,6 +44,7 @@
 #include "fulltab.h"
 #include "fused_job.h"
 #include "gf128_dev.h"
+#include "gf128_short.h"
 #include "nibtab.h"
 
 namespace flock {
@@ -54,48 +55,78 @@ namespace flock {
 // products).  One wavefront per group; it builds the 8 KiB nibble tables of
 // its own twiddles, NB at a time, in NB * 8 KiB of LDS (NB = 2: two products
 // by different twiddles in flight; NB = 1: twice the wavefronts per CU).
+// With SJ = 0 (no SJ < F is instantiated: mixed with table layers in one
+// kernel, the compiler hoists all of a table product's lookups ahead of its
+// XORs and runs out of registers) every layer takes shifted XORs instead
+// (gf128_short.h): no table, no LDS and no barrier, at a cost proportional
+// to the twiddle's set bits, which pays in the transform's deepest layers
+// (twiddles of at most d and 2d - 3 bits in the last two layers of a
+// transform of 2^d positions).  Any twiddle gives the right product either
+// way; the plan decides where each variant runs.
 // Step (J, C) handles twiddles C .. C + NT - 1 of layer offset J; template
 // recursion keeps every register index constant.
-template <int F, int NB, int J, int C>
+
+// Twiddle n of a group's pass, in scalar registers, from the vector in
+// which lane n holds twiddle n (one load for all of them, 4 VGPRs).
+__device__ __forceinline__ f128 f128_mul_shifts(f128 v, f128 lanes, int n) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+  uint32_t o[4];
+  shortmul::mul(vw, __builtin_amdgcn_readlane(lanes.x, n), __builtin_amdgcn_readlane(lanes.y, n),
+                __builtin_amdgcn_readlane(lanes.z, n), __builtin_amdgcn_readlane(lanes.w, n), o);
+  return make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+template <int F, int NB, int SJ, int J, int C>
 __device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__restrict__ tw, int l, uint64_t B,
-                                               char *lds, const NibtabLane &NL, uint32_t lane) {
-  constexpr int NT = (NB == 2 && (1 << J) - C >= 2) ? 2 : 1;
+                                               f128 twl, char *lds, const NibtabLane &NL, uint32_t lane) {
   constexpr int nb = 1 << (F - 1 - J);
-  const uint64_t tb = (1ull << (l + J)) - 1ull + (B << J);
-  __syncthreads();  // the previous step's tables are read
-  build_nibtab<0>(tw[tb + C], lds, lane);
-  if constexpr (NT == 2) build_nibtab<NIBTAB_BYTES>(tw[tb + C + 1], lds, lane);
-  __syncthreads();
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int el = 0; el < nb; ++el) {
-    const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-    const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<0>(v[e1], lds, NL));
-    v[e0] = u0;
-    f128_xor_into(v[e1], u0);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  if constexpr (NT == 2) {
+  constexpr int NT = (J < SJ && NB == 2 && (1 << J) - C >= 2) ? 2 : 1;
+  if constexpr (J >= SJ) {
 #pragma unroll
     for (int el = 0; el < nb; ++el) {
-      const int e0 = ((C + 1) << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<NIBTAB_BYTES>(v[e1], lds, NL));
+      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const f128 u0 = f128_xor(v[e0], f128_mul_shifts(v[e1], twl, (1 << J) - 1 + C));
       v[e0] = u0;
       f128_xor_into(v[e1], u0);
       __builtin_amdgcn_sched_barrier(0);
     }
+  } else {
+    const uint64_t tb = (1ull << (l + J)) - 1ull + (B << J);
+    __syncthreads();  // the previous step's tables are read
+    build_nibtab<0>(tw[tb + C], lds, lane);
+    if constexpr (NT == 2) build_nibtab<NIBTAB_BYTES>(tw[tb + C + 1], lds, lane);
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int el = 0; el < nb; ++el) {
+      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<0>(v[e1], lds, NL));
+      v[e0] = u0;
+      f128_xor_into(v[e1], u0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (NT == 2) {
+#pragma unroll
+      for (int el = 0; el < nb; ++el) {
+        const int e0 = ((C + 1) << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+        const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<NIBTAB_BYTES>(v[e1], lds, NL));
+        v[e0] = u0;
+        f128_xor_into(v[e1], u0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
   }
   if constexpr (C + NT < (1 << J))
-    tail_pass_step<F, NB, J, C + NT>(v, tw, l, B, lds, NL, lane);
+    tail_pass_step<F, NB, SJ, J, C + NT>(v, tw, l, B, twl, lds, NL, lane);
   else if constexpr (J + 1 < F)
-    tail_pass_step<F, NB, J + 1, 0>(v, tw, l, B, lds, NL, lane);
+    tail_pass_step<F, NB, SJ, J + 1, 0>(v, tw, l, B, twl, lds, NL, lane);
 }
 
 // src is the codeword itself, or (first pass) the message, read with
 // positions masked by src_mask: position p of the replicated codeword holds
 // message position p & src_mask, the exact state after the replication layers
 // (pcs/commit.rs:451-467).
-template <int F, int NB>
+template <int F, int NB, int SJ>
 __device__ __forceinline__ void ntt_tail_body(f128 *data, const f128 *__restrict__ tw, int k, int l, const f128 *src,
                                               uint64_t src_mask, uint64_t g, char *lds) {
   const uint32_t lane = threadIdx.x;
@@ -107,24 +138,33 @@ __device__ __forceinline__ void ntt_tail_body(f128 *data, const f128 *__restrict
   f128 v[1 << F];
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_mask) << 6) + lane];
-  tail_pass_step<F, NB, 0, 0>(v, tw, l, B, lds, NL, lane);
+  // lane n = 2^j - 1 + c < 2^F - 1 with j >= SJ loads twiddle n (layer l + j,
+  // block (B << j) + c) for the shifted-XOR layers
+  f128 twl = f128_zero();
+  if constexpr (SJ < F) {
+    if (lane >= (1u << SJ) - 1u && lane < (1u << F) - 1u) {
+      const uint32_t j = 31 - __builtin_clz(lane + 1), c = lane + 1 - (1u << j);
+      twl = tw[(1ull << (l + j)) - 1ull + (B << j) + c];
+    }
+  }
+  tail_pass_step<F, NB, SJ, 0, 0>(v, tw, l, B, twl, lds, NL, lane);
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
 }
 
-template <int F, int NB>
+template <int F, int NB, int SJ>
 __global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *data, const f128 *__restrict__ tw, int k, int l,
                                                       const f128 *src, uint64_t src_mask) {
-  __shared__ __align__(16) char lds[NB * NIBTAB_BYTES];
-  ntt_tail_body<F, NB>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+  __shared__ __align__(16) char lds[SJ > 0 ? NB * NIBTAB_BYTES : 16];
+  ntt_tail_body<F, NB, SJ>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
 }
 
-template <int F, int NB>
+template <int F, int NB, int SJ>
 __global__ __launch_bounds__(64) void ntt_tail_batch_kernel(const NttJob *__restrict__ jobs, uint32_t n_jobs) {
-  __shared__ __align__(16) char lds[NB * NIBTAB_BYTES];
+  __shared__ __align__(16) char lds[SJ > 0 ? NB * NIBTAB_BYTES : 16];
   const NttJob &J = job_of(jobs, n_jobs);
-  ntt_tail_body<F, NB>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data, J.src ? J.src_pos_mask : ~0ull,
-                       blockIdx.x - J.wg_begin, lds);
+  ntt_tail_body<F, NB, SJ>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data, J.src ? J.src_pos_mask : ~0ull,
+                           blockIdx.x - J.wg_begin, lds);
 }
 
 // ---------------------------------------------------------------- shared-table passes
@@ -145,40 +185,66 @@ constexpr int SHARED_TWIDDLES = 15;  // 2^F - 1 for F <= 4
 // wavefronts per workgroup, whose 128-VGPR budget holds no more).
 template <int F, int G, int THREADS, int J, int C>
 __device__ __forceinline__ void shared_pass_step(f128 (&v)[G][1 << F], f128 *tabs, const f128 *basis,
-                                                 const FulltabLane &FL, uint32_t tid) {
+                                                 const FulltabLane &FL, uint32_t tid, uint32_t zmask) {
   constexpr int N = (1 << J) - 1 + C;
   constexpr bool HI = N & 1;
   const char *tbase = (const char *)tabs;  // buffer 0 at +0, buffer 1 at +64 KiB
-  build_fulltab<THREADS>(basis + N * 128, tabs + (HI ? 4096 : 0), tid);
-  __syncthreads();
   constexpr int nb = 1 << (F - 1 - J);
-  // product slots (group, butterfly) in pairs
   constexpr int SLOTS = G * nb;
-  constexpr int STEP = THREADS > 512 ? 1 : 2;
+  if ((zmask >> N) & 1u) {
+    // a zero twiddle (block 0 of a layer): u0 = v0, u1 = v1 + v0.  The
+    // barrier stays, so buffer N & 1 is not rebuilt (for twiddle N + 2)
+    // while a wave still reads it for twiddle N - 1 ... N + 1.
+    __syncthreads();
 #pragma unroll
-  for (int sl = 0; sl < SLOTS; sl += STEP) {
-    const int ga = sl / nb, la = sl % nb, gb = (sl + 1) / nb, lb = (sl + 1) % nb;
-    const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
-    if (STEP == 2 && sl + 1 < SLOTS) {
-      const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
-      f128 p0, p1;
-      f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
-      const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
-      v[ga][a0] = u0;
-      f128_xor_into(v[ga][a1], u0);
-      v[gb][b0] = u1;
-      f128_xor_into(v[gb][b1], u1);
-    } else {
-      const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
-      v[ga][a0] = u0;
-      f128_xor_into(v[ga][a1], u0);
+    for (int sl = 0; sl < SLOTS; ++sl) {
+      const int ga = sl / nb, la = sl % nb;
+      const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
+      f128_xor_into(v[ga][a1], v[ga][a0]);
     }
-    __builtin_amdgcn_sched_barrier(0);
+  } else {
+    build_fulltab<THREADS>(basis + N * 128, tabs + (HI ? 4096 : 0), tid);
+    __syncthreads();
+    // product slots (group, butterfly) in pairs
+    constexpr int STEP = THREADS > 512 ? 1 : 2;
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl += STEP) {
+      const int ga = sl / nb, la = sl % nb, gb = (sl + 1) / nb, lb = (sl + 1) % nb;
+      const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
+      if (STEP == 2 && sl + 1 < SLOTS) {
+        const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
+        f128 p0, p1;
+        f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
+        const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
+        v[ga][a0] = u0;
+        f128_xor_into(v[ga][a1], u0);
+        v[gb][b0] = u1;
+        f128_xor_into(v[gb][b1], u1);
+      } else {
+        const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
+        v[ga][a0] = u0;
+        f128_xor_into(v[ga][a1], u0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
   if constexpr (C + 1 < (1 << J))
-    shared_pass_step<F, G, THREADS, J, C + 1>(v, tabs, basis, FL, tid);
+    shared_pass_step<F, G, THREADS, J, C + 1>(v, tabs, basis, FL, tid, zmask);
   else if constexpr (J + 1 < F)
-    shared_pass_step<F, G, THREADS, J + 1, 0>(v, tabs, basis, FL, tid);
+    shared_pass_step<F, G, THREADS, J + 1, 0>(v, tabs, basis, FL, tid, zmask);
+}
+
+// Bit n set when twiddle n = 2^j - 1 + c of the pass (layer l + j, block
+// (B << j) + c) is zero: block 0 of a layer, which only the workgroups of
+// block B = 0 meet.  Wavefront-uniform.
+__device__ __forceinline__ uint32_t pass_zero_twiddles(const f128 *__restrict__ tw, int l, int F, uint64_t B) {
+  uint32_t zmask = 0;
+  for (uint32_t n = 0; n < (1u << F) - 1u; ++n) {
+    const uint32_t j = 31 - __builtin_clz(n + 1), c = n + 1 - (1u << j);
+    const f128 t = tw[(1ull << (l + j)) - 1ull + (B << j) + c];
+    zmask |= (uint32_t)(__builtin_amdgcn_readfirstlane(t.x | t.y | t.z | t.w) == 0u) << n;
+  }
+  return zmask;
 }
 
 constexpr int SHARED_LDS_F128 = 2 * 4096 + SHARED_TWIDDLES * 128;
@@ -214,8 +280,9 @@ __device__ __forceinline__ void ntt_pass_shared_body(f128 *data, const f128 *__r
 #pragma unroll
     for (int e = 0; e < (1 << F); ++e) v[gi][e] = src[(((base[gi] | ((uint64_t)e << s)) & src_mask) << 6) + lane];
   }
+  const uint32_t zmask = B == 0 ? pass_zero_twiddles(tw, l, F, B) : 0u;
   __syncthreads();  // basis visible
-  shared_pass_step<F, G, THREADS, 0, 0>(v, tabs, basis, FL, tid);
+  shared_pass_step<F, G, THREADS, 0, 0>(v, tabs, basis, FL, tid, zmask);
 #pragma unroll
   for (int gi = 0; gi < G; ++gi)
 #pragma unroll
@@ -246,21 +313,31 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_batch_kernel(const
 // lookups (16 with full byte tables), but nothing is rebuilt per twiddle, and
 // a lane's 128-VGPR budget lets 16 wavefronts share a CU.
 template <int F, int J, int C>
-__device__ __forceinline__ void nibshared_step(f128 (&v)[1 << F], const char *lds, const NibtabLane &NL) {
+__device__ __forceinline__ void nibshared_step(f128 (&v)[1 << F], const char *lds, const NibtabLane &NL,
+                                               uint32_t zmask) {
   constexpr int N = (1 << J) - 1 + C;
   constexpr int nb = 1 << (F - 1 - J);
+  if ((zmask >> N) & 1u) {
+    // a zero twiddle: u0 = v0, u1 = v1 + v0
 #pragma unroll
-  for (int el = 0; el < nb; ++el) {
-    const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-    const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<(uint32_t)N * NIBTAB_BYTES>(v[e1], lds, NL));
-    v[e0] = u0;
-    f128_xor_into(v[e1], u0);
-    __builtin_amdgcn_sched_barrier(0);
+    for (int el = 0; el < nb; ++el) {
+      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      f128_xor_into(v[e1], v[e0]);
+    }
+  } else {
+#pragma unroll
+    for (int el = 0; el < nb; ++el) {
+      const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
+      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<(uint32_t)N * NIBTAB_BYTES>(v[e1], lds, NL));
+      v[e0] = u0;
+      f128_xor_into(v[e1], u0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
   if constexpr (C + 1 < (1 << J))
-    nibshared_step<F, J, C + 1>(v, lds, NL);
+    nibshared_step<F, J, C + 1>(v, lds, NL, zmask);
   else if constexpr (J + 1 < F)
-    nibshared_step<F, J + 1, 0>(v, lds, NL);
+    nibshared_step<F, J + 1, 0>(v, lds, NL, zmask);
 }
 
 template <int F, int WAVES>
@@ -276,13 +353,16 @@ __device__ __forceinline__ void ntt_pass_nibshared_body(f128 *data, const f128 *
   f128 v[1 << F];
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) v[e] = src[(((base | ((uint64_t)e << s)) & src_mask) << 6) + lane];
-  // twiddle n = 2^j - 1 + c of the pass (layer offset j, block c)
+  // twiddle n = 2^j - 1 + c of the pass (layer offset j, block c); a zero
+  // twiddle gets no table
+  const uint32_t zmask = B == 0 ? pass_zero_twiddles(tw, l, F, B) : 0u;
   for (uint32_t n = wave; n < (1u << F) - 1u; n += WAVES) {
     const uint32_t j = 31 - __builtin_clz(n + 1), c = n + 1 - (1u << j);
-    build_nibtab<0>(tw[(1ull << (l + j)) - 1ull + (B << j) + c], lds + n * NIBTAB_BYTES, lane);
+    if (!((zmask >> n) & 1u))
+      build_nibtab<0>(tw[(1ull << (l + j)) - 1ull + (B << j) + c], lds + n * NIBTAB_BYTES, lane);
   }
   __syncthreads();
-  nibshared_step<F, 0, 0>(v, lds, NL);
+  nibshared_step<F, 0, 0>(v, lds, NL, zmask);
 #pragma unroll
   for (int e = 0; e < (1 << F); ++e) data[((base | ((uint64_t)e << s)) << 6) + lane] = v[e];
 }
@@ -333,17 +413,17 @@ void launch_nibshared(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob
                        a.mask);
 }
 
-template <int F, int NB>
+template <int F, int NB, int SJ>
 void launch_tail(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *jobs, uint32_t n_jobs) {
   if (jobs)
-    hipLaunchKernelGGL((ntt_tail_batch_kernel<F, NB>), grid, dim3(64), 0, st, jobs, n_jobs);
+    hipLaunchKernelGGL((ntt_tail_batch_kernel<F, NB, SJ>), grid, dim3(64), 0, st, jobs, n_jobs);
   else
-    hipLaunchKernelGGL((ntt_tail_kernel<F, NB>), grid, dim3(64), 0, st, a.data, a.tw, a.k, a.l, a.src, a.mask);
+    hipLaunchKernelGGL((ntt_tail_kernel<F, NB, SJ>), grid, dim3(64), 0, st, a.data, a.tw, a.k, a.l, a.src, a.mask);
 }
 
-const char *kernel_phase(NttKernel kn) {
-  return kn == NTT_TAIL_NB2 || kn == NTT_TAIL_NB1 ? "ntt_tail_kernel" : "ntt_pass_shared_kernel";
-}
+bool is_tail(int kn) { return kn == NTT_TAIL_NB2 || kn == NTT_TAIL_NB1 || kn == NTT_TAIL_SHIFT; }
+
+const char *kernel_phase(NttKernel kn) { return is_tail(kn) ? "ntt_tail_kernel" : "ntt_pass_shared_kernel"; }
 
 // One pass, directly (jobs null) or as a fused launch over jobs.
 int launch_pass(Ctx *ctx, hipStream_t st, int F, NttKernel kn, uint64_t wgs, const PassArgs &a, const NttJob *jobs,
@@ -353,22 +433,26 @@ int launch_pass(Ctx *ctx, hipStream_t st, int F, NttKernel kn, uint64_t wgs, con
   switch ((int)kn * 8 + F) {
 #define FLOCK_SHARED(KN, F_, W, G) \
   case (int)KN * 8 + F_: launch_shared<F_, W, G>(grid, st, a, jobs, n_jobs); break;
-#define FLOCK_TAIL(KN, F_, NB) \
-  case (int)KN * 8 + F_: launch_tail<F_, NB>(grid, st, a, jobs, n_jobs); break;
+#define FLOCK_TAIL(KN, F_, NB, SJ) \
+  case (int)KN * 8 + F_: launch_tail<F_, NB, SJ>(grid, st, a, jobs, n_jobs); break;
     FLOCK_SHARED(NTT_SHARED_W8G2, 3, 8, 2)
     FLOCK_SHARED(NTT_SHARED_W8G2, 4, 8, 2)
     FLOCK_SHARED(NTT_SHARED_W8G1, 3, 8, 1)
     FLOCK_SHARED(NTT_SHARED_W8G1, 4, 8, 1)
     FLOCK_SHARED(NTT_SHARED_W16G1, 3, 16, 1)
     FLOCK_SHARED(NTT_SHARED_W16G1, 4, 16, 1)
-    FLOCK_TAIL(NTT_TAIL_NB2, 1, 2)
-    FLOCK_TAIL(NTT_TAIL_NB2, 2, 2)
-    FLOCK_TAIL(NTT_TAIL_NB2, 3, 2)
-    FLOCK_TAIL(NTT_TAIL_NB2, 4, 2)
-    FLOCK_TAIL(NTT_TAIL_NB1, 1, 1)
-    FLOCK_TAIL(NTT_TAIL_NB1, 2, 1)
-    FLOCK_TAIL(NTT_TAIL_NB1, 3, 1)
-    FLOCK_TAIL(NTT_TAIL_NB1, 4, 1)
+    FLOCK_TAIL(NTT_TAIL_NB2, 1, 2, 1)
+    FLOCK_TAIL(NTT_TAIL_NB2, 2, 2, 2)
+    FLOCK_TAIL(NTT_TAIL_NB2, 3, 2, 3)
+    FLOCK_TAIL(NTT_TAIL_NB2, 4, 2, 4)
+    FLOCK_TAIL(NTT_TAIL_NB1, 1, 1, 1)
+    FLOCK_TAIL(NTT_TAIL_NB1, 2, 1, 2)
+    FLOCK_TAIL(NTT_TAIL_NB1, 3, 1, 3)
+    FLOCK_TAIL(NTT_TAIL_NB1, 4, 1, 4)
+    FLOCK_TAIL(NTT_TAIL_SHIFT, 1, 1, 0)
+    FLOCK_TAIL(NTT_TAIL_SHIFT, 2, 1, 0)
+    FLOCK_TAIL(NTT_TAIL_SHIFT, 3, 1, 0)
+    FLOCK_TAIL(NTT_TAIL_SHIFT, 4, 1, 0)
 #define FLOCK_NIBSHARED(KN, F_, W) \
   case (int)KN * 8 + F_: launch_nibshared<F_, W>(grid, st, a, jobs, n_jobs); break;
     FLOCK_NIBSHARED(NTT_NIBSHARED_W16, 3, 16)
@@ -400,7 +484,7 @@ int ntt_kernel_groups_log(NttKernel kn) {
 
 bool ntt_pass_feasible(int k, int l, int F, int kn) {
   if (l < 0 || F < 1 || F > 4 || l + F > k || kn < 0 || kn >= NTT_KERNELS) return false;
-  if (kn == NTT_TAIL_NB2 || kn == NTT_TAIL_NB1) return true;
+  if (is_tail(kn)) return true;
   return F >= 3 && k - l - F >= ntt_kernel_groups_log((NttKernel)kn);
 }
 

@@ -129,7 +129,9 @@ int flock_commit_device_noalloc(flock_ctx *ctx, const flock_pcs_params *params, 
  *            groups, 8 x 1, 16 x 1; they need 2^(k - l - F) >= 16, 8, 16);
  *   3, 4:    tail passes (F = 1..4; two or one nibble tables in flight);
  *   5, 6:    shared nibble-table passes (F = 3 or 4; 16 or 8 wavefronts,
- *            needing 2^(k - l - F) >= 16 or 8).
+ *            needing 2^(k - l - F) >= 16 or 8);
+ *   7:       tail passes without tables (F = 1..4; products by shifted
+ *            XORs, for the short twiddles of the deepest layers).
  * Every valid plan gives the same codeword; only the time differs.  Plans
  * live in a process-wide table, loaded at first use from $FLOCK_NTT_PLANS or
  * else tune/ntt_plans_gfx950.json in the directory above the library's

@@ -59,7 +59,7 @@ def test_commit_rejects_bad_shapes(ctx):
 
 # Plans for the transform of m = 24, rate 1/2 (k = 12 from layer 1) that
 # exercise every kernel and layer count: (l, F, kernel) passes.
-S82, S81, S161, T2, T1, N16, N8 = range(7)
+S82, S81, S161, T2, T1, N16, N8, SH = range(8)
 PLANS_K12 = {
     "default": None,
     "16-wave shared, one-table tail": [(1, 3, S161), (4, 4, S161), (8, 4, T1)],
@@ -68,6 +68,10 @@ PLANS_K12 = {
     "one layer per pass": [(l, 1, T1 if l % 2 else T2) for l in range(1, 12)],
     "nibble-shared 16 and 8": [(1, 4, N16), (5, 4, N8), (9, 3, T1)],
     "nibble-shared, three layers": [(1, 3, N8), (4, 3, N16), (7, 3, T2), (10, 2, T1)],
+    # shifted-XOR tails at every layer count, on the short deepest twiddles
+    # and on long ones (first pass from the message, zero block-0 twiddles)
+    "shift tails, deepest layers": [(1, 3, N8), (4, 4, S82), (8, 4, SH)],
+    "shift tails everywhere": [(1, 4, SH), (5, 3, SH), (8, 2, SH), (10, 1, SH), (11, 1, T1)],
 }
 
 

@@ -8,7 +8,7 @@ import pytest
 import gpu_util
 
 flockgpu = gpu_util.flockgpu
-S82, S81, S161, T2, T1, N16, N8 = range(7)
+S82, S81, S161, T2, T1, N16, N8, SH = range(8)
 PLAN_FILE = os.path.join(os.path.dirname(flockgpu.__file__), "tune", "ntt_plans_gfx950.json")
 
 
@@ -18,12 +18,13 @@ def groups_log(kernel):
 
 def feasible(k, l, F, kernel):
     """A shared-table pass needs F in {3, 4} and 2^(k-l-F) groups per block,
-    at least the groups one workgroup takes; a tail pass any F in 1..4."""
+    at least the groups one workgroup takes; a tail pass (with nibble tables
+    or shifted XORs) any F in 1..4."""
     if F < 1 or F > 4 or l + F > k:
         return False
     if kernel in (S82, S81, S161, N16, N8):
         return F >= 3 and k - l - F >= groups_log(kernel)
-    return kernel in (T2, T1)
+    return kernel in (T2, T1, SH)
 
 
 def test_default_plans_are_passes_of_four_layers():
@@ -52,7 +53,8 @@ def test_default_plans_are_passes_of_four_layers():
     [(1, 4, T2), (6, 4, T2), (10, 2, T2)],       # skips layer 5
     [(2, 4, T2), (6, 4, T2), (10, 2, T2)],       # does not start at layer 1
     [(1, 4, N16), (5, 4, N16), (9, 3, T2)],     # layer 5: 8 groups per block < 16
-    [(1, 4, T2), (5, 4, T2), (9, 3, 7)],         # no kernel 7
+    [(1, 4, T2), (5, 4, T2), (9, 3, 8)],         # no kernel 8
+    [(1, 4, SH), (5, 4, SH), (9, 4, SH)],        # shift tail with F = 4 past layer 12
 ])
 def test_invalid_plans_are_rejected_and_leave_the_table(bad):
     before = flockgpu.ntt_plan_get(12, 1)
