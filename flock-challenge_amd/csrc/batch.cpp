@@ -15,12 +15,12 @@
 // is captured once per shape in a HIP graph and replayed: later batches of
 // the same shape rewrite the job tables and launch the graph.
 //
-// Larger instances run as whole proofs on worker threads, each with its own
-// context and stream, largest first, while the fused fronts run; then the
-// workers take the rest of the small instances' proofs (zerocheck onwards,
-// which waits on Fiat-Shamir challenges).  The proofs overlap on the device
-// as in the proof server.  Every proof is the same bytes as a solo proof of
-// its input.
+// Once the fused fronts are done, worker threads, each with its own context
+// and stream, take the larger instances as whole proofs (largest first) and
+// the rest of the small instances' proofs (zerocheck onwards, which waits on
+// Fiat-Shamir challenges), one worker at a time on the small ones while
+// large ones remain.  The proofs overlap on the device as in the proof
+// server.  Every proof is the same bytes as a solo proof of its input.
 #include "batch.h"
 
 #include <string.h>
@@ -339,8 +339,8 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
     }
     small.push_back(i);
   }
-  // The work list: large items first (whole proofs, largest first), then
-  // the small ones once their fused fronts are done.
+  // Large items are taken largest first; small ones after their fused
+  // fronts.
   std::stable_sort(large.begin(), large.end(), [&](size_t a, size_t b) { return in[a].n_log > in[b].n_log; });
   std::stable_sort(small.begin(), small.end(), [&](size_t a, size_t b) { return in[a].n_log > in[b].n_log; });
   // The small items' regions in the arena, in the order the fronts take
@@ -362,9 +362,16 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
     total = align256(total + (((size_t)2 << (x.m - 12)) - 1) * 32);
     g.end = total;
   }
+  // Two work lists: the large items (whole proofs) and the small items'
+  // rests once their fronts are done.  One worker at a time takes small
+  // items while large ones remain: their proofs are latency-bound (short
+  // launches, a Fiat-Shamir round trip each round) and overlap with the
+  // large proofs' device-bound kernels, instead of all running at the end
+  // on a device they cannot fill.
   std::mutex qmu;
   std::condition_variable qcv;
-  std::deque<size_t> work(large.begin(), large.end());
+  std::deque<size_t> work(large.begin(), large.end()), work_small;
+  int running_small = 0;
   bool fronts_done = small.empty();
   proofs.assign(in.size(), {});
   done_s.assign(in.size(), 0.0);
@@ -412,7 +419,7 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
       if (!front_rc)
         for (size_t q = 0; q < small.size(); ++q) {
           in[small[q]] = fused[q];
-          work.push_back(small[q]);
+          work_small.push_back(small[q]);
         }
       fronts_done = true;
     }
@@ -425,12 +432,21 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
       hipSetDevice(wc->device);
       for (;;) {
         size_t i;
+        bool is_small = false;
         {
           std::unique_lock<std::mutex> ql(qmu);
-          qcv.wait(ql, [&] { return !work.empty() || fronts_done; });
-          if (work.empty()) return;
-          i = work.front();
-          work.pop_front();
+          qcv.wait(ql, [&] { return !work.empty() || !work_small.empty() || fronts_done; });
+          if (!work_small.empty() && (running_small == 0 || work.empty())) {
+            i = work_small.front();
+            work_small.pop_front();
+            is_small = true;
+            ++running_small;
+          } else if (!work.empty()) {
+            i = work.front();
+            work.pop_front();
+          } else {
+            return;
+          }
         }
         const Inst &x = in[i];
         if (x.n_log > FUSE_MAX_LOG2) {
@@ -449,6 +465,10 @@ int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *do
         }
         if (rcs[i]) errs[i] = wc->last_error;
         done_s[i] = now_s() - t0;
+        if (is_small) {
+          std::lock_guard<std::mutex> ql(qmu);
+          --running_small;
+        }
       }
     });
   for (auto &t : ts) t.join();

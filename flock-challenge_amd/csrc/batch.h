@@ -51,10 +51,10 @@ constexpr int FUSE_MAX_LOG2 = 14;
 // Proof bytes of every item (the same as a solo prove of its input).  The
 // fronts of the items of at most 2^FUSE_MAX_LOG2 compressions run fused on
 // ctx's stream (replayed from a HIP graph when use_graph and a batch with
-// the same small items ran before), the rest of their proofs on the
-// workers afterwards; larger items run as whole proofs on the n_workers
-// worker contexts from the start, largest first.  done_s[i] = seconds from
-// the call's start to proof i's bytes.
+// the same small items ran before), then the n_workers worker contexts take
+// larger items as whole proofs (largest first) and the rest of the small
+// items' proofs, one worker at a time on small items while large ones
+// remain.  done_s[i] = seconds from the call's start to proof i's bytes.
 int prove_batch(Ctx *ctx, const std::vector<BatchItem> &items, const uint8_t *domain, size_t domain_len,
                 int n_workers, bool use_graph, std::vector<std::vector<uint8_t>> &proofs, std::vector<double> &done_s,
                 BatchStats *stats);
