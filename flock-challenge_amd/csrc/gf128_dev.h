@@ -5,8 +5,9 @@
 //   x = bits 0..31, y = 32..63, z = 64..95, w = 96..127,
 // modulus x^128 + x^7 + x^2 + x + 1.  CDNA4 has no carry-less multiply, so
 // products by a wavefront-uniform constant go through per-constant lookup
-// tables in LDS (see ByteTable), and general products through 4-bit windowed
-// tables built per operand (gf_mul).
+// tables in LDS (fulltab.h, nibtab.h) or, for short constants, shifted XORs
+// (gf128_short.h), and general products through 4-bit windowed tables built
+// per operand (f128_mul below).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -43,72 +44,6 @@ __device__ __forceinline__ f128 f128_mulx(f128 v) {
 
 // h * (x^7 + x^2 + x + 1) for h < 2^25 fits in one dword.
 __device__ __forceinline__ uint32_t fold87(uint32_t h) { return h ^ (h << 1) ^ (h << 2) ^ (h << 7); }
-
-// a * x^16 mod P: the 16 bits shifted out fold back as h * 0x87 (< 2^23).
-__device__ __forceinline__ f128 f128_shl16(f128 a) {
-  uint32_t h = a.w >> 16;
-  f128 r;
-  r.w = __builtin_amdgcn_alignbit(a.w, a.z, 16);
-  r.z = __builtin_amdgcn_alignbit(a.z, a.y, 16);
-  r.y = __builtin_amdgcn_alignbit(a.y, a.x, 16);
-  r.x = (a.x << 16) ^ fold87(h);
-  return r;
-}
-
-// Multiply by a wavefront-uniform constant t through two 256-entry tables in
-// LDS: tab[b] = b*t and tab[256 + b] = (b * x^8)*t.  Horner over the eight
-// 16-bit digits of v, most significant first.
-__device__ __forceinline__ f128 f128_mul_bytetab(f128 v, const f128 *__restrict__ tab) {
-  f128 acc = f128_zero();
-  uint32_t a = v.w, b = v.z, c = v.y, d = v.x;
-  // Not unrolled on purpose: a fully unrolled product lets the scheduler
-  // hoist all 16 table loads (64 VGPRs) of several products at once.
-#pragma unroll 1
-  for (int q = 0; q < 4; ++q) {
-    uint32_t h = a >> 16;
-    acc = f128_shl16(acc);
-    f128_xor_into(acc, tab[h & 255u]);
-    f128_xor_into(acc, tab[256u + (h >> 8)]);
-    h = a & 0xffffu;
-    acc = f128_shl16(acc);
-    f128_xor_into(acc, tab[h & 255u]);
-    f128_xor_into(acc, tab[256u + (h >> 8)]);
-    a = b;
-    b = c;
-    c = d;
-  }
-  return acc;
-}
-
-// Build the two byte tables for constant t with 64 cooperating lanes
-// (lane = 0..63 within the calling group).  Lane L fills eight entries of
-// table (L >> 5): b = 8*(L & 31) + k, k = 0..7.
-__device__ __forceinline__ void f128_build_bytetab(f128 t, f128 *__restrict__ tab, uint32_t lane) {
-  uint32_t which = lane >> 5;
-  uint32_t u = lane & 31u;
-  f128 g = t;
-  if (which) {
-#pragma unroll
-    for (int s = 0; s < 8; ++s) g = f128_mulx(g);
-  }
-  f128 e[8];
-  e[0] = g;
-#pragma unroll
-  for (int s = 1; s < 8; ++s) e[s] = f128_mulx(e[s - 1]);
-  f128 hi = f128_zero();
-#pragma unroll
-  for (int s = 0; s < 5; ++s)
-    if ((u >> s) & 1u) f128_xor_into(hi, e[3 + s]);
-  f128 *dst = tab + (which << 8) + (u << 3);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    f128 val = hi;
-    if (k & 1) f128_xor_into(val, e[0]);
-    if (k & 2) f128_xor_into(val, e[1]);
-    if (k & 4) f128_xor_into(val, e[2]);
-    dst[k] = val;
-  }
-}
 
 // a * x^4 mod P: the 4 bits shifted out fold back as h * 0x87 (< 2^11).
 __device__ __forceinline__ f128 f128_shl4(f128 a) {

@@ -17,6 +17,7 @@
 #include "ctx.h"
 #include "fulltab.h"
 #include "gf128_dev.h"
+#include "gf128_short.h"
 #include "nibtab.h"
 #include "prove_kernels.h"
 #include "transcript_dev.h"
@@ -649,14 +650,16 @@ int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint6
   return xor_rows_device(ctx, st, part, g, 3, out3, seq);
 }
 
-// acc[i] += beta * x[i]  (SumcheckProver::glue, pcs/ligerito.rs:5766-5790)
+// acc[i] += beta * x[i]  (SumcheckProver::glue, pcs/ligerito.rs:5766-5790),
+// the products through the workgroup's nibble tables of beta.
 __global__ __launch_bounds__(256) void axpy_kernel(f128 *__restrict__ acc, const f128 *__restrict__ x, uint64_t n,
                                                    f128 beta) {
-  __shared__ f128 tab[512];
-  if (threadIdx.x < 64) f128_build_bytetab(beta, tab, threadIdx.x);
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
+  build_nibtab256<0>(beta, nt, threadIdx.x);
   __syncthreads();
+  const NibtabLane NL(threadIdx.x & 15u);
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-    f128_xor_into(acc[i], f128_mul_bytetab(x[i], tab));
+    f128_xor_into(acc[i], f128_mul_nibtab<0>(x[i], nt, NL));
 }
 
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta) {
@@ -682,13 +685,12 @@ __device__ __forceinline__ f128 wave_xor(f128 v) {
   return v;
 }
 
+// The column's one product by alpha (a kernel argument, so wavefront-
+// uniform) takes shifted XORs (gf128_short.h): no table to build for it.
 __global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict__ ap, const uint32_t *__restrict__ ar,
                                                       const uint32_t *__restrict__ bp, const uint32_t *__restrict__ br,
                                                       const f128 *__restrict__ eq, f128 alpha, f128 beta, uint32_t pin,
                                                       f128 *__restrict__ comb) {
-  __shared__ f128 tab[512];
-  if (threadIdx.x < 64) f128_build_bytetab(alpha, tab, threadIdx.x);
-  __syncthreads();
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
   f128 ta = f128_zero(), tb = f128_zero();
@@ -696,8 +698,11 @@ __global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict
   for (uint32_t k = bp[c] + lane; k < bp[c + 1]; k += 64) f128_xor_into(tb, eq[br[k]]);
   ta = wave_xor(ta);
   tb = wave_xor(tb);
+  const uint32_t tw[4] = {ta.x, ta.y, ta.z, ta.w};
+  uint32_t o[4];
+  shortmul::mul(tw, alpha.x, alpha.y, alpha.z, alpha.w, o);
   if (lane == 0) {
-    f128 v = f128_xor(f128_mul_bytetab(ta, tab), tb);
+    f128 v = f128_xor(make_uint4(o[0], o[1], o[2], o[3]), tb);
     if (c == pin) f128_xor_into(v, beta);
     comb[c] = v;
   }
