@@ -104,6 +104,7 @@ def algorithmic_bytes(name, m):
     return {
         "ntt_pass_shared_kernel": 2 * cw,   # read + write the codeword
         "ntt_tail_kernel": 2 * cw,
+        "ntt_pass_from_msg": cw // 2 + cw,  # read the message, write the codeword
         "witness": 3 * W + (1 << (m - 14)) * 112,  # write z, a, b; read the compressions
         "merkle_leaf_kernel": cw + 32 * (cw // 1024),
     }.get(name)

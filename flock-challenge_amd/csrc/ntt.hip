@@ -49,6 +49,22 @@
 
 namespace flock {
 
+// The workgroup a pass launch's hardware index w stands for.  A first pass
+// (reading the message, src_mask != ~0) starts at layer l = log_inv_rate, so
+// the 2^l blocks B of its first layer are the 2^l replicas of every message
+// position, and B is the top l bits of the workgroup index.  The mapping
+// below gives the replicas of the same positions to consecutive workgroups
+// of one XCD (hardware indices 8 apart: dispatch round-robins over the 8
+// XCDs, each with its own L2), so all but the first read of a message row
+// hit that L2.  A bijection on [0, n_wgs) (a power of two); any other pass
+// keeps w.
+__device__ __forceinline__ uint64_t pass_wg(uint64_t w, uint64_t n_wgs, int l, uint64_t src_mask) {
+  if (src_mask == ~0ull || l == 0 || n_wgs < (8ull << l)) return w;
+  const uint64_t xcd = w & 7, t = w >> 3;
+  const uint64_t b = t & ((1ull << l) - 1), u = t >> l;
+  return b * (n_wgs >> l) + (u << 3) + xcd;
+}
+
 // ---------------------------------------------------------------- tail passes
 // Layers whose twiddles are shared by too few groups to pay for full byte
 // tables (the last layers of the transform, where a twiddle meets 64 .. 512
@@ -156,7 +172,7 @@ template <int F, int NB, int SJ>
 __global__ __launch_bounds__(64) void ntt_tail_kernel(f128 *data, const f128 *__restrict__ tw, int k, int l,
                                                       const f128 *src, uint64_t src_mask) {
   __shared__ __align__(16) char lds[SJ > 0 ? NB * NIBTAB_BYTES : 16];
-  ntt_tail_body<F, NB, SJ>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+  ntt_tail_body<F, NB, SJ>(data, tw, k, l, src, src_mask, pass_wg(blockIdx.x, gridDim.x, l, src_mask), lds);
 }
 
 template <int F, int NB, int SJ>
@@ -293,7 +309,8 @@ template <int F, int WAVES, int G>
 __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_kernel(f128 *data, const f128 *__restrict__ tw, int k,
                                                                      int l, const f128 *src, uint64_t src_mask) {
   __shared__ f128 lds[SHARED_LDS_F128];
-  ntt_pass_shared_body<F, WAVES, G>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+  ntt_pass_shared_body<F, WAVES, G>(data, tw, k, l, src, src_mask, pass_wg(blockIdx.x, gridDim.x, l, src_mask),
+                                    lds);
 }
 
 template <int F, int WAVES, int G>
@@ -372,7 +389,8 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_nibshared_kernel(f128 *da
                                                                         int k, int l, const f128 *src,
                                                                         uint64_t src_mask) {
   __shared__ __align__(16) char lds[((1 << F) - 1) * NIBTAB_BYTES];
-  ntt_pass_nibshared_body<F, WAVES>(data, tw, k, l, src, src_mask, blockIdx.x, lds);
+  ntt_pass_nibshared_body<F, WAVES>(data, tw, k, l, src, src_mask, pass_wg(blockIdx.x, gridDim.x, l, src_mask),
+                                    lds);
 }
 
 template <int F, int WAVES>
@@ -425,10 +443,11 @@ bool is_tail(int kn) { return kn == NTT_TAIL_NB2 || kn == NTT_TAIL_NB1 || kn == 
 
 const char *kernel_phase(NttKernel kn) { return is_tail(kn) ? "ntt_tail_kernel" : "ntt_pass_shared_kernel"; }
 
-// One pass, directly (jobs null) or as a fused launch over jobs.
+// One pass, directly (jobs null) or as a fused launch over jobs; timed as
+// `phase` (default: by kernel kind).
 int launch_pass(Ctx *ctx, hipStream_t st, int F, NttKernel kn, uint64_t wgs, const PassArgs &a, const NttJob *jobs,
-                uint32_t n_jobs) {
-  ScopedPhase ph(ctx, kernel_phase(kn), st);
+                uint32_t n_jobs, const char *phase = nullptr) {
+  ScopedPhase ph(ctx, phase ? phase : kernel_phase(kn), st);
   const dim3 grid((unsigned)wgs);
   switch ((int)kn * 8 + F) {
 #define FLOCK_SHARED(KN, F_, W, G) \
@@ -681,7 +700,10 @@ int ntt_forward_interleaved_device(Ctx *ctx, hipStream_t st, void *d_data, int k
     const bool from_msg = first && d_msg;
     const PassArgs a{(f128 *)d_data, (const f128 *)tw, k, q.l, from_msg ? (const f128 *)d_msg : (const f128 *)d_data,
                      from_msg ? (1ull << (k - start_layer)) - 1 : ~0ull};
-    if (int rc = launch_pass(ctx, st, q.F, q.kernel, ntt_pass_wgs(k, q.F, q.kernel), a, nullptr, 0)) return rc;
+    // the pass that reads the message moves fewer bytes: its own phase
+    if (int rc = launch_pass(ctx, st, q.F, q.kernel, ntt_pass_wgs(k, q.F, q.kernel), a, nullptr, 0,
+                             from_msg ? "ntt_pass_from_msg" : nullptr))
+      return rc;
     first = false;
   }
   return FLOCK_OK;
