@@ -11,21 +11,11 @@
 #include "ctx.h"
 #include "fs_dev.h"
 #include "gf128_dev.h"
+#include "gf128_short.h"
 
 namespace flock {
 
 namespace {
-
-// a * b by shifts (one product per lane; for a few dozen products).
-__device__ f128 f128_mul_shift(f128 a, f128 b) {
-  f128 acc = f128_zero();
-  const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
-  for (int i = 127; i >= 0; --i) {
-    acc = f128_mulx(acc);
-    if ((bw[i >> 5] >> (i & 31)) & 1) f128_xor_into(acc, a);
-  }
-  return acc;
-}
 
 // Byte tables of rho L_s(z): out[v * 8 + p] = sum over the set bits t of v
 // of rho ws[8 p + t] (the fold after zerocheck round 0 evaluates
@@ -34,13 +24,26 @@ __global__ __launch_bounds__(256) void rho_bytetab_kernel(const f128 *__restrict
                                                           f128 *__restrict__ out) {
   __shared__ f128 wr[64];
   const uint32_t t = threadIdx.x;
-  if (t < 64) wr[t] = f128_mul_shift(*rho, ws[t]);
+  if (t < 64) {
+    // rho is one value for every lane: its product with ws[t] by shifted
+    // XORs over rho's set bits (a scalar loop)
+    const f128 r = *rho, w = ws[t];
+    const uint32_t wv[4] = {w.x, w.y, w.z, w.w};
+    uint32_t o[4];
+    shortmul::mul(wv, __builtin_amdgcn_readfirstlane(r.x), __builtin_amdgcn_readfirstlane(r.y),
+                  __builtin_amdgcn_readfirstlane(r.z), __builtin_amdgcn_readfirstlane(r.w), o);
+    wr[t] = make_uint4(o[0], o[1], o[2], o[3]);
+  }
   __syncthreads();
   for (uint32_t e = t; e < 2048; e += 256) {
     const uint32_t v = e >> 3, p = e & 7;
     f128 acc = f128_zero();
-    for (int b = 0; b < 8; ++b)
-      if ((v >> b) & 1) f128_xor_into(acc, wr[8 * p + b]);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t mk = 0u - ((v >> b) & 1u);
+      const f128 x = wr[8 * p + b];
+      acc = f128_xor(acc, make_uint4(x.x & mk, x.y & mk, x.z & mk, x.w & mk));
+    }
     out[e] = acc;
   }
 }
