@@ -74,11 +74,11 @@ int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds);
  *  "device_transcript" (default 0): 1 runs the zerocheck rounds' Fiat-Shamir
  *    steps on the device (the BLAKE3 transcript state moves to device memory
  *    and back; each round's reduction kernel absorbs the round's message and
- *    samples its challenge) instead of one host round trip per round.  The
- *    one-lane hashing costs about what the round trip does and the host
- *    learns the challenges only at the end, so measured proofs are slower
- *    with it at every size; it is for callers that capture or replay a
- *    proof's rounds without the host.
+ *    samples its challenge, each BLAKE3 compression spread over a quad of
+ *    lanes) instead of one host round trip per round.  The hashing costs
+ *    about what the round trip does, so whole proofs measure within 1 %
+ *    either way; it is for callers that capture or replay a proof's rounds
+ *    without the host.
  *  "zc_graph" (default 1): with the device transcript, the zerocheck rounds'
  *    launches are captured once per size and buffer set in a HIP graph and
  *    replayed by later proofs.
@@ -276,9 +276,10 @@ typedef struct {
  * launches make launch overhead matter, run fused: one launch per step for
  * all of them, the launch sequence captured in a HIP graph per shape (the
  * list of their sizes) and replayed when the same shape comes again, if
- * use_graph; the rest of their proofs follows on the workers.  Larger items
- * run as whole proofs on the n_workers (1..16) worker streams from the
- * start, largest first, in parallel with the fused fronts.
+ * use_graph.  Then the n_workers (1..16) worker streams take the larger
+ * items as whole proofs, largest first, and the rest of the small items'
+ * proofs, one worker at a time on small items while large ones remain (their
+ * latency-bound proofs overlap the large proofs' kernels).
  * out receives the proofs back to back: proof i is out[offsets[i] ..
  * offsets[i + 1]) (offsets has n_items + 1 entries); if cap is too small,
  * FLOCK_E_CAP with offsets filled and nothing copied.  done_s (n_items, may
