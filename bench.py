@@ -558,7 +558,7 @@ def main():
     ap.add_argument("--no-verify", action="store_true", help="skip the verifier check (profiling runs only)")
     ap.add_argument("--no-serve", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--no-sizes", action="store_true", help="skip the single-proof latency by size")
-    ap.add_argument("--workers", type=int, default=3, help="concurrent proofs in the batched/served runs")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent proofs in the batched/served runs")
     ap.add_argument("--rate", type=float, default=100.0, help="served: offered load, requests/s")
     ap.add_argument("--sweep", default="150,200,250",
                     help="served: further offered loads (requests/s) measured after --rate; '' to skip")
