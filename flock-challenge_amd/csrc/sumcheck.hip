@@ -800,17 +800,26 @@ int rs_basis_device(Ctx *ctx, hipStream_t st, const void *tables, const void *lo
 // (the positions that exchange data in layers l .. l+F-1) in registers.
 // Forward applies layers l .. l+F-1; transposed applies their transposes in
 // reverse order (l+F-1 .. l), which is the next F steps of the transposed
-// transform.  Twiddle products use the per-lane general product.
-template <int F, bool TRANSPOSED>
+// transform.
+// Twiddle products: when the 2^(6 - log_nl) groups of a wavefront share
+// their block (UT: 2^s >= groups per wavefront), each twiddle is
+// wavefront-uniform, so the wavefront builds its nibble tables together (8
+// entries per lane) and a product is 32 lookups of pre-reduced entries;
+// otherwise (the deepest layers) each lane builds a comb table of its own
+// twiddle (the general product).
+template <int F, bool TRANSPOSED, bool UT>
 __global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__restrict__ d,
                                                                        const f128 *__restrict__ tw, int k, int l,
                                                                        int log_nl, uint64_t n_threads) {
-  __shared__ f128 mt[MUL_WAVES][1024];
+  // comb: 256 B per lane; nibble tables: 8 KiB per wavefront
+  __shared__ f128 mt[MUL_WAVES][UT ? NIBTAB_BYTES / 16 : 1024];
   const uint32_t lane = threadIdx.x & 63;
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
+  const NibtabLane NL(lane & 15u);
+  char *wave_tab = (char *)&mt[threadIdx.x >> 6][0];  // UT: the wavefront's nibble tables
   const uint64_t t = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x;
-  if (t >= n_threads) return;
+  if (t >= n_threads) return;  // with UT, n_threads is a multiple of 64: whole wavefronts return
   const uint64_t ln = t & ((1ull << log_nl) - 1), g = t >> log_nl;
   const int s = k - l - F;
   const uint64_t r = g & ((1ull << s) - 1), B = g >> s;
@@ -820,11 +829,13 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__re
   for (int e = 0; e < (1 << F); ++e) v[e] = d[((base | ((uint64_t)e << s)) << log_nl) + ln];
   if (TRANSPOSED) {
     // The induced-basis transform starts from a vector with a few hundred
-    // nonzeros: an all-zero group maps to zeros (the map is linear).
+    // nonzeros: an all-zero group maps to zeros (the map is linear).  With
+    // UT the wavefront works together, so it returns only if all its
+    // groups are zero.
     uint32_t any = 0;
 #pragma unroll
     for (int e = 0; e < (1 << F); ++e) any |= v[e].x | v[e].y | v[e].z | v[e].w;
-    if (!any) return;
+    if (UT ? __builtin_amdgcn_ballot_w64(any != 0) == 0 : !any) return;
   }
 #pragma unroll
   for (int jj = 0; jj < F; ++jj) {
@@ -832,18 +843,33 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__re
     const uint64_t tw_base = (1ull << (l + j)) - 1ull + (B << j);
 #pragma unroll
     for (int c = 0; c < (1 << j); ++c) {
-      f128_mul_build(tw[tw_base + c], tab, CL);  // one table per twiddle
+      if constexpr (UT) {
+        // every lane's previous lookups precede the rebuild (a wavefront's
+        // LDS accesses execute in order; the fences keep the compiler's)
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        build_nibtab<0>(tw[tw_base + c], wave_tab, lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_sched_barrier(0);
+      } else {
+        f128_mul_build(tw[tw_base + c], tab, CL);  // one table per twiddle
+      }
 #pragma unroll
       for (int el = 0; el < (1 << (F - 1 - j)); ++el) {
         const int e0 = (c << (F - j)) | el, e1 = e0 | (1 << (F - 1 - j));
         if (!TRANSPOSED) {
-          const f128 u = f128_xor(v[e0], f128_mul_comb(v[e1], tab, CL));
+          const f128 p = UT ? f128_mul_nibtab<0>(v[e1], wave_tab, NL) : f128_mul_comb(v[e1], tab, CL);
+          const f128 u = f128_xor(v[e0], p);
           v[e0] = u;
           v[e1] = f128_xor(v[e1], u);
         } else {
           const f128 sum = f128_xor(v[e0], v[e1]);
           v[e0] = sum;
-          v[e1] = f128_xor(f128_mul_comb(sum, tab, CL), v[e1]);
+          const f128 p = UT ? f128_mul_nibtab<0>(sum, wave_tab, NL) : f128_mul_comb(sum, tab, CL);
+          v[e1] = f128_xor(p, v[e1]);
         }
       }
     }
@@ -871,12 +897,18 @@ int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, 
     const dim3 grid((unsigned)((n_threads + MUL_THREADS - 1) / MUL_THREADS)), block(MUL_THREADS);
     f128 *dd = (f128 *)d;
     const f128 *t = (const f128 *)tw;
-#define NARROW_CASE(FF)                                                                                         \
-  case FF:                                                                                                     \
-    if (transposed)                                                                                            \
-      hipLaunchKernelGGL((ntt_pass_narrow_kernel<FF, true>), grid, block, 0, st, dd, t, k, l, log_nl, n_threads); \
-    else                                                                                                       \
-      hipLaunchKernelGGL((ntt_pass_narrow_kernel<FF, false>), grid, block, 0, st, dd, t, k, l, log_nl, n_threads); \
+    // wavefront-uniform twiddles: the wavefront's 2^(6 - log_nl) groups share
+    // their block, and every wavefront is whole
+    const bool ut = log_nl <= 6 && k - l - F >= 6 - log_nl && k - F + log_nl >= 6;
+#define NARROW_LAUNCH(FF, TR, U) \
+  hipLaunchKernelGGL((ntt_pass_narrow_kernel<FF, TR, U>), grid, block, 0, st, dd, t, k, l, log_nl, n_threads)
+#define NARROW_CASE(FF)                                                            \
+  case FF:                                                                        \
+    if (transposed) {                                                             \
+      if (ut) NARROW_LAUNCH(FF, true, true); else NARROW_LAUNCH(FF, true, false);  \
+    } else {                                                                      \
+      if (ut) NARROW_LAUNCH(FF, false, true); else NARROW_LAUNCH(FF, false, false); \
+    }                                                                             \
     break;
     switch (F) {
       NARROW_CASE(1)
@@ -885,6 +917,7 @@ int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, 
       NARROW_CASE(4)
     }
 #undef NARROW_CASE
+#undef NARROW_LAUNCH
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     done += F;
   }
