@@ -97,8 +97,15 @@ int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, 
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
                    uint32_t seq = 0);
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);
-int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
-                   const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb);
+// A transposed R1CS matrix on the device for the lincheck combination: the
+// nonzero rows of column c are rows[ptr[c] .. ptr[c + 1]) (16-bit, ascending),
+// those below row 2^13 up to split[c]; rows is padded by 8 entries.
+struct LcCsc {
+  const uint32_t *ptr, *split;
+  const uint16_t *rows;
+};
+int lc_comb_device(Ctx *ctx, hipStream_t st, LcCsc a, LcCsc b, const void *eq, F128 alpha, F128 beta, uint32_t pin,
+                   void *comb);
 // out[i] = M_0(lo0[i mod 2^k] hi0[i >> k]) + M_1(lo1[i mod 2^k] hi1[i >> k])
 // for i < 2^(k + n_hi_log), M_c given by its nibble tables (tables[c][p][v]
 // = M_c(v x^(4p)), 2 x 32 x 16 F128).

@@ -77,6 +77,58 @@ const ZcTables &zc_tables() {
   return T;
 }
 
+// The transposes of A_0 and B_0 as the lincheck kernel reads them, in one
+// upload: per matrix, ptr (K + 1 words), split (K words: the first entry of
+// each column whose row is in the upper half, rows >= K / 2) and the rows as
+// 16-bit indices, padded by 8 entries for the kernel's 16-byte loads.
+struct LcImage {
+  std::vector<uint8_t> bytes;
+  size_t off[2][3];  // [matrix][ptr, split, rows]
+  bool ok = true;
+  LcCsc at(const uint8_t *base, int mi) const {
+    return {(const uint32_t *)(base + off[mi][0]), (const uint32_t *)(base + off[mi][1]),
+            (const uint16_t *)(base + off[mi][2])};
+  }
+};
+
+const LcImage &lc_image() {
+  static LcImage img;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const r1cs::Matrices &mx = r1cs::matrices();
+    static_assert(r1cs::K <= 1 << 16, "16-bit row indices");
+    size_t at = 0;
+    auto reserve = [&](size_t bytes) {
+      const size_t o = at;
+      at = (at + bytes + 255) & ~(size_t)255;
+      return o;
+    };
+    const r1cs::Csr *t[2] = {&mx.a_t, &mx.b_t};
+    for (int mi = 0; mi < 2; ++mi) {
+      img.off[mi][0] = reserve(4ull * (r1cs::K + 1));
+      img.off[mi][1] = reserve(4ull * r1cs::K);
+      img.off[mi][2] = reserve(2ull * (t[mi]->cols.size() + 8));
+    }
+    img.bytes.assign(at, 0);
+    for (int mi = 0; mi < 2; ++mi) {
+      const r1cs::Csr &c = *t[mi];
+      uint32_t *ptr = (uint32_t *)(img.bytes.data() + img.off[mi][0]);
+      uint32_t *split = (uint32_t *)(img.bytes.data() + img.off[mi][1]);
+      uint16_t *rows = (uint16_t *)(img.bytes.data() + img.off[mi][2]);
+      for (int col = 0; col <= r1cs::K; ++col) ptr[col] = c.ptr[col];
+      for (size_t k = 0; k < c.cols.size(); ++k) rows[k] = (uint16_t)c.cols[k];
+      for (int col = 0; col < r1cs::K; ++col) {
+        uint32_t k = c.ptr[col];
+        while (k < c.ptr[col + 1] && c.cols[k] < (uint32_t)r1cs::K / 2) ++k;
+        for (uint32_t q = k; q < c.ptr[col + 1]; ++q)
+          if (c.cols[q] < (uint32_t)r1cs::K / 2) img.ok = false;  // rows not ascending: the split is wrong
+        split[col] = k;
+      }
+    }
+  });
+  return img;
+}
+
 // Node indices of merkle_multi_proof (merkle.rs:879-925) in emission order.
 std::vector<uint64_t> multiproof_nodes(size_t num_leaves, const std::vector<size_t> &pos) {
   std::vector<uint64_t> nodes;
@@ -929,18 +981,20 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
     ScopedPhase ph(ctx, "lincheck", st);
     t.label("flock-lincheck-v0");
     const F128 alpha = t.sample(), beta = t.sample();
-    const r1cs::Matrices &mx = r1cs::matrices();
-    uint32_t *dap = (uint32_t *)P.dev("lc.ap", mx.a_t.ptr.size() * 4), *dar = (uint32_t *)P.dev("lc.ar", mx.a_t.cols.size() * 4);
-    uint32_t *dbp = (uint32_t *)P.dev("lc.bp", mx.b_t.ptr.size() * 4), *dbr = (uint32_t *)P.dev("lc.br", mx.b_t.cols.size() * 4);
+    const LcImage &img = lc_image();
+    if (!img.ok) {
+      ctx->last_error = "lincheck: transposed matrix rows are not ascending";
+      return FLOCK_E_DEVICE;
+    }
+    uint8_t *dimg = (uint8_t *)P.dev("lc.csc", img.bytes.size());
     void *deq = P.dev("lc.eq", 16 << 14), *dcomb = P.dev("lc.comb", 16 << 14), *dzv = P.dev("lc.zv", 16 << 14);
     void *deqo = P.dev("lc.eqo", 16ull << (m - 14));
-    if (!dap || !dar || !dbp || !dbr || !deq || !dcomb || !dzv || !deqo) return FLOCK_E_OOM;
+    if (!dimg || !deq || !dcomb || !dzv || !deqo) return FLOCK_E_OOM;
     if (!ctx->csc_uploaded) {
-      if ((rc = P.h2d(dap, mx.a_t.ptr.data(), mx.a_t.ptr.size() * 4)) || (rc = P.h2d(dar, mx.a_t.cols.data(), mx.a_t.cols.size() * 4)) ||
-          (rc = P.h2d(dbp, mx.b_t.ptr.data(), mx.b_t.ptr.size() * 4)) || (rc = P.h2d(dbr, mx.b_t.cols.data(), mx.b_t.cols.size() * 4)))
-        return rc;
+      if ((rc = P.h2d(dimg, img.bytes.data(), img.bytes.size()))) return rc;
       ctx->csc_uploaded = true;
     }
+    const LcCsc lca = img.at(dimg, 0), lcb = img.at(dimg, 1);
     // quirky eq table (lincheck.rs:1127-1142): index i_skip + 64 i_rest,
     // the outer product lam(i_skip) eq(rhos[0..8], i_rest) expanded on the device
     {
@@ -955,7 +1009,7 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
       if ((rc = P.h2d(dlr, lr, sizeof lr)) || (rc = eq_expand_device(ctx, st, dlr, 6, dlr + 64 * 16, 8, deq)))
         return rc;
     }
-    if ((rc = lc_comb_device(ctx, st, dap, dar, dbp, dbr, deq, alpha, beta, r1cs::Z_CONST, dcomb))) return rc;
+    if ((rc = lc_comb_device(ctx, st, lca, lcb, deq, alpha, beta, r1cs::Z_CONST, dcomb))) return rc;
     if ((rc = P.eq_device(&rhos[8], m - 14, deqo))) return rc;
     // z_vec[128 c + r] = sum_blk eq_outer[blk] bit_r(16-byte chunk c of block blk)
     if ((rc = bitfold_device(ctx, st, z, 7, 1ull << (m - 14), deqo, nullptr, dzv))) return rc;

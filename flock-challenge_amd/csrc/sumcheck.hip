@@ -671,9 +671,19 @@ int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, 
 
 // ---------------------------------------------------------------- lincheck
 // comb[c] = alpha (A_0^T eq)[c] + (B_0^T eq)[c] (+ beta at the pinned column)
-// (fold_alpha_batched, lincheck.rs:1158-1240; const pin, :1530-1545).  One
-// wavefront per column: lanes stride the column's nonzeros, then a butterfly
-// XOR across the wavefront.
+// (fold_alpha_batched, lincheck.rs:1158-1240; const pin, :1530-1545).
+//
+// The two transposes hold about 21M nonzeros (rows as 16-bit indices) against
+// a 2^14-entry eq table (256 KiB).  A workgroup per CU stages half of eq in
+// LDS at a time (128 KiB) and gathers from it: each column's rows are
+// ascending, so its entries in the lower half come first (up to split[c]).
+// A wavefront takes four columns; its lanes read the rows 8 at a time (one
+// 16-byte load) and XOR the eq entries into per-lane sums, reduced across
+// the wavefront at the end.  The product by alpha (a workgroup constant) uses
+// its nibble tables, one per column.
+constexpr int LC_THREADS = 1024, LC_WGS = 256, LC_HALF = 1 << 13;
+constexpr int LC_COLS_PER_WAVE = (1 << 14) / (LC_WGS * (LC_THREADS / 64));
+
 __device__ __forceinline__ f128 wave_xor(f128 v) {
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -685,35 +695,89 @@ __device__ __forceinline__ f128 wave_xor(f128 v) {
   return v;
 }
 
-// The column's one product by alpha (a kernel argument, so wavefront-
-// uniform) takes shifted XORs (gf128_short.h): no table to build for it.
-__global__ __launch_bounds__(256) void lc_comb_kernel(const uint32_t *__restrict__ ap, const uint32_t *__restrict__ ar,
-                                                      const uint32_t *__restrict__ bp, const uint32_t *__restrict__ br,
-                                                      const f128 *__restrict__ eq, f128 alpha, f128 beta, uint32_t pin,
-                                                      f128 *__restrict__ comb) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  f128 ta = f128_zero(), tb = f128_zero();
-  for (uint32_t k = ap[c] + lane; k < ap[c + 1]; k += 64) f128_xor_into(ta, eq[ar[k]]);
-  for (uint32_t k = bp[c] + lane; k < bp[c + 1]; k += 64) f128_xor_into(tb, eq[br[k]]);
-  ta = wave_xor(ta);
-  tb = wave_xor(tb);
-  const uint32_t tw[4] = {ta.x, ta.y, ta.z, ta.w};
-  uint32_t o[4];
-  shortmul::mul(tw, alpha.x, alpha.y, alpha.z, alpha.w, o);
-  if (lane == 0) {
-    f128 v = f128_xor(make_uint4(o[0], o[1], o[2], o[3]), tb);
+// acc ^= eqh[row - base] for the 8 rows packed in w, entries k .. k + 7 of
+// the column's row array, those of them inside [b, e) (all in the staged half).
+__device__ __forceinline__ void lc_take(uint4 w, uint32_t k, uint32_t b, uint32_t e, const f128 *eqh, uint32_t base,
+                                        f128 &acc) {
+  const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t kj = k + (uint32_t)j;
+    if (kj >= b && kj < e) f128_xor_into(acc, eqh[((ww[j >> 1] >> (16 * (j & 1))) & 0xFFFFu) - base]);
+  }
+}
+
+// A's entries [ab, ae) into sa and B's [bb, be) into sb: lanes read the rows 8
+// at a time (16-byte loads from ab, bb rounded down to 8 entries; the row
+// arrays are padded so the last load stays inside), four loads in flight.
+__device__ __forceinline__ void lc_gather(const uint16_t *__restrict__ ra, uint32_t ab, uint32_t ae,
+                                          const uint16_t *__restrict__ rb, uint32_t bb, uint32_t be,
+                                          const f128 *eqh, uint32_t base, uint32_t lane, f128 &sa, f128 &sb) {
+  uint32_t ka = (ab & ~7u) + 8u * lane, kb = (bb & ~7u) + 8u * lane;
+  while (ka < ae || kb < be) {
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    const uint4 a0 = ka < ae ? *(const uint4 *)(ra + ka) : z;
+    const uint4 a1 = ka + 512 < ae ? *(const uint4 *)(ra + ka + 512) : z;
+    const uint4 b0 = kb < be ? *(const uint4 *)(rb + kb) : z;
+    const uint4 b1 = kb + 512 < be ? *(const uint4 *)(rb + kb + 512) : z;
+    lc_take(a0, ka, ab, ae, eqh, base, sa);
+    lc_take(a1, ka + 512, ab, ae, eqh, base, sa);
+    lc_take(b0, kb, bb, be, eqh, base, sb);
+    lc_take(b1, kb + 512, bb, be, eqh, base, sb);
+    ka += 1024;
+    kb += 1024;
+  }
+}
+
+__global__ __launch_bounds__(LC_THREADS) void lc_comb_kernel(LcCsc A, LcCsc B, const f128 *__restrict__ eq,
+                                                             f128 alpha, f128 beta, uint32_t pin,
+                                                             f128 *__restrict__ comb) {
+  __shared__ f128 eqh[LC_HALF];
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
+  __shared__ f128 cols[LC_COLS_PER_WAVE * (LC_THREADS / 64)][2];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid < 256) build_nibtab256<0>(alpha, nt, tid);
+  f128 sa[LC_COLS_PER_WAVE], sb[LC_COLS_PER_WAVE];
+#pragma unroll
+  for (int i = 0; i < LC_COLS_PER_WAVE; ++i) sa[i] = sb[i] = f128_zero();
+#pragma unroll 1
+  for (uint32_t h = 0; h < 2; ++h) {
+    __syncthreads();  // the previous half's gathers are done
+    for (uint32_t i = tid; i < LC_HALF; i += LC_THREADS) eqh[i] = eq[h * LC_HALF + i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < LC_COLS_PER_WAVE; ++i) {
+      const uint32_t c = blockIdx.x + LC_WGS * (w + (LC_THREADS / 64) * (uint32_t)i);
+      const uint32_t ab = h ? A.split[c] : A.ptr[c], ae = h ? A.ptr[c + 1] : A.split[c];
+      const uint32_t bb = h ? B.split[c] : B.ptr[c], be = h ? B.ptr[c + 1] : B.split[c];
+      lc_gather(A.rows, ab, ae, B.rows, bb, be, eqh, h * LC_HALF, lane, sa[i], sb[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < LC_COLS_PER_WAVE; ++i) {
+    const f128 ta = wave_xor(sa[i]), tb = wave_xor(sb[i]);
+    if (lane == 0) {
+      cols[w + (LC_THREADS / 64) * i][0] = ta;
+      cols[w + (LC_THREADS / 64) * i][1] = tb;
+    }
+  }
+  __syncthreads();
+  constexpr uint32_t N_COLS = LC_COLS_PER_WAVE * (LC_THREADS / 64);
+  if (tid < N_COLS) {
+    const uint32_t c = blockIdx.x + LC_WGS * tid;
+    f128 v = f128_xor(f128_mul_nibtab<0>(cols[tid][0], nt, NibtabLane(tid & 15u)), cols[tid][1]);
     if (c == pin) f128_xor_into(v, beta);
     comb[c] = v;
   }
 }
 
-int lc_comb_device(Ctx *ctx, hipStream_t st, const uint32_t *ap, const uint32_t *ar, const uint32_t *bp,
-                   const uint32_t *br, const void *eq, F128 alpha, F128 beta, uint32_t pin, void *comb) {
+int lc_comb_device(Ctx *ctx, hipStream_t st, LcCsc a, LcCsc b, const void *eq, F128 alpha, F128 beta, uint32_t pin,
+                   void *comb) {
   ScopedPhase ph(ctx, "lc_comb", st);
+  static_assert(LC_COLS_PER_WAVE * (LC_THREADS / 64) * LC_WGS == 1 << 14, "columns per wavefront");
   auto cv = [](F128 v) { return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32)); };
-  hipLaunchKernelGGL(lc_comb_kernel, dim3((1 << 14) / 4), dim3(256), 0, st, ap, ar, bp, br, (const f128 *)eq,
-                     cv(alpha), cv(beta), pin, (f128 *)comb);
+  hipLaunchKernelGGL(lc_comb_kernel, dim3(LC_WGS), dim3(LC_THREADS), 0, st, a, b, (const f128 *)eq, cv(alpha),
+                     cv(beta), pin, (f128 *)comb);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return FLOCK_OK;
 }
