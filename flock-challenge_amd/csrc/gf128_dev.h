@@ -116,8 +116,9 @@ __device__ __forceinline__ void f128_mul_build(f128 a, f128 *__restrict__ mt, co
   for (int e = 0; e < 16; ++e) *(f128 *)(T + e * 256) = v[e];
 }
 
-// a * b with the lane's table of a already built (f128_mul_build).
-__device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ mt, const CombLane &C) {
+// The comb's 256-bit product of b by the lane's table (f128_mul_build), in
+// d[0..7] (its reduction is the product).
+__device__ __forceinline__ void f128_comb256(f128 b, const f128 *__restrict__ mt, const CombLane &C, uint32_t (&d)[8]) {
   const char *T = (const char *)mt;
   // even / odd nibbles of each word, one per byte, with the lane's block bits
   const uint32_t bw[4] = {b.x, b.y, b.z, b.w};
@@ -154,7 +155,39 @@ __device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ m
       d0 <<= 4;
     }
   }
-  return f128_reduce256(d0, d1, d2, d3, d4, d5, d6, d7);
+  d[0] = d0;
+  d[1] = d1;
+  d[2] = d2;
+  d[3] = d3;
+  d[4] = d4;
+  d[5] = d5;
+  d[6] = d6;
+  d[7] = d7;
+}
+
+// a * b with the lane's table of a already built (f128_mul_build).
+__device__ __forceinline__ f128 f128_mul_comb(f128 b, const f128 *__restrict__ mt, const CombLane &C) {
+  uint32_t d[8];
+  f128_comb256(b, mt, C, d);
+  return f128_reduce256(d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
+}
+
+// A sum of products kept before reduction (reduction is linear, so the sum
+// reduces once).
+struct F256w {
+  uint32_t d[8];
+};
+
+__device__ __forceinline__ f128 f128_reduce256w(const F256w &a) {
+  return f128_reduce256(a.d[0], a.d[1], a.d[2], a.d[3], a.d[4], a.d[5], a.d[6], a.d[7]);
+}
+
+// acc ^= a * b before reduction, with the lane's table of a already built.
+__device__ __forceinline__ void f128_mul_comb_acc(f128 b, const f128 *__restrict__ mt, const CombLane &C, F256w &acc) {
+  uint32_t d[8];
+  f128_comb256(b, mt, C, d);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc.d[i] ^= d[i];
 }
 
 __device__ __forceinline__ f128 f128_mul(f128 a, f128 b, f128 *__restrict__ mt, const CombLane &C) {

@@ -392,13 +392,14 @@ int fold_pair_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, uin
 constexpr int SMALL_THREADS = 256;
 static_assert(SMALL_PAIRS <= (uint64_t)SMALL_THREADS, "one pair per thread in the single-workgroup rounds");
 
-// v[4x] + r (v[4x] + v[4x+1]) and v[4x+2] + r (v[4x+2] + v[4x+3]), with the
-// lane's comb table holding r.
-__device__ __forceinline__ void fold4(const f128 *__restrict__ v, uint64_t x, const f128 *tab, const CombLane &CL,
+// v[4x] + r (v[4x] + v[4x+1]) and v[4x+2] + r (v[4x+2] + v[4x+3]), the
+// products through the workgroup's nibble tables of r (r is the round's one
+// challenge: a table built once per workgroup, not a comb per product).
+__device__ __forceinline__ void fold4(const f128 *__restrict__ v, uint64_t x, const char *nt, const NibtabLane &NL,
                                       f128 &o0, f128 &o1) {
   const f128 a0 = v[4 * x], a1 = v[4 * x + 1], a2 = v[4 * x + 2], a3 = v[4 * x + 3];
-  o0 = f128_xor(a0, f128_mul_comb(f128_xor(a0, a1), tab, CL));
-  o1 = f128_xor(a2, f128_mul_comb(f128_xor(a2, a3), tab, CL));
+  o0 = f128_xor(a0, f128_mul_nibtab<0>(f128_xor(a0, a1), nt, NL));
+  o1 = f128_xor(a2, f128_mul_nibtab<0>(f128_xor(a2, a3), nt, NL));
 }
 
 // XOR of s0, s1 over the workgroup into out[0], out[1] (out[2] = 0 when
@@ -441,25 +442,28 @@ __global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
     f128 *__restrict__ E_next, f128 *out, uint32_t *flag, uint32_t seq, FsState *fs, f128 *rho_out) {
   const f128 rho = rho_dev ? *rho_dev : rho_arg;
   __shared__ f128 mt[SMALL_THREADS / 64][1024];
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
   __shared__ FsState fss;
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
+  const NibtabLane NL(threadIdx.x & 15u);
   const uint32_t x = threadIdx.x;
+  if (x < 64) build_nibtab<0>(rho, nt, x);
+  __syncthreads();
+  // the next round's eq table apart from the products (as in zc_fold_round_kernel)
+  if (E_next && x < half / 2) E_next[x] = f128_xor(E[2 * x], E[2 * x + 1]);
   f128 g1 = f128_zero(), gi = f128_zero();
   if (x < half) {
     f128 a0, a1, b0, b1;
-    f128_mul_build(rho, tab, CL);
-    fold4(A, x, tab, CL, a0, a1);
-    fold4(B, x, tab, CL, b0, b1);
+    fold4(A, x, nt, NL, a0, a1);
+    fold4(B, x, nt, NL, b0, b1);
     A2[2 * x] = a0;
     A2[2 * x + 1] = a1;
     B2[2 * x] = b0;
     B2[2 * x + 1] = b1;
     const f128 p1 = f128_mul(a1, b1, tab, CL);
     const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
-    const f128 e = E[x];
-    if (E_next && !(x & 1)) E_next[x >> 1] = f128_xor(e, E[x + 1]);
-    f128_mul_build(e, tab, CL);
+    f128_mul_build(E[x], tab, CL);
     g1 = f128_mul_comb(p1, tab, CL);
     gi = f128_mul_comb(pi, tab, CL);
   }
@@ -476,15 +480,18 @@ __global__ __launch_bounds__(SMALL_THREADS) void lig_fold_msg_small_kernel(const
                                                                            f128 *__restrict__ out, uint32_t *flag,
                                                                            uint32_t seq) {
   __shared__ f128 mt[SMALL_THREADS / 64][1024];
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
+  const NibtabLane NL(threadIdx.x & 15u);
   const uint32_t x = threadIdx.x;
+  if (x < 64) build_nibtab<0>(r, nt, x);
+  __syncthreads();
   f128 u0 = f128_zero(), u2 = f128_zero();
   if (x < half) {
     f128 f0, f1, b0, b1;
-    f128_mul_build(r, tab, CL);
-    fold4(f, x, tab, CL, f0, f1);
-    fold4(b, x, tab, CL, b0, b1);
+    fold4(f, x, nt, NL, f0, f1);
+    fold4(b, x, nt, NL, b0, b1);
     f2[2 * x] = f0;
     f2[2 * x + 1] = f1;
     b2[2 * x] = b0;
@@ -509,23 +516,31 @@ __global__ __launch_bounds__(MUL_THREADS) void zc_fold_round_kernel(const f128 *
   const f128 rho = rho_dev ? *rho_dev : rho_arg;
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
+  const NibtabLane NL(threadIdx.x & 15u);
+  if (threadIdx.x < 64) build_nibtab<0>(rho, nt, threadIdx.x);
+  __syncthreads();
+  // the next round's eq table first, in a loop of its own: a lane-dependent
+  // store in the products' loop leaves the compiler holding more than 256
+  // registers (AGPR spills, one wavefront per SIMD)
+  if (E_next)
+    for (uint64_t y = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; y < half / 2;
+         y += (uint64_t)gridDim.x * MUL_THREADS)
+      E_next[y] = f128_xor(E[2 * y], E[2 * y + 1]);
   f128 g1 = f128_zero(), gi = f128_zero();
   for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
     f128 a0, a1, b0, b1;
-    f128_mul_build(rho, tab, CL);
-    fold4(A, x, tab, CL, a0, a1);
-    fold4(B, x, tab, CL, b0, b1);
+    fold4(A, x, nt, NL, a0, a1);
+    fold4(B, x, nt, NL, b0, b1);
     A2[2 * x] = a0;
     A2[2 * x + 1] = a1;
     B2[2 * x] = b0;
     B2[2 * x + 1] = b1;
     const f128 p1 = f128_mul(a1, b1, tab, CL);
     const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
-    const f128 e = E[x];
-    if (E_next && !(x & 1)) E_next[x >> 1] = f128_xor(e, E[x + 1]);
-    f128_mul_build(e, tab, CL);
+    f128_mul_build(E[x], tab, CL);
     f128_xor_into(g1, f128_mul_comb(p1, tab, CL));
     f128_xor_into(gi, f128_mul_comb(pi, tab, CL));
   }
@@ -539,14 +554,17 @@ __global__ __launch_bounds__(MUL_THREADS) void lig_fold_msg_kernel(const f128 *_
                                                                    f128 *__restrict__ partial) {
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
+  __shared__ __align__(16) char nt[NIBTAB_BYTES];
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
+  const NibtabLane NL(threadIdx.x & 15u);
+  if (threadIdx.x < 64) build_nibtab<0>(r, nt, threadIdx.x);
+  __syncthreads();
   f128 u0 = f128_zero(), u2 = f128_zero();
   for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
     f128 f0, f1, b0, b1;
-    f128_mul_build(r, tab, CL);
-    fold4(f, x, tab, CL, f0, f1);
-    fold4(b, x, tab, CL, b0, b1);
+    fold4(f, x, nt, NL, f0, f1);
+    fold4(b, x, nt, NL, b0, b1);
     f2[2 * x] = f0;
     f2[2 * x + 1] = f1;
     b2[2 * x] = b0;
@@ -617,25 +635,34 @@ int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, 
 // round_msg_lsb (pcs/ligerito.rs:3725-3772): u0 = sum f0 b0,
 // u2 = sum (f0 + f1)(b0 + b1); with_eval adds y = sum f0 b0 + f1 b1
 // (round_msg_and_eval_lsb, :3775-3816).
-__global__ __launch_bounds__(MUL_THREADS) void lig_msg_kernel(const f128 *__restrict__ f, const f128 *__restrict__ b,
-                                                              uint64_t half, int with_eval,
-                                                              f128 *__restrict__ partial) {
+// (The 3-wavefront minimum caps the registers at 168: left at the LDS's two
+// wavefronts per SIMD, the scheduler fills 256 and spills into AGPRs, which
+// leaves one.)
+template <bool EVAL>
+__global__ __launch_bounds__(MUL_THREADS, 2) void lig_msg_kernel(const f128 *__restrict__ f, const f128 *__restrict__ b,
+                                                              uint64_t half, f128 *__restrict__ partial) {
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
   const uint32_t lane = threadIdx.x & 63;
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
-  f128 u0 = f128_zero(), u2 = f128_zero(), y = f128_zero();
+  // the sums stay unreduced (256-bit) until the end; y = u0 + sum f1 b1
+  F256w u0 = {}, u2 = {}, y1 = {};
   for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
     const f128 f0 = f[2 * x], f1 = f[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
-    const f128 e0 = f128_mul(f0, b0, tab, CL);
-    f128_xor_into(u0, e0);
-    f128_xor_into(u2, f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, CL));
-    if (with_eval) f128_xor_into(y, f128_xor(e0, f128_mul(f1, b1, tab, CL)));
+    f128_mul_build(f0, tab, CL);
+    f128_mul_comb_acc(b0, tab, CL, u0);
+    f128_mul_build(f128_xor(f0, f1), tab, CL);
+    f128_mul_comb_acc(f128_xor(b0, b1), tab, CL, u2);
+    if constexpr (EVAL) {
+      f128_mul_build(f1, tab, CL);
+      f128_mul_comb_acc(b1, tab, CL, y1);
+    }
   }
-  wg_xor_reduce(u0, red, &partial[3 * blockIdx.x]);
-  wg_xor_reduce(u2, red, &partial[3 * blockIdx.x + 1]);
-  wg_xor_reduce(y, red, &partial[3 * blockIdx.x + 2]);
+  const f128 r0 = f128_reduce256w(u0);
+  wg_xor_reduce(r0, red, &partial[3 * blockIdx.x]);
+  wg_xor_reduce(f128_reduce256w(u2), red, &partial[3 * blockIdx.x + 1]);
+  wg_xor_reduce(EVAL ? f128_xor(r0, f128_reduce256w(y1)) : f128_zero(), red, &partial[3 * blockIdx.x + 2]);
 }
 
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
@@ -644,8 +671,12 @@ int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint6
   unsigned g = grid_for(half, MUL_THREADS, 4096);
   f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
-  hipLaunchKernelGGL(lig_msg_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b, half,
-                     with_eval, part);
+  if (with_eval)
+    hipLaunchKernelGGL(lig_msg_kernel<true>, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b, half,
+                       part);
+  else
+    hipLaunchKernelGGL(lig_msg_kernel<false>, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b,
+                       half, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
   return xor_rows_device(ctx, st, part, g, 3, out3, seq);
 }
