@@ -73,4 +73,18 @@ __device__ __forceinline__ void compress_full(const uint32_t cv[8], const uint32
   out[12] = s12 ^ cv[4]; out[13] = s13 ^ cv[5]; out[14] = s14 ^ cv[6]; out[15] = s15 ^ cv[7];
 }
 
+// Proof of work (challenger.rs:1246-1300): BLAKE3(digest | nonce_le | 0^24)
+// as one CHUNK_START|CHUNK_END|ROOT block; the leading `bits` bits of its
+// output bytes must be zero.
+__device__ __forceinline__ bool pow_ok(const uint32_t (&dg)[8], uint64_t nonce, uint32_t bits) {
+  uint32_t m[16] = {dg[0], dg[1], dg[2], dg[3], dg[4], dg[5], dg[6], dg[7], (uint32_t)nonce, (uint32_t)(nonce >> 32),
+                    0, 0, 0, 0, 0, 0};
+  uint32_t cv[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) cv[i] = IV[i];
+  compress_cv(cv, m, 0, 64, CHUNK_START | CHUNK_END | ROOT);
+  const uint64_t be = ((uint64_t)__builtin_bswap32(cv[0]) << 32) | __builtin_bswap32(cv[1]);
+  return bits == 0 || (bits <= 64 && (be >> (64 - bits)) == 0);
+}
+
 }  // namespace b3

@@ -13,4 +13,22 @@ constexpr size_t FS_STATE_BYTES = 1848;
 // sum over the set bits t of v of rho ws[8 p + t] (2048 F128).
 int rho_bytetab_device(Ctx *ctx, hipStream_t st, const void *d_ws, const void *d_rho, void *d_out);
 
+// The device side of a Ligerito fold round's challenge with the transcript
+// state at d_fs: with bits > 0, the proof of work (Transcript::state_digest,
+// the smallest nonce whose BLAKE3(digest | nonce_le | 0^24) starts with
+// `bits` zero bits, Transcript::observe_bytes of the nonce, into *d_nonce),
+// then Transcript::sample into *d_r.  d_work: FLOCK_POW_WORK_BYTES of device
+// memory the steps share: the scan's best nonce in its first 8 bytes (~0
+// before the first call; every call leaves it so) and the state's digest in
+// bytes 32..63 (lig_pow_digest(d_work)); with digest_ready, a proof of work of
+// more than FLOCK_POW_WAVE_BITS bits takes the digest already there (written
+// by the previous message's transcript step, or uploaded) instead of
+// computing it.  A proof of work that finds no nonce in its bounded search
+// sets *d_status to 1 (and carries on with nonce 0).
+constexpr size_t FLOCK_POW_WORK_BYTES = 64;
+constexpr uint32_t FLOCK_POW_WAVE_BITS = 6;
+inline void *lig_pow_digest(void *d_work) { return (uint8_t *)d_work + 32; }
+int lig_challenge_device(Ctx *ctx, hipStream_t st, void *d_fs, uint32_t bits, void *d_work, bool digest_ready,
+                         void *d_nonce, void *d_r, void *d_status);
+
 }  // namespace flock

@@ -36,11 +36,14 @@ int signal_device(Ctx *ctx, hipStream_t st, void *d_flag, uint32_t seq);
 struct FsStep {
   void *state = nullptr;
   void *rho = nullptr;
+  void *digest = nullptr;  // observe-only steps: the state's digest then goes here (32 bytes)
 };
 // out[w] = xor_r in[r * width + w] (width <= 4); a nonzero seq is then
 // published at the context's readback flag (see Ctx::READBACK_FLAG), which
 // the reductions ending in the readback area use in place of signal_device.
-// With fs.state (width 2) the kernel then runs the transcript step over out.
+// With fs.state the kernel then runs the transcript step over out[0],
+// out[1]: observe both and sample into fs.rho (width 2), or, with fs.rho
+// null, observe them only (width 2 or 3).
 int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, uint32_t width, void *out,
                     uint32_t seq = 0, const FsStep &fs = FsStep());
 // Zerocheck round 1, product part, in two steps: zc_y_device computes
@@ -92,10 +95,13 @@ constexpr uint64_t SMALL_PAIRS = 256;
 int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B, F128 rho, const void *E,
                          uint64_t half, void *A2, void *B2, void *E_next, void *out2, uint32_t seq,
                          const void *d_rho = nullptr, const FsStep &fs = FsStep());
+// With d_r the Ligerito fold reads its challenge from device memory, and
+// with fs.state (fs.rho null) the message's reduction then observes (u0, u2)
+// in the device transcript.
 int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half, void *f2,
-                        void *b2, void *out3, uint32_t seq);
+                        void *b2, void *out3, uint32_t seq, const void *d_r = nullptr, const FsStep &fs = FsStep());
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
-                   uint32_t seq = 0);
+                   uint32_t seq = 0, const FsStep &fs = FsStep());
 int axpy_device(Ctx *ctx, hipStream_t st, void *acc, const void *x, uint64_t n, F128 beta);
 // A transposed R1CS matrix on the device for the lincheck combination: the
 // nonzero rows of column c are rows[ptr[c] .. ptr[c + 1]) (16-bit, ascending),

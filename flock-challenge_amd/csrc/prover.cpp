@@ -1229,22 +1229,119 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
       push(o3[0], o3[1]);
       return FLOCK_OK;
     };
+    // The same fold and message with the challenge in device memory, the
+    // message written to out3 (device) and observed by the device transcript.
+    const bool dfs_lig = ctx->device_transcript;
+    constexpr int LIG_MAX_FOLDS = 16;
+    uint8_t *d_lfs = nullptr, *d_lwork = nullptr, *d_lmsg = nullptr, *d_lr = nullptr, *d_lnonce = nullptr;
+    if (dfs_lig) {
+      d_lfs = (uint8_t *)P.dev("fs.state", FS_STATE_BYTES);
+      d_lwork = (uint8_t *)P.dev("lig.fs.work", FLOCK_POW_WORK_BYTES + 16);  // + the status word
+      d_lmsg = (uint8_t *)P.dev("lig.fs.msgs", 48 * LIG_MAX_FOLDS);
+      d_lr = (uint8_t *)P.dev("lig.fs.r", 16 * LIG_MAX_FOLDS);
+      d_lnonce = (uint8_t *)P.dev("lig.fs.nonce", 8 * LIG_MAX_FOLDS);
+      if (!d_lfs || !d_lwork || !d_lmsg || !d_lr || !d_lnonce) return FLOCK_E_OOM;
+    }
+    auto fold_dev = [&](const uint8_t *d_r, uint8_t *out3, void *d_digest) -> int {
+      void *nf = (f == fb) ? fc : fb, *ncb = (cur_cb == cbb) ? cbc : cbb;
+      const FsStep fs{d_lfs, nullptr, d_digest};
+      int e;
+      if (len >= 4 && len / 4 <= (1u << 16)) {
+        e = lig_fold_msg_device(ctx, st, f, cur_cb, F128(), len / 4, nf, ncb, out3, 0, d_r, fs);
+        f = nf;
+        cur_cb = ncb;
+        len /= 2;
+        return e;
+      }
+      if ((e = fold_pair_device(ctx, st, f, cur_cb, len / 2, F128(), nf, ncb, d_r))) return e;
+      f = nf;
+      cur_cb = ncb;
+      len /= 2;
+      return lig_msg_device(ctx, st, f, cur_cb, len / 2, 0, out3, 0, fs);
+    };
+    // A level's `count` fold rounds: round j's proof of work (fold_bits - j
+    // bits, when positive), its challenge, the fold and the next message
+    // (pcs/ligerito.rs:6300-6340).  With the device transcript the rounds
+    // run on the device back to back: the state goes up once, and the
+    // messages, nonces and the state come back once.
+    auto fold_rounds = [&](int fold_bits, int count) -> int {
+      if (!dfs_lig) {
+        for (int j = 0; j < count; ++j) {
+          const int bits = fold_bits - j;
+          if (bits > 0) {
+            uint64_t nc;
+            if (int e = P.grind(t, (uint32_t)bits, &nc)) return e;
+            fold_nonces.push_back(nc);
+          }
+          if (int e = fold(t.sample())) return e;
+        }
+        return FLOCK_OK;
+      }
+      if (count > LIG_MAX_FOLDS) {
+        ctx->last_error = "ligerito: too many fold rounds in one level";
+        return FLOCK_E_SHAPE;
+      }
+      uint8_t init[FS_STATE_BYTES];
+      t.export_state(init);
+      if (int e = P.h2d(d_lfs, init, FS_STATE_BYTES)) return e;
+      // the scan's best nonce (none yet), the first round's digest and the
+      // status word (0)
+      uint8_t work[FLOCK_POW_WORK_BYTES + 16];
+      memset(work, 0, sizeof work);
+      memset(work, 0xFF, 8);
+      t.state_digest(work + 32);
+      if (int e = P.h2d(d_lwork, work, sizeof work)) return e;
+      uint8_t *d_status = d_lwork + FLOCK_POW_WORK_BYTES;
+      auto grid_pow = [&](int j) { return fold_bits - j > (int)FLOCK_POW_WAVE_BITS; };
+      for (int j = 0; j < count; ++j) {
+        const int bits = fold_bits - j;
+        if (int e = lig_challenge_device(ctx, st, d_lfs, bits > 0 ? (uint32_t)bits : 0u, d_lwork, true,
+                                         d_lnonce + 8 * j, d_lr + 16 * j, d_status))
+          return e;
+        // the message's transcript step leaves the next proof of work's base
+        void *next_dg = j + 1 < count && grid_pow(j + 1) ? lig_pow_digest(d_lwork) : nullptr;
+        if (int e = fold_dev(d_lr + 16 * j, d_lmsg + 48 * j, next_dg)) return e;
+      }
+      const size_t nm = 48 * (size_t)count, nn = 8 * (size_t)count;
+      if (nm + nn + 16 + FS_STATE_BYTES > ctx->landing_bytes) {
+        ctx->last_error = "ligerito: landing area too small";
+        return FLOCK_E_DEVICE;
+      }
+      uint8_t *land = ctx->landing;
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land, d_lmsg, nm, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm, d_lnonce, nn, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm + nn, d_status, 16, hipMemcpyDeviceToHost, st));
+      FLOCK_HIP_CHECK(ctx, hipMemcpyAsync(land + nm + nn + 16, d_lfs, FS_STATE_BYTES, hipMemcpyDeviceToHost, st));
+      {
+        HostPhase hp(ctx, "wait_d2h");
+        if (int e = P.wait_signal()) return e;
+      }
+      uint32_t status;
+      memcpy(&status, land + nm + nn, 4);
+      if (status) {
+        ctx->last_error = "ligerito: the proof of work found no nonce in its search";
+        return FLOCK_E_DEVICE;
+      }
+      for (int j = 0; j < count; ++j) {
+        if (fold_bits - j > 0) {
+          uint64_t nc;
+          memcpy(&nc, land + nm + 8 * j, 8);
+          fold_nonces.push_back(nc);
+        }
+        F128 u0, u2;
+        memcpy(&u0, land + 48 * j, 16);
+        memcpy(&u2, land + 48 * j + 16, 16);
+        msgs.push_back(u0);  // observed on the device
+        msgs.push_back(u2);
+      }
+      t.import_state(land + nm + nn + 16);
+      return FLOCK_OK;
+    };
     auto glue = [&](const void *bn, F128 be) { return axpy_device(ctx, st, cur_cb, bn, len, be); };
     if ((rc = P.msg(f, cur_cb, len, 0, o3))) return rc;
     push(o3[0], o3[1]);
     const int ik = cfg->initial_k;
-    std::vector<F128> ris;
-    for (int j = 0; j < ik; ++j) {
-      int bits = cfg->levels[0].fold_grinding_bits - j;
-      if (bits > 0) {
-        uint64_t nc;
-        if ((rc = P.grind(t, (uint32_t)bits, &nc))) return rc;
-        fold_nonces.push_back(nc);
-      }
-      F128 rj = t.sample();
-      ris.push_back(rj);
-      if ((rc = fold(rj))) return rc;
-    }
+    if ((rc = fold_rounds(cfg->levels[0].fold_grinding_bits, ik))) return rc;
     const int n1 = L - ik;
     Witness prev;
     prev.cw = cw0;
@@ -1293,17 +1390,7 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
     const int R = cfg->n_levels - 1;
     for (int i = 0; i < R; ++i) {
       const int ki = cfg->levels[i + 1].k_recursive;
-      std::vector<F128> lr;
-      for (int j = 0; j < ki; ++j) {
-        int bits = cfg->levels[i + 1].fold_grinding_bits - j;
-        if (bits > 0) {
-          if ((rc = P.grind(t, (uint32_t)bits, &nc))) return rc;
-          fold_nonces.push_back(nc);
-        }
-        F128 rj = t.sample();
-        lr.push_back(rj);
-        if ((rc = fold(rj))) return rc;
-      }
+      if ((rc = fold_rounds(cfg->levels[i + 1].fold_grinding_bits, ki))) return rc;
       if (i == R - 1) {
         yr.resize(len);
         if ((rc = P.d2h(yr.data(), f, len * 16))) return rc;

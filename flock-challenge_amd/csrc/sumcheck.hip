@@ -474,12 +474,15 @@ __global__ __launch_bounds__(SMALL_THREADS) void zc_fold_round_small_kernel(
 // Ligerito round over `half` pairs after folding f, b (4 half entries each)
 // by r into f2, b2: (u0, u2) as lig_msg_kernel without the evaluation.
 __global__ __launch_bounds__(SMALL_THREADS) void lig_fold_msg_small_kernel(const f128 *__restrict__ f,
-                                                                           const f128 *__restrict__ b, f128 r,
+                                                                           const f128 *__restrict__ b, f128 r_arg,
+                                                                           const f128 *__restrict__ r_dev,
                                                                            uint32_t half, f128 *__restrict__ f2,
                                                                            f128 *__restrict__ b2,
                                                                            f128 *__restrict__ out, uint32_t *flag,
-                                                                           uint32_t seq) {
+                                                                           uint32_t seq, FsState *fs, uint32_t *dg) {
+  const f128 r = r_dev ? *r_dev : r_arg;
   __shared__ f128 mt[SMALL_THREADS / 64][1024];
+  __shared__ FsState fss;
   __shared__ __align__(16) char nt[NIBTAB_BYTES];
   f128 *tab = &mt[0][0];
   const CombLane CL(threadIdx.x);
@@ -500,6 +503,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void lig_fold_msg_small_kernel(const
     u2 = f128_mul(f128_xor(f0, f1), f128_xor(b0, b1), tab, CL);
   }
   small_publish(u0, u2, tab, out, 3, flag, seq);
+  if (fs) fs_observe2_block(fs, fss, out, dg);  // thread 0 reads back what it wrote
 }
 
 // The same fused rounds over more pairs: a grid of MUL_THREADS-thread
@@ -549,9 +553,11 @@ __global__ __launch_bounds__(MUL_THREADS) void zc_fold_round_kernel(const f128 *
 }
 
 __global__ __launch_bounds__(MUL_THREADS) void lig_fold_msg_kernel(const f128 *__restrict__ f,
-                                                                   const f128 *__restrict__ b, f128 r, uint64_t half,
+                                                                   const f128 *__restrict__ b, f128 r_arg,
+                                                                   const f128 *__restrict__ r_dev, uint64_t half,
                                                                    f128 *__restrict__ f2, f128 *__restrict__ b2,
                                                                    f128 *__restrict__ partial) {
+  const f128 r = r_dev ? *r_dev : r_arg;
   __shared__ f128 mt[MUL_WAVES][1024];
   __shared__ f128 red[MUL_THREADS];
   __shared__ __align__(16) char nt[NIBTAB_BYTES];
@@ -609,7 +615,11 @@ int zc_fold_round_device(Ctx *ctx, hipStream_t st, const void *A, const void *B,
 }
 
 int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, F128 r, uint64_t half,
-                              void *f2, void *b2, void *out3, uint32_t seq) {
+                              void *f2, void *b2, void *out3, uint32_t seq, const void *d_r, const FsStep &fs) {
+  if (fs.rho) {
+    ctx->last_error = "ligerito: a round's transcript step observes its message only";
+    return FLOCK_E_ARG;
+  }
   ScopedPhase ph(ctx, "lig_msg", st);
   if (half == 0) {
     ctx->last_error = "ligerito: fused round needs at least one pair";
@@ -617,8 +627,8 @@ int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, 
   }
   if (half <= SMALL_PAIRS) {
     hipLaunchKernelGGL(lig_fold_msg_small_kernel, dim3(1), dim3(SMALL_THREADS), 0, st, (const f128 *)f,
-                       (const f128 *)b, to_f128(r), (uint32_t)half, (f128 *)f2, (f128 *)b2, (f128 *)out3,
-                       ctx->readback_flag_dev(), seq);
+                       (const f128 *)b, to_f128(r), (const f128 *)d_r, (uint32_t)half, (f128 *)f2, (f128 *)b2,
+                       (f128 *)out3, ctx->readback_flag_dev(), seq, (FsState *)fs.state, (uint32_t *)fs.digest);
     FLOCK_HIP_CHECK(ctx, hipGetLastError());
     return FLOCK_OK;
   }
@@ -626,17 +636,17 @@ int lig_fold_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, 
   f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
   hipLaunchKernelGGL(lig_fold_msg_kernel, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b,
-                     to_f128(r), half, (f128 *)f2, (f128 *)b2, part);
+                     to_f128(r), (const f128 *)d_r, half, (f128 *)f2, (f128 *)b2, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 3, out3, seq);
+  return xor_rows_device(ctx, st, part, g, 3, out3, seq, fs);
 }
 
 // ---------------------------------------------------------------- Ligerito sumcheck
 // round_msg_lsb (pcs/ligerito.rs:3725-3772): u0 = sum f0 b0,
 // u2 = sum (f0 + f1)(b0 + b1); with_eval adds y = sum f0 b0 + f1 b1
 // (round_msg_and_eval_lsb, :3775-3816).
-// (The 3-wavefront minimum caps the registers at 168: left at the LDS's two
-// wavefronts per SIMD, the scheduler fills 256 and spills into AGPRs, which
+// (The launch bounds ask for the two wavefronts per SIMD the LDS allows:
+// left to itself, the scheduler goes past 256 registers into AGPRs, which
 // leaves one.)
 template <bool EVAL>
 __global__ __launch_bounds__(MUL_THREADS, 2) void lig_msg_kernel(const f128 *__restrict__ f, const f128 *__restrict__ b,
@@ -666,7 +676,11 @@ __global__ __launch_bounds__(MUL_THREADS, 2) void lig_msg_kernel(const f128 *__r
 }
 
 int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint64_t half, int with_eval, void *out3,
-                   uint32_t seq) {
+                   uint32_t seq, const FsStep &fs) {
+  if (fs.rho) {
+    ctx->last_error = "ligerito: a round's transcript step observes its message only";
+    return FLOCK_E_ARG;
+  }
   ScopedPhase ph(ctx, "lig_msg", st);
   unsigned g = grid_for(half, MUL_THREADS, 4096);
   f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
@@ -678,7 +692,7 @@ int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint6
     hipLaunchKernelGGL(lig_msg_kernel<false>, dim3(g), dim3(MUL_THREADS), 0, st, (const f128 *)f, (const f128 *)b,
                        half, part);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return xor_rows_device(ctx, st, part, g, 3, out3, seq);
+  return xor_rows_device(ctx, st, part, g, 3, out3, seq, fs);
 }
 
 // acc[i] += beta * x[i]  (SumcheckProver::glue, pcs/ligerito.rs:5766-5790),
@@ -1020,23 +1034,15 @@ int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, 
 }
 
 // ---------------------------------------------------------------- proof of work
-// BLAKE3(digest | nonce_le | 0^24) as one CHUNK_START|CHUNK_END|ROOT block; the
-// leading `bits` bits of the digest bytes must be zero (challenger.rs:1246-1300).
-// The smallest satisfying nonce of the window wins (atomicMin).
+// The nonces [start, start + count) through b3::pow_ok; the smallest
+// satisfying one of the window wins (atomicMin).
 __global__ __launch_bounds__(256) void pow_kernel(uint4 d0, uint4 d1, uint64_t start, uint64_t count, uint32_t bits,
                                                   unsigned long long *__restrict__ best) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= count) return;
   const uint64_t nonce = start + t;
-  uint32_t m[16] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w, (uint32_t)nonce, (uint32_t)(nonce >> 32),
-                    0, 0, 0, 0, 0, 0};
-  uint32_t cv[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) cv[i] = b3::IV[i];
-  b3::compress_cv(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT);
-  const uint64_t be = ((uint64_t)__builtin_bswap32(cv[0]) << 32) | __builtin_bswap32(cv[1]);
-  const bool ok = bits == 0 || (bits <= 64 && (be >> (64 - bits)) == 0);
-  if (ok) atomicMin(best, (unsigned long long)nonce);
+  const uint32_t dg[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+  if (b3::pow_ok(dg, nonce, bits)) atomicMin(best, (unsigned long long)nonce);
 }
 
 int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,

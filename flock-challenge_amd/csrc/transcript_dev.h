@@ -235,13 +235,14 @@ __device__ __forceinline__ void fs_absorb(FsState &s, FsRegs &r, const uint32_t 
   wave_lds_sync();
 }
 
-// The first 16 bytes of the root output of the current state
-// (Hasher::finalize_xof): the open chunk's output merged with the stack,
-// then the root compression at output block 0.  Returns them in every lane.
-__device__ __forceinline__ void fs_squeeze16(const FsState &s, const FsRegs &r, uint32_t *pm, uint32_t lane,
-                                             uint32_t out[4]) {
+// The root compression of the current state (Hasher::finalize_xof, output
+// block 0): the open chunk's output merged with the stack, then the root
+// compression.  Lane i of each quad gets its output words i, 4 + i (o[0],
+// o[1]).
+__device__ __forceinline__ void fs_root_output(const FsState &s, const FsRegs &r, uint32_t *pm, uint32_t lane,
+                                               uint32_t o[4]) {
   const uint32_t qi = lane & 3u;
-  uint32_t cv[8], o[4];
+  uint32_t cv[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) cv[i] = r.cv[i];
   const uint32_t *m = (const uint32_t *)s.block;
@@ -267,8 +268,25 @@ __device__ __forceinline__ void fs_squeeze16(const FsState &s, const FsRegs &r, 
     flags = b3::PARENT;
   }
   compress_quad(cv, m, 0, len, flags | b3::ROOT, qi, o);
+}
+
+// The first 16 bytes of the root output (Hasher::finalize_xof of 16 bytes),
+// in every lane.
+__device__ __forceinline__ void fs_squeeze16(const FsState &s, const FsRegs &r, uint32_t *pm, uint32_t lane,
+                                             uint32_t out[4]) {
+  uint32_t o[4];
+  fs_root_output(s, r, pm, lane, o);
 #pragma unroll
   for (int k = 0; k < 4; ++k) out[k] = __builtin_amdgcn_readlane(o[0], k);
+}
+
+// The first 32 bytes (Transcript::state_digest, the proof-of-work base), in
+// every lane.
+__device__ __forceinline__ void fs_digest32(const FsState &s, const FsRegs &r, uint32_t *pm, uint32_t lane,
+                                            uint32_t (&out)[8]) {
+  uint32_t o[4];
+  fs_root_output(s, r, pm, lane, o);
+  quad_cv(o, out);
 }
 
 // One zerocheck round's transcript steps, Transcript::observe of the two
@@ -287,11 +305,39 @@ __device__ __forceinline__ f128 fs_round2(FsState &s, FsRegs &r, const f128 m0, 
   return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
-// One zerocheck round's transcript steps, run by a workgroup: the state at g
-// is staged through the LDS copy s, wavefront 0 observes msg[0], msg[1] and
-// samples the challenge into *rho (lane 0 stores it), and the state goes
-// back to g.  msg may be memory thread 0 itself has just written.
-__device__ __forceinline__ void fs_round_block(FsState *g, FsState &s, const f128 *msg, f128 *rho) {
+// Transcript::observe of m0 and m1 (tag bytes 3, 1, then the value's 16
+// bytes, each), run by one wavefront.
+__device__ __forceinline__ void fs_observe2(FsState &s, FsRegs &r, const f128 m0, const f128 m1, uint32_t *pm,
+                                            uint32_t lane) {
+  const uint32_t w[9] = {3u | (1u << 8) | (m0.x << 16), (m0.x >> 16) | (m0.y << 16), (m0.y >> 16) | (m0.z << 16),
+                         (m0.z >> 16) | (m0.w << 16),   (m0.w >> 16) | (3u << 16) | (1u << 24), m1.x, m1.y, m1.z,
+                         m1.w};
+  fs_absorb<36>(s, r, w, pm, lane);
+}
+
+// Transcript::sample of one F128 (tag bytes 4, 1, squeeze 16 bytes, absorb
+// them), run by one wavefront.
+__device__ __forceinline__ f128 fs_sample(FsState &s, FsRegs &r, uint32_t *pm, uint32_t lane) {
+  const uint32_t t[1] = {4u | (1u << 8)};
+  fs_absorb<2>(s, r, t, pm, lane);
+  uint32_t v[4];
+  fs_squeeze16(s, r, pm, lane, v);
+  fs_absorb<16>(s, r, v, pm, lane);
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+// Transcript::observe_bytes of a u64 (tag byte 5, the length 8 as a u64,
+// the 8 bytes), run by one wavefront.
+__device__ __forceinline__ void fs_observe_u64(FsState &s, FsRegs &r, uint64_t v, uint32_t *pm, uint32_t lane) {
+  const uint32_t w[5] = {5u | (8u << 8), 0u, (uint32_t)(v << 8), (uint32_t)(v >> 24), (uint32_t)(v >> 56)};
+  fs_absorb<17>(s, r, w, pm, lane);
+}
+
+// A transcript step run by a workgroup: the state at g is staged through
+// the LDS copy s, wavefront 0 runs body(s, r, pm, lane) and stores the
+// scalar fields back, and the state returns to g.
+template <class Body>
+__device__ __forceinline__ void fs_block(FsState *g, FsState &s, Body body) {
   __shared__ uint32_t pm[16];
   const uint32_t *src = (const uint32_t *)g;
   uint32_t *dst = (uint32_t *)&s;
@@ -309,15 +355,42 @@ __device__ __forceinline__ void fs_round_block(FsState *g, FsState &s, const f12
     r.len = __builtin_amdgcn_readfirstlane(r.len);
     r.done = __builtin_amdgcn_readfirstlane(r.done);
     r.depth = __builtin_amdgcn_readfirstlane(r.depth);
-    const f128 rv = fs_round2(s, r, msg[0], msg[1], pm, lane);
-    if (lane == 0) {
-      *rho = rv;
-      fs_regs_store(s, r);
-    }
+    body(s, r, pm, lane);
+    if (lane == 0) fs_regs_store(s, r);
   }
   __syncthreads();
   uint32_t *back = (uint32_t *)g;
   for (uint32_t i = threadIdx.x; i < sizeof(FsState) / 4; i += blockDim.x) back[i] = dst[i];
+}
+
+// Observe msg[0], msg[1] with the state at g (a Ligerito round message: its
+// challenge comes after the next proof of work).  msg may be memory thread 0
+// itself has just written.
+// With dg, the state's digest (the next proof of work's base) then goes to
+// dg[0..7].
+__device__ __forceinline__ void fs_observe2_block(FsState *g, FsState &s, const f128 *msg, uint32_t *dg = nullptr) {
+  fs_block(g, s, [&](FsState &st, FsRegs &r, uint32_t *pm, uint32_t lane) {
+    fs_observe2(st, r, msg[0], msg[1], pm, lane);
+    if (dg) {
+      uint32_t d[8];
+      fs_digest32(st, r, pm, lane, d);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dg[i] = d[i];
+      }
+    }
+  });
+}
+
+// One zerocheck round's transcript steps, run by a workgroup: the state at g
+// is staged through the LDS copy s, wavefront 0 observes msg[0], msg[1] and
+// samples the challenge into *rho (lane 0 stores it), and the state goes
+// back to g.  msg may be memory thread 0 itself has just written.
+__device__ __forceinline__ void fs_round_block(FsState *g, FsState &s, const f128 *msg, f128 *rho) {
+  fs_block(g, s, [&](FsState &st, FsRegs &r, uint32_t *pm, uint32_t lane) {
+    const f128 rv = fs_round2(st, r, msg[0], msg[1], pm, lane);
+    if (lane == 0) *rho = rv;
+  });
 }
 
 }  // namespace flock

@@ -200,7 +200,8 @@ __global__ __launch_bounds__(ZCY_THREADS) void zc_y_kernel(const uint64_t *__res
 // then absorbs out into the transcript state and samples *rho.
 template <int W>
 __global__ __launch_bounds__(256) void xor_rows_kernel(const f128 *__restrict__ in, uint64_t count, f128 *out,
-                                                       uint32_t *flag, uint32_t seq, FsState *fs, f128 *rho) {
+                                                       uint32_t *flag, uint32_t seq, FsState *fs, f128 *rho,
+                                                       uint32_t *dg) {
   __shared__ f128 red[W][256];
   const uint32_t tid = threadIdx.x;
   f128 s[W];
@@ -236,9 +237,14 @@ __global__ __launch_bounds__(256) void xor_rows_kernel(const f128 *__restrict__ 
       __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
-  if constexpr (W == 2) {
+  if constexpr (W == 2 || W == 3) {
     __shared__ FsState fss;
-    if (fs) fs_round_block(fs, fss, out, rho);  // thread 0 reads back what it wrote
+    if (fs) {  // thread 0 reads back what it wrote
+      if (rho)
+        fs_round_block(fs, fss, out, rho);
+      else
+        fs_observe2_block(fs, fss, out, dg);
+    }
   }
 }
 
@@ -247,17 +253,18 @@ int xor_rows_device(Ctx *ctx, hipStream_t st, const void *in, uint64_t count, ui
   const f128 *i = (const f128 *)in;
   f128 *o = (f128 *)out;
   uint32_t *flag = ctx->readback_flag_dev();
-  if (fs.state && width != 2) {
+  if (fs.state && width != 2 && !(width == 3 && !fs.rho)) {
     ctx->last_error = "xor_rows: the transcript step takes two values";
     return FLOCK_E_ARG;
   }
   FsState *S = (FsState *)fs.state;
   f128 *R = (f128 *)fs.rho;
+  uint32_t *D = (uint32_t *)fs.digest;
   switch (width) {
-    case 1: hipLaunchKernelGGL(xor_rows_kernel<1>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
-    case 2: hipLaunchKernelGGL(xor_rows_kernel<2>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
-    case 3: hipLaunchKernelGGL(xor_rows_kernel<3>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
-    case 4: hipLaunchKernelGGL(xor_rows_kernel<4>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R); break;
+    case 1: hipLaunchKernelGGL(xor_rows_kernel<1>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R, D); break;
+    case 2: hipLaunchKernelGGL(xor_rows_kernel<2>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R, D); break;
+    case 3: hipLaunchKernelGGL(xor_rows_kernel<3>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R, D); break;
+    case 4: hipLaunchKernelGGL(xor_rows_kernel<4>, dim3(1), dim3(256), 0, st, i, count, o, flag, seq, S, R, D); break;
     default:
       ctx->last_error = "xor_rows: width out of range";
       return FLOCK_E_SHAPE;

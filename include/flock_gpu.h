@@ -71,14 +71,17 @@ const char *flock_last_error(const flock_ctx *ctx);
 int flock_ctx_set_wait_timeout(flock_ctx *ctx, double seconds);
 
 /* Context options (the proof bytes never depend on them):
- *  "device_transcript" (default 0): 1 runs the zerocheck rounds' Fiat-Shamir
- *    steps on the device (the BLAKE3 transcript state moves to device memory
- *    and back; each round's reduction kernel absorbs the round's message and
- *    samples its challenge, each BLAKE3 compression spread over a quad of
- *    lanes) instead of one host round trip per round.  The hashing costs
- *    about what the round trip does, so whole proofs measure within 1 %
- *    either way; it is for callers that capture or replay a proof's rounds
- *    without the host.
+ *  "device_transcript" (default 0): 1 runs the Fiat-Shamir steps of the
+ *    zerocheck rounds and of the Ligerito fold rounds on the device (the
+ *    BLAKE3 transcript state moves to device memory and back; each round's
+ *    reduction kernel absorbs the round's message and, in the zerocheck,
+ *    samples its challenge; a Ligerito round's proof of work and challenge
+ *    run in kernels of their own; each BLAKE3 compression spread over a quad
+ *    of lanes) instead of host round trips per round.  The serial hashing
+ *    and the launches cost about what the round trips do: 2^18-compression
+ *    proofs measure within 1 % either way, small ones up to 5 % slower; it
+ *    is for callers that capture or replay a proof's rounds without the
+ *    host.
  *  "zc_graph" (default 1): with the device transcript, the zerocheck rounds'
  *    launches are captured once per size and buffer set in a HIP graph and
  *    replayed by later proofs.

@@ -145,11 +145,12 @@ def test_shape_errors_are_reported(ctx):
         s.close()
 
 
-@pytest.mark.parametrize("log2", [8, 12, 16])
+@pytest.mark.parametrize("log2", [8, 12, 16, 18])
 def test_device_and_host_transcripts_give_the_same_bytes(ctx, log2):
-    """The zerocheck rounds' Fiat-Shamir steps on the device (the first
-    proof of a size captures the rounds in a HIP graph, later ones replay
-    it) and on the host (the default) give identical proofs, for three
+    """The Fiat-Shamir steps of the zerocheck rounds and of the Ligerito
+    fold rounds (their proofs of work included) on the device (the first
+    proof of a size captures the zerocheck rounds in a HIP graph, later ones
+    replay it) and on the host (the default) give identical proofs, for three
     inputs in a row; at 2^8 and 2^12 they also equal the oracle's."""
     _say(f"device vs host transcript log2={log2}")
     s = flockgpu.Blake3Setup(ctx, 1 << log2)
