@@ -11,9 +11,9 @@
 #include <algorithm>
 
 #include "../../include/flock_gpu.h"
+#include "blake3_dev.h"
 #include "ctx.h"
 #include "fs_dev.h"
-#include "blake3_dev.h"
 #include "gf128_dev.h"
 #include "gf128_short.h"
 #include "transcript_dev.h"
