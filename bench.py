@@ -571,7 +571,7 @@ def main():
                     help="also run a plain 1 GiB device copy (counter calibration for profiled runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "bench.py measures one GPU (one process, one device); --gpus must be 1"}))
         return 2
     if args.workload == "commit":
         out = bench_commit(args)
