@@ -26,6 +26,14 @@ int rho_bytetab_device(Ctx *ctx, hipStream_t st, const void *d_ws, const void *d
 // computing it.  A proof of work that finds no nonce in its bounded search
 // sets *d_status to 1 (and carries on with nonce 0).
 constexpr size_t FLOCK_POW_WORK_BYTES = 64;
+// The proof-of-work scan alone, host transcript or device: the smallest
+// nonce >= start for the 32-byte digest (host; null: the one at
+// lig_pow_digest(d_work)) into the first 8 bytes of d_work (atomicMin: ~0 on
+// entry), through an ordered scan of up to 64 windows of 2^max(bits, 10)
+// nonces (2^30 at most); *span (if not null) gets the nonces it covers, after
+// which nothing found leaves ~0.
+int pow_chain_device(Ctx *ctx, hipStream_t st, void *d_work, const uint8_t *digest, uint32_t bits, uint64_t start,
+                     uint64_t *span);
 constexpr uint32_t FLOCK_POW_WAVE_BITS = 6;
 inline void *lig_pow_digest(void *d_work) { return (uint8_t *)d_work + 32; }
 int lig_challenge_device(Ctx *ctx, hipStream_t st, void *d_fs, uint32_t bits, void *d_work, bool digest_ready,

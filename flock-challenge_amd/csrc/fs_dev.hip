@@ -70,24 +70,28 @@ __global__ __launch_bounds__(64) void fs_digest_kernel(FsState *g, uint32_t *dg)
   });
 }
 
-// The smallest nonce n with BLAKE3(dg | n_le | 0^24) starting with `bits`
-// zero bits (challenger.rs:1246-1300), into *best (atomicMin; it holds ~0 on
-// entry).  Windows of `window` nonces are scanned in order, each split
+// The smallest nonce n >= start with BLAKE3(dg | n_le | 0^24) starting with
+// `bits` zero bits (challenger.rs:1246-1300), into *best (atomicMin; it holds
+// ~0 on entry).  Windows of `window` nonces are scanned in order, each split
 // evenly over the workgroups; a workgroup stops after a window once *best
 // lies in it or an earlier one, so every nonce below the final *best has
 // been tried by the workgroup that owns it, and every workgroup stops after
 // max_windows windows at the latest.
-constexpr uint32_t POW_THREADS = 256;
-__global__ __launch_bounds__(POW_THREADS) void pow_chain_kernel(const uint32_t *__restrict__ dg, uint32_t bits,
-                                                                uint64_t window, uint32_t max_windows,
-                                                                unsigned long long *best) {
+constexpr uint32_t POW_THREADS = 256, POW_WINDOWS = 64;
+// The digest comes from dg (device memory) or, with dg null, as the kernel
+// arguments d0, d1.
+__global__ __launch_bounds__(POW_THREADS) void pow_chain_kernel(const uint32_t *__restrict__ dg, uint4 d0, uint4 d1,
+                                                                uint32_t bits, uint64_t start, uint64_t window,
+                                                                uint32_t max_windows, unsigned long long *best) {
   __shared__ unsigned long long seen;
-  uint32_t d[8];
+  uint32_t d[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+  if (dg) {
 #pragma unroll
-  for (int i = 0; i < 8; ++i) d[i] = dg[i];
+    for (int i = 0; i < 8; ++i) d[i] = dg[i];
+  }
   const uint64_t per_wg = window / gridDim.x;
   for (uint32_t w = 0; w < max_windows; ++w) {
-    const uint64_t base = (uint64_t)w * window + (uint64_t)blockIdx.x * per_wg;
+    const uint64_t base = start + (uint64_t)w * window + (uint64_t)blockIdx.x * per_wg;
     for (uint64_t i = threadIdx.x; i < per_wg; i += POW_THREADS) {
       const uint64_t nonce = base + i;
       if (b3::pow_ok(d, nonce, bits)) atomicMin(best, (unsigned long long)nonce);
@@ -95,7 +99,7 @@ __global__ __launch_bounds__(POW_THREADS) void pow_chain_kernel(const uint32_t *
     __syncthreads();
     if (threadIdx.x == 0) seen = __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    if (seen < (uint64_t)(w + 1) * window) break;  // workgroup-uniform
+    if (seen < start + (uint64_t)(w + 1) * window) break;  // workgroup-uniform
     __syncthreads();  // seen is read before thread 0 rewrites it
   }
 }
@@ -160,15 +164,36 @@ int lig_challenge_device(Ctx *ctx, hipStream_t st, void *d_fs, uint32_t bits, vo
   uint32_t *dg = (uint32_t *)lig_pow_digest(d_work);
   if (bits > POW_WAVE_BITS) {
     if (!digest_ready) hipLaunchKernelGGL(fs_digest_kernel, dim3(1), dim3(64), 0, st, (FsState *)d_fs, dg);
-    // windows of 4 expected solutions (at least 2^16 nonces), at most 64 of
-    // them: a search that finds nothing has probability below e^-256
-    const uint64_t window = 1ull << std::max<uint32_t>(bits + 2, 16);
-    const uint64_t wgs = std::min<uint64_t>(window / POW_THREADS, 1024);
-    hipLaunchKernelGGL(pow_chain_kernel, dim3((unsigned)wgs), dim3(POW_THREADS), 0, st, dg, bits, window, 64u, best);
+    // one search from nonce 0: with bits <= 30 it finds nothing with
+    // probability below e^-64 (the status word then says so)
+    if (int rc = pow_chain_device(ctx, st, d_work, nullptr, bits, 0, nullptr)) return rc;
   }
   hipLaunchKernelGGL(lig_nonce_sample_kernel, dim3(1), dim3(64), 0, st, (FsState *)d_fs, best, bits,
                      (uint64_t *)d_nonce, (f128 *)d_r, (uint32_t *)d_status);
   FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  return FLOCK_OK;
+}
+
+int pow_chain_device(Ctx *ctx, hipStream_t st, void *d_work, const uint8_t *digest, uint32_t bits, uint64_t start,
+                     uint64_t *span) {
+  if (bits == 0 || bits > 40) {
+    ctx->last_error = "proof of work: 1 to 40 bits";
+    return FLOCK_E_ARG;
+  }
+  // windows of one expected solution (at least 2^10 nonces, at most 2^30):
+  // about 1.6 windows are scanned on average
+  const uint64_t window = 1ull << std::min<uint32_t>(std::max<uint32_t>(bits, 10), 30);
+  const uint64_t wgs = std::min<uint64_t>(window / POW_THREADS, 1024);
+  uint4 d0 = make_uint4(0u, 0u, 0u, 0u), d1 = d0;
+  if (digest) {
+    memcpy(&d0, digest, 16);
+    memcpy(&d1, digest + 16, 16);
+  }
+  hipLaunchKernelGGL(pow_chain_kernel, dim3((unsigned)wgs), dim3(POW_THREADS), 0, st,
+                     digest ? nullptr : (const uint32_t *)lig_pow_digest(d_work), d0, d1, bits, start, window,
+                     POW_WINDOWS, (unsigned long long *)d_work);
+  FLOCK_HIP_CHECK(ctx, hipGetLastError());
+  if (span) *span = window * POW_WINDOWS;
   return FLOCK_OK;
 }
 

@@ -120,10 +120,8 @@ int rs_basis_device(Ctx *ctx, hipStream_t st, const void *tables, const void *lo
 // Whole forward (layers l0..k-1) or transposed (k-1..l0) NTT over 2^k
 // positions x 2^log_nl lanes, up to 4 layers per launch.
 int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, int l0, int log_nl, int transposed);
-int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
-                    void *d_best);
-// out (device-visible) = *best, then *best = all-ones for the next window;
-// then publishes a nonzero seq as xor_rows_device does.
+// out (device-visible) = *best, then *best = all-ones for the next search
+// (pow_chain_device); then publishes a nonzero seq as xor_rows_device does.
 int pow_take_device(Ctx *ctx, hipStream_t st, void *d_best, void *out, uint32_t seq = 0);
 int gather_device(Ctx *ctx, hipStream_t st, const void *src, const uint64_t *d_idx, uint32_t n, uint32_t unit_bytes,
                   void *out);

@@ -338,20 +338,21 @@ struct Prover {
       }
     } else if (bits) {
       ScopedPhase ph(ctx, "pow", st);
-      uint8_t dg[32];
-      t.state_digest(dg);
-      unsigned long long *best = (unsigned long long *)dev("pow.best", 8);
-      if (!best) return FLOCK_E_OOM;
-      if (!pow_armed) {  // every window's take re-arms it afterwards
+      // the best nonce (first 8 bytes; the digest goes as kernel arguments)
+      uint8_t *work = (uint8_t *)dev("pow.work", FLOCK_POW_WORK_BYTES);
+      if (!work) return FLOCK_E_OOM;
+      if (!pow_armed) {  // every search's take re-arms it afterwards
         unsigned long long none = ~0ull;
-        if (int rc = h2d(best, &none, 8)) return rc;
+        if (int rc = h2d(work, &none, 8)) return rc;
         pow_armed = true;
       }
-      uint64_t start = 0, window = 1ull << std::min<uint32_t>(bits + 2, 26);
-      for (;;) {
+      uint8_t dg[32];
+      t.state_digest(dg);
+      for (uint64_t start = 0;;) {
         const uint32_t seq = ctx->next_readback_seq();
-        int rc = pow_scan_device(ctx, st, dg, start, window, bits, best);
-        if (!rc) rc = pow_take_device(ctx, st, best, ctx->readback_dev, seq);
+        uint64_t span = 0;
+        int rc = pow_chain_device(ctx, st, work, dg, bits, start, &span);
+        if (!rc) rc = pow_take_device(ctx, st, work, ctx->readback_dev, seq);
         unsigned long long got = 0;
         if (!rc) rc = fetch(&got, 8, seq);
         if (rc) return rc;
@@ -359,7 +360,7 @@ struct Prover {
           nonce = got;
           break;
         }
-        start += window;
+        start += span;
       }
     }
     t.observe_bytes(&nonce, 8);

@@ -1034,30 +1034,8 @@ int ntt_narrow_device(Ctx *ctx, hipStream_t st, void *d, const void *tw, int k, 
 }
 
 // ---------------------------------------------------------------- proof of work
-// The nonces [start, start + count) through b3::pow_ok; the smallest
-// satisfying one of the window wins (atomicMin).
-__global__ __launch_bounds__(256) void pow_kernel(uint4 d0, uint4 d1, uint64_t start, uint64_t count, uint32_t bits,
-                                                  unsigned long long *__restrict__ best) {
-  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= count) return;
-  const uint64_t nonce = start + t;
-  const uint32_t dg[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-  if (b3::pow_ok(dg, nonce, bits)) atomicMin(best, (unsigned long long)nonce);
-}
-
-int pow_scan_device(Ctx *ctx, hipStream_t st, const uint8_t digest[32], uint64_t start, uint64_t count, uint32_t bits,
-                    void *d_best) {
-  uint4 d0, d1;
-  memcpy(&d0, digest, 16);
-  memcpy(&d1, digest + 16, 16);
-  hipLaunchKernelGGL(pow_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, d0, d1, start, count, bits,
-                     (unsigned long long *)d_best);
-  FLOCK_HIP_CHECK(ctx, hipGetLastError());
-  return FLOCK_OK;
-}
-
-// Hand the window's result to the host and re-arm the scan: out = best,
-// best = none (so the next window needs no upload).
+// Hand a proof-of-work search's result to the host and re-arm it: out =
+// best, best = none (so the next search needs no upload for it).
 __global__ void pow_take_kernel(unsigned long long *__restrict__ best, unsigned long long *__restrict__ out,
                                 uint32_t *flag, uint32_t seq) {
   if (threadIdx.x == 0) {
