@@ -51,7 +51,8 @@ struct MerkleJob {
 // The kernels that can run an NTT pass of F layers: shared-table passes
 // (F = 3 or 4; W wavefronts per workgroup with G groups each, 2^s >= W G
 // groups sharing the pass's twiddles; full byte tables rebuilt per twiddle,
-// or all the pass's nibble tables built once), and per-wavefront tail passes
+// or all the pass's nibble tables built once; or one table buffer, so that
+// two workgroups share a CU), and per-wavefront tail passes
 // (F = 1..4, NB nibble tables in flight, or no tables: products by shifted
 // XORs, for the short twiddles of the deepest layers).
 enum NttKernel : uint8_t {
@@ -63,7 +64,9 @@ enum NttKernel : uint8_t {
   NTT_NIBSHARED_W16 = 5,
   NTT_NIBSHARED_W8 = 6,
   NTT_TAIL_SHIFT = 7,
-  NTT_KERNELS = 8
+  NTT_SHARED1_W8G1 = 8,
+  NTT_SHARED1_W4G2 = 9,
+  NTT_KERNELS = 10
 };
 // One pass: layers [l, l + F) with kernel.
 struct NttPass {

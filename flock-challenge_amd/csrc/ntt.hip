@@ -195,6 +195,50 @@ __global__ __launch_bounds__(64) void ntt_tail_batch_kernel(const NttJob *__rest
 // previous barrier.
 constexpr int SHARED_TWIDDLES = 15;  // 2^F - 1 for F <= 4
 
+// The butterflies of twiddle n = 2^J - 1 + C of a pass of F layers in each
+// of the wavefront's G groups, the products through full byte tables at
+// tbase (+ 64 KiB when HI), two in flight per lane when PAIR.
+template <int F, int G, bool PAIR, bool HI, int J, int C>
+__device__ __forceinline__ void fulltab_butterflies(f128 (&v)[G][1 << F], const char *tbase, const FulltabLane &FL) {
+  constexpr int nb = 1 << (F - 1 - J);
+  constexpr int SLOTS = G * nb;
+  constexpr int STEP = PAIR ? 2 : 1;
+  // product slots (group, butterfly) in pairs
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl += STEP) {
+    const int ga = sl / nb, la = sl % nb, gb = (sl + 1) / nb, lb = (sl + 1) % nb;
+    const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
+    if (STEP == 2 && sl + 1 < SLOTS) {
+      const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
+      f128 p0, p1;
+      f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
+      const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
+      v[ga][a0] = u0;
+      f128_xor_into(v[ga][a1], u0);
+      v[gb][b0] = u1;
+      f128_xor_into(v[gb][b1], u1);
+    } else {
+      const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
+      v[ga][a0] = u0;
+      f128_xor_into(v[ga][a1], u0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The same butterflies for a zero twiddle (block 0 of a layer): u0 = v0,
+// u1 = v1 + v0.
+template <int F, int G, int J, int C>
+__device__ __forceinline__ void zero_butterflies(f128 (&v)[G][1 << F]) {
+  constexpr int nb = 1 << (F - 1 - J);
+#pragma unroll
+  for (int sl = 0; sl < G * nb; ++sl) {
+    const int ga = sl / nb, la = sl % nb;
+    const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
+    f128_xor_into(v[ga][a1], v[ga][a0]);
+  }
+}
+
 // Twiddle n = 2^J - 1 + C of a shared-table pass (layer offset J, block C):
 // build its tables into buffer n & 1, then every butterfly of it in each of
 // the wave's G groups, two products in flight per lane (one with 16
@@ -205,44 +249,15 @@ __device__ __forceinline__ void shared_pass_step(f128 (&v)[G][1 << F], f128 *tab
   constexpr int N = (1 << J) - 1 + C;
   constexpr bool HI = N & 1;
   const char *tbase = (const char *)tabs;  // buffer 0 at +0, buffer 1 at +64 KiB
-  constexpr int nb = 1 << (F - 1 - J);
-  constexpr int SLOTS = G * nb;
   if ((zmask >> N) & 1u) {
-    // a zero twiddle (block 0 of a layer): u0 = v0, u1 = v1 + v0.  The
-    // barrier stays, so buffer N & 1 is not rebuilt (for twiddle N + 2)
+    // The barrier stays, so buffer N & 1 is not rebuilt (for twiddle N + 2)
     // while a wave still reads it for twiddle N - 1 ... N + 1.
     __syncthreads();
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int ga = sl / nb, la = sl % nb;
-      const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
-      f128_xor_into(v[ga][a1], v[ga][a0]);
-    }
+    zero_butterflies<F, G, J, C>(v);
   } else {
     build_fulltab<THREADS>(basis + N * 128, tabs + (HI ? 4096 : 0), tid);
     __syncthreads();
-    // product slots (group, butterfly) in pairs
-    constexpr int STEP = THREADS > 512 ? 1 : 2;
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; sl += STEP) {
-      const int ga = sl / nb, la = sl % nb, gb = (sl + 1) / nb, lb = (sl + 1) % nb;
-      const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
-      if (STEP == 2 && sl + 1 < SLOTS) {
-        const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
-        f128 p0, p1;
-        f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
-        const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
-        v[ga][a0] = u0;
-        f128_xor_into(v[ga][a1], u0);
-        v[gb][b0] = u1;
-        f128_xor_into(v[gb][b1], u1);
-      } else {
-        const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
-        v[ga][a0] = u0;
-        f128_xor_into(v[ga][a1], u0);
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    fulltab_butterflies<F, G, (THREADS <= 512), HI, J, C>(v, tbase, FL);
   }
   if constexpr (C + 1 < (1 << J))
     shared_pass_step<F, G, THREADS, J, C + 1>(v, tabs, basis, FL, tid, zmask);
@@ -320,6 +335,97 @@ __global__ __launch_bounds__(64 * WAVES) void ntt_pass_shared_batch_kernel(const
   const NttJob &J = job_of(jobs, n_jobs);
   ntt_pass_shared_body<F, WAVES, G>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data, J.src ? J.src_pos_mask : ~0ull,
                                     blockIdx.x - J.wg_begin, lds);
+}
+
+// ---------------------------------------------------------------- single-table shared passes
+// The shared-table pass with one 64 KiB table buffer and the basis of the
+// current and the next twiddle only (68 KiB of LDS), so two workgroups share
+// a CU: one's loads, table builds and barriers overlap the other's products.
+// Two barriers per twiddle: the table is rebuilt once every wavefront is
+// done with the previous one, and read once it is built.  PAIR: two
+// products in flight per lane.
+constexpr int SHARED1_LDS_F128 = 4096 + 2 * 128 + 16;
+
+// Step of twiddle n = 2^J - 1 + C: its basis is in basis buffer n & 1 (made
+// visible by the first barrier), twiddle n + 1's goes into the other one
+// while the table is built; that buffer was last read by the previous
+// step's build, before the previous step's second barrier.
+template <int F, int G, int THREADS, bool PAIR, int J, int C>
+__device__ __forceinline__ void shared1_pass_step(f128 (&v)[G][1 << F], f128 *tab, f128 *basis, const f128 *twv,
+                                                  const FulltabLane &FL, uint32_t tid, uint32_t zmask) {
+  constexpr int N = (1 << J) - 1 + C, NT = (1 << F) - 1;
+  const bool zero = (zmask >> N) & 1u;
+  __syncthreads();  // every wavefront done with table N - 1; basis N visible
+  if (!zero) build_fulltab<THREADS>(basis + (N & 1) * 128, tab, tid);
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (N + 1 < NT) fulltab_basis(twv[N + 1], basis + ((N + 1) & 1) * 128, tid);
+  __builtin_amdgcn_sched_barrier(0);
+  __syncthreads();  // table N and basis N + 1 visible
+  if (zero)
+    zero_butterflies<F, G, J, C>(v);
+  else
+    fulltab_butterflies<F, G, PAIR, false, J, C>(v, (const char *)tab, FL);
+  if constexpr (C + 1 < (1 << J))
+    shared1_pass_step<F, G, THREADS, PAIR, J, C + 1>(v, tab, basis, twv, FL, tid, zmask);
+  else if constexpr (J + 1 < F)
+    shared1_pass_step<F, G, THREADS, PAIR, J + 1, 0>(v, tab, basis, twv, FL, tid, zmask);
+}
+
+template <int F, int WAVES, int G, bool PAIR>
+__device__ __forceinline__ void ntt_pass_shared1_body(f128 *data, const f128 *__restrict__ tw, int k, int l,
+                                                      const f128 *src, uint64_t src_mask, uint64_t wg, f128 *lds) {
+  constexpr uint32_t THREADS = 64 * WAVES;
+  constexpr uint32_t NT = (1u << F) - 1u;
+  f128 *tab = lds;                   // 16 x 256 entries
+  f128 *basis = lds + 4096;          // 2 x 128
+  f128 *twv = lds + 4096 + 2 * 128;  // the pass's twiddles, n = 2^j - 1 + c
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const FulltabLane FL(lane & 15u);
+  const int s = k - l - F;
+  const uint64_t g0 = (wg * WAVES + __builtin_amdgcn_readfirstlane(tid >> 6)) * G;
+  const uint64_t B = g0 >> s;  // the same for the whole workgroup
+  if (tid < NT) {
+    const uint32_t j = 31 - __builtin_clz(tid + 1), c = tid + 1 - (1u << j);
+    twv[tid] = tw[(1ull << (l + j)) - 1ull + (B << j) + c];
+  }
+  fulltab_basis(tw[(1ull << l) - 1ull + B], basis, tid);  // twiddle 0
+  __builtin_amdgcn_sched_barrier(0);  // its registers free before the loads
+  f128 v[G][1 << F];
+  uint64_t base[G];
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi) {
+    const uint64_t g = g0 + gi;
+    base[gi] = (B << (k - l)) | (g & ((1ull << s) - 1ull));
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e) v[gi][e] = src[(((base[gi] | ((uint64_t)e << s)) & src_mask) << 6) + lane];
+  }
+  const uint32_t zmask = B == 0 ? pass_zero_twiddles(tw, l, F, B) : 0u;
+  shared1_pass_step<F, G, THREADS, PAIR, 0, 0>(v, tab, basis, twv, FL, tid, zmask);
+#pragma unroll
+  for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+    for (int e = 0; e < (1 << F); ++e) data[((base[gi] | ((uint64_t)e << s)) << 6) + lane] = v[gi][e];
+}
+
+// Two workgroups per CU (WAVES / 2 wavefronts per SIMD): at 8 wavefronts
+// each a lane may hold 128 VGPRs, at 4 wavefronts 256.
+template <int F, int WAVES, int G, bool PAIR>
+__global__ __launch_bounds__(64 * WAVES, WAVES / 2) void ntt_pass_shared1_kernel(f128 *data,
+                                                                                  const f128 *__restrict__ tw, int k,
+                                                                                  int l, const f128 *src,
+                                                                                  uint64_t src_mask) {
+  __shared__ f128 lds[SHARED1_LDS_F128];
+  ntt_pass_shared1_body<F, WAVES, G, PAIR>(data, tw, k, l, src, src_mask,
+                                           pass_wg(blockIdx.x, gridDim.x, l, src_mask), lds);
+}
+
+template <int F, int WAVES, int G, bool PAIR>
+__global__ __launch_bounds__(64 * WAVES, WAVES / 2) void ntt_pass_shared1_batch_kernel(
+    const NttJob *__restrict__ jobs, uint32_t n_jobs) {
+  __shared__ f128 lds[SHARED1_LDS_F128];
+  const NttJob &J = job_of(jobs, n_jobs);
+  ntt_pass_shared1_body<F, WAVES, G, PAIR>(J.data, J.tw, J.k, J.l, J.src ? J.src : J.data,
+                                           J.src ? J.src_pos_mask : ~0ull, blockIdx.x - J.wg_begin, lds);
 }
 
 // ---------------------------------------------------------------- nibble-table shared passes
@@ -422,6 +528,15 @@ void launch_shared(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *j
                        a.mask);
 }
 
+template <int F, int W, int G, bool PAIR>
+void launch_shared1(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *jobs, uint32_t n_jobs) {
+  if (jobs)
+    hipLaunchKernelGGL((ntt_pass_shared1_batch_kernel<F, W, G, PAIR>), grid, dim3(64 * W), 0, st, jobs, n_jobs);
+  else
+    hipLaunchKernelGGL((ntt_pass_shared1_kernel<F, W, G, PAIR>), grid, dim3(64 * W), 0, st, a.data, a.tw, a.k, a.l,
+                       a.src, a.mask);
+}
+
 template <int F, int W>
 void launch_nibshared(dim3 grid, hipStream_t st, const PassArgs &a, const NttJob *jobs, uint32_t n_jobs) {
   if (jobs)
@@ -478,7 +593,14 @@ int launch_pass(Ctx *ctx, hipStream_t st, int F, NttKernel kn, uint64_t wgs, con
     FLOCK_NIBSHARED(NTT_NIBSHARED_W16, 4, 16)
     FLOCK_NIBSHARED(NTT_NIBSHARED_W8, 3, 8)
     FLOCK_NIBSHARED(NTT_NIBSHARED_W8, 4, 8)
+#define FLOCK_SHARED1(KN, F_, W, G, PAIR) \
+  case (int)KN * 8 + F_: launch_shared1<F_, W, G, PAIR>(grid, st, a, jobs, n_jobs); break;
+    FLOCK_SHARED1(NTT_SHARED1_W8G1, 3, 8, 1, false)
+    FLOCK_SHARED1(NTT_SHARED1_W8G1, 4, 8, 1, false)
+    FLOCK_SHARED1(NTT_SHARED1_W4G2, 3, 4, 2, true)
+    FLOCK_SHARED1(NTT_SHARED1_W4G2, 4, 4, 2, true)
 #undef FLOCK_SHARED
+#undef FLOCK_SHARED1
 #undef FLOCK_TAIL
 #undef FLOCK_NIBSHARED
     default: ctx->last_error = "ntt: no kernel for this pass"; return FLOCK_E_ARG;
@@ -497,6 +619,8 @@ int ntt_kernel_groups_log(NttKernel kn) {
     case NTT_SHARED_W16G1: return 4;
     case NTT_NIBSHARED_W16: return 4;
     case NTT_NIBSHARED_W8: return 3;
+    case NTT_SHARED1_W8G1: return 3;
+    case NTT_SHARED1_W4G2: return 3;
     default: return 0;
   }
 }

@@ -165,9 +165,10 @@ def device_count():
 # Commit NTT plans (process-wide; no GPU needed).  Kernel ids: shared
 # full-table passes 8x2, 8x1, 16x1 (wavefronts x groups), tail passes with 2
 # or 1 nibble tables in flight, shared nibble-table passes of 16 or 8
-# wavefronts.
+# wavefronts, tail passes by shifted XORs, and one-table shared passes 8x1
+# and 4x2 (two workgroups per CU).
 (NTT_SHARED_W8G2, NTT_SHARED_W8G1, NTT_SHARED_W16G1, NTT_TAIL_NB2, NTT_TAIL_NB1, NTT_NIBSHARED_W16,
- NTT_NIBSHARED_W8) = range(7)
+ NTT_NIBSHARED_W8, NTT_TAIL_SHIFT, NTT_SHARED1_W8G1, NTT_SHARED1_W4G2) = range(10)
 
 
 def ntt_plan_set(k, start, passes, ctx=None):

@@ -58,7 +58,8 @@ def main():
     finally:
         ctx.close()
     doc = {"arch": "gfx950", "kernels": ["shared 8x2", "shared 8x1", "shared 16x1", "tail nb2", "tail nb1",
-                                         "nibble-shared 16", "nibble-shared 8", "tail shift"],
+                                         "nibble-shared 16", "nibble-shared 8", "tail shift",
+                                         "one-table 8x1", "one-table 4x2"],
            "reps": a.reps, "plans": plans}
     with open(a.out, "w") as f:
         json.dump(doc, f, indent=1)

@@ -134,7 +134,10 @@ int flock_commit_device_noalloc(flock_ctx *ctx, const flock_pcs_params *params, 
  *   5, 6:    shared nibble-table passes (F = 3 or 4; 16 or 8 wavefronts,
  *            needing 2^(k - l - F) >= 16 or 8);
  *   7:       tail passes without tables (F = 1..4; products by shifted
- *            XORs, for the short twiddles of the deepest layers).
+ *            XORs, for the short twiddles of the deepest layers);
+ *   8, 9:    shared full-table passes with one table buffer (F = 3 or 4;
+ *            8 x 1 and 4 x 2 wavefronts x groups), so that two
+ *            workgroups share a CU.
  * Every valid plan gives the same codeword; only the time differs.  Plans
  * live in a process-wide table, loaded at first use from $FLOCK_NTT_PLANS or
  * else tune/ntt_plans_gfx950.json in the directory above the library's

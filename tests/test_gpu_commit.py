@@ -59,7 +59,7 @@ def test_commit_rejects_bad_shapes(ctx):
 
 # Plans for the transform of m = 24, rate 1/2 (k = 12 from layer 1) that
 # exercise every kernel and layer count: (l, F, kernel) passes.
-S82, S81, S161, T2, T1, N16, N8, SH = range(8)
+S82, S81, S161, T2, T1, N16, N8, SH, O81, O42 = range(10)
 PLANS_K12 = {
     "default": None,
     "16-wave shared, one-table tail": [(1, 3, S161), (4, 4, S161), (8, 4, T1)],
@@ -72,6 +72,10 @@ PLANS_K12 = {
     # and on long ones (first pass from the message, zero block-0 twiddles)
     "shift tails, deepest layers": [(1, 3, N8), (4, 4, S82), (8, 4, SH)],
     "shift tails everywhere": [(1, 4, SH), (5, 3, SH), (8, 2, SH), (10, 1, SH), (11, 1, T1)],
+    # one-table passes (two workgroups per CU), from the message and on the
+    # codeword, zero block-0 twiddles included
+    "one-table 8x1 and 4x2": [(1, 4, O81), (5, 4, O42), (9, 3, T1)],
+    "one-table, three layers": [(1, 3, O42), (4, 3, O81), (7, 2, T2), (9, 3, SH)],
 }
 
 

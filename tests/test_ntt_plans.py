@@ -8,12 +8,12 @@ import pytest
 import gpu_util
 
 flockgpu = gpu_util.flockgpu
-S82, S81, S161, T2, T1, N16, N8, SH = range(8)
+S82, S81, S161, T2, T1, N16, N8, SH, O81, O42 = range(10)
 PLAN_FILE = os.path.join(os.path.dirname(flockgpu.__file__), "tune", "ntt_plans_gfx950.json")
 
 
 def groups_log(kernel):
-    return {S82: 4, S81: 3, S161: 4, N16: 4, N8: 3}.get(kernel, 0)
+    return {S82: 4, S81: 3, S161: 4, N16: 4, N8: 3, O81: 3, O42: 3}.get(kernel, 0)
 
 
 def feasible(k, l, F, kernel):
@@ -22,7 +22,7 @@ def feasible(k, l, F, kernel):
     or shifted XORs) any F in 1..4."""
     if F < 1 or F > 4 or l + F > k:
         return False
-    if kernel in (S82, S81, S161, N16, N8):
+    if kernel in (S82, S81, S161, N16, N8, O81, O42):
         return F >= 3 and k - l - F >= groups_log(kernel)
     return kernel in (T2, T1, SH)
 
