@@ -22,6 +22,22 @@ __device__ __forceinline__ void f128_xor_into(f128 &a, f128 b) {
   a.x ^= b.x; a.y ^= b.y; a.z ^= b.z; a.w ^= b.w;
 }
 
+// Streaming accesses (the nt cache policy): for tables read or written once
+// per sweep, far larger than the L2.
+typedef uint32_t f128_v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f128 f128_load_nt(const f128 *p) {
+  const f128_v4 v = __builtin_nontemporal_load((const f128_v4 *)p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void f128_store_nt(f128 *p, f128 a) {
+  f128_v4 v;
+  v.x = a.x;
+  v.y = a.y;
+  v.z = a.z;
+  v.w = a.w;
+  __builtin_nontemporal_store(v, (f128_v4 *)p);
+}
+
 // a ^ b ^ c in one v_bitop3_b32 per dword (truth table 0x96).
 __device__ __forceinline__ uint32_t xor3_u32(uint32_t a, uint32_t b, uint32_t c) {
   return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);

@@ -346,11 +346,11 @@ __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__res
     for (int u = 0; u < FOLD_UNROLL; ++u) {
       const uint64_t x = x0 + u * stride;
       if (x < half) {
-        a0[u] = A[2 * x];
-        a1[u] = A[2 * x + 1];
+        a0[u] = f128_load_nt(A + 2 * x);
+        a1[u] = f128_load_nt(A + 2 * x + 1);
         if (B) {
-          b0[u] = B[2 * x];
-          b1[u] = B[2 * x + 1];
+          b0[u] = f128_load_nt(B + 2 * x);
+          b1[u] = f128_load_nt(B + 2 * x + 1);
         }
       }
     }
