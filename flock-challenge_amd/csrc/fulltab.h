@@ -59,13 +59,15 @@ __device__ __forceinline__ uint32_t f128_word(const f128 &x, int q) {
   return q == 0 ? x.x : q == 1 ? x.y : q == 2 ? x.z : x.w;
 }
 
-// v * t from the interleaved tables of t at base (+ 64 KiB when HI).  The
-// four steps are unrolled with scheduling barriers between them, so one
-// step's four lookups are in flight at a time and nothing is hoisted.
+// s + v * t from the interleaved tables of t at base (+ 64 KiB when HI).
+// The four steps are unrolled with scheduling barriers between them, so one
+// step's four lookups are in flight at a time and nothing is hoisted.  The
+// sum starts at s (a butterfly's XOR costs nothing).
 template <bool HI = false>
-__device__ __forceinline__ f128 f128_mul_fulltab(f128 v, const char *__restrict__ base, const FulltabLane &L) {
+__device__ __forceinline__ f128 f128_mul_fulltab_add(f128 v, const char *__restrict__ base, const FulltabLane &L,
+                                                     f128 s) {
   const f128 x = L.xperm(v);
-  f128 acc = f128_zero();
+  f128 acc = s;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const uint32_t w = f128_word(x, q);
@@ -81,14 +83,19 @@ __device__ __forceinline__ f128 f128_mul_fulltab(f128 v, const char *__restrict_
   return acc;
 }
 
-// Two independent products, interleaved for latency: v0 by the table at
-// base (+ 64 KiB when HI), v1 by the one HI1 selects (the same by default).
+// v * t.
+template <bool HI = false>
+__device__ __forceinline__ f128 f128_mul_fulltab(f128 v, const char *__restrict__ base, const FulltabLane &L) {
+  return f128_mul_fulltab_add<HI>(v, base, L, f128_zero());
+}
+
+// Two independent products added into p0 and p1, interleaved for latency:
+// v0 by the table at base (+ 64 KiB when HI), v1 by the one HI1 selects (the
+// same by default).
 template <bool HI = false, bool HI1 = HI>
-__device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *__restrict__ base,
-                                                  const FulltabLane &L, f128 &p0, f128 &p1) {
+__device__ __forceinline__ void f128_mul2_fulltab_add(f128 v0, f128 v1, const char *__restrict__ base,
+                                                      const FulltabLane &L, f128 &p0, f128 &p1) {
   const f128 x0 = L.xperm(v0), x1 = L.xperm(v1);
-  p0 = f128_zero();
-  p1 = f128_zero();
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const uint32_t w0 = f128_word(x0, q), w1 = f128_word(x1, q);
@@ -106,6 +113,15 @@ __device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *
     p1 = f128_xor3(p1, t1[2], t1[3]);
     __builtin_amdgcn_sched_barrier(0);
   }
+}
+
+// p0 = v0 * t, p1 = v1 * t' (as f128_mul2_fulltab_add from zero).
+template <bool HI = false, bool HI1 = HI>
+__device__ __forceinline__ void f128_mul2_fulltab(f128 v0, f128 v1, const char *__restrict__ base,
+                                                  const FulltabLane &L, f128 &p0, f128 &p1) {
+  p0 = f128_zero();
+  p1 = f128_zero();
+  f128_mul2_fulltab_add<HI, HI1>(v0, v1, base, L, p0, p1);
 }
 
 // t * x^i mod P for 0 <= i < 128 (i may differ per lane): the dword shift

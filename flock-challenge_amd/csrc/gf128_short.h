@@ -90,13 +90,14 @@ FLOCK_SHORT_HD void add_word(uint32_t (&acc)[8], const uint32_t (&v)[4], uint32_
   add_shifts<OFF>(acc, v, t & ~1u);
 }
 
-// out = v * t mod P for t = t0 + 2^32 t1 + 2^64 t2 + 2^96 t3 (any t; the
-// cost is proportional to its set bits, and a zero word costs a scalar
-// test).  v and out are the four little-endian words of an F128 (bits 0..31
-// first).
-FLOCK_SHORT_HD void mul(const uint32_t (&v)[4], uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3,
-                        uint32_t (&out)[4]) {
-  uint32_t acc[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+// out = s + v * t mod P for t = t0 + 2^32 t1 + 2^64 t2 + 2^96 t3 (any t;
+// the cost is proportional to its set bits, and a zero word costs a scalar
+// test).  v, s and out are the four little-endian words of an F128 (bits
+// 0..31 first); s enters the low half of the unreduced sum, which the
+// reduction leaves as it is.
+FLOCK_SHORT_HD void mul_add(const uint32_t (&v)[4], uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3,
+                            const uint32_t (&s)[4], uint32_t (&out)[4]) {
+  uint32_t acc[8] = {s[0], s[1], s[2], s[3], 0u, 0u, 0u, 0u};
   if (t0) add_word<0>(acc, v, t0);
   if (t1) add_word<1>(acc, v, t1);
   if (t2) add_word<2>(acc, v, t2);
@@ -113,6 +114,13 @@ FLOCK_SHORT_HD void mul(const uint32_t (&v)[4], uint32_t t0, uint32_t t1, uint32
   out[1] = acc[1] ^ r1;
   out[2] = acc[2] ^ r2;
   out[3] = acc[3] ^ r3;
+}
+
+// out = v * t mod P.
+FLOCK_SHORT_HD void mul(const uint32_t (&v)[4], uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3,
+                        uint32_t (&out)[4]) {
+  const uint32_t zero[4] = {0u, 0u, 0u, 0u};
+  mul_add(v, t0, t1, t2, t3, zero, out);
 }
 
 }  // namespace shortmul

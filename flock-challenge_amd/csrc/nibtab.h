@@ -50,9 +50,11 @@ __device__ __forceinline__ constexpr uint32_t nibtab_sel(int i, int j) {
   return (4u + (uint32_t)i) | ((uint32_t)j << 8) | (12u << 16) | (12u << 24);
 }
 
-// v * t from the nibble tables of t at byte offset TAB of lds.
+// s + v * t from the nibble tables of t at byte offset TAB of lds (the sum
+// starts at s: a butterfly's XOR costs nothing).
 template <uint32_t TAB>
-__device__ __forceinline__ f128 f128_mul_nibtab(f128 v, const char *__restrict__ lds, const NibtabLane &L) {
+__device__ __forceinline__ f128 f128_mul_nibtab_add(f128 v, const char *__restrict__ lds, const NibtabLane &L,
+                                                    f128 s) {
   // dword d of w = dword d ^ (rot bit 3) of v
   const uint32_t w0 = __builtin_amdgcn_perm(v.y, v.x, L.dsel), w1 = __builtin_amdgcn_perm(v.x, v.y, L.dsel);
   const uint32_t w2 = __builtin_amdgcn_perm(v.w, v.z, L.dsel), w3 = __builtin_amdgcn_perm(v.z, v.w, L.dsel);
@@ -65,7 +67,7 @@ __device__ __forceinline__ f128 f128_mul_nibtab(f128 v, const char *__restrict__
     E[d] = __builtin_amdgcn_perm(od, ev, L.sel_e);
     O[d] = __builtin_amdgcn_perm(od, ev, L.sel_o);
   }
-  f128 acc = f128_zero();
+  f128 acc = s;
 #pragma unroll
   for (int g = 0; g < 8; ++g) {
     f128 t[4];
@@ -81,6 +83,12 @@ __device__ __forceinline__ f128 f128_mul_nibtab(f128 v, const char *__restrict__
     __builtin_amdgcn_sched_barrier(0);
   }
   return acc;
+}
+
+// v * t from the nibble tables of t at byte offset TAB of lds.
+template <uint32_t TAB>
+__device__ __forceinline__ f128 f128_mul_nibtab(f128 v, const char *__restrict__ lds, const NibtabLane &L) {
+  return f128_mul_nibtab_add<TAB>(v, lds, L, f128_zero());
 }
 
 // The 512 entries of t's nibble tables at byte offset TAB, built by one

@@ -82,13 +82,14 @@ __device__ __forceinline__ uint64_t pass_wg(uint64_t w, uint64_t n_wgs, int l, u
 // Step (J, C) handles twiddles C .. C + NT - 1 of layer offset J; template
 // recursion keeps every register index constant.
 
-// Twiddle n of a group's pass, in scalar registers, from the vector in
-// which lane n holds twiddle n (one load for all of them, 4 VGPRs).
-__device__ __forceinline__ f128 f128_mul_shifts(f128 v, f128 lanes, int n) {
-  const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+// s + v t for twiddle t = n of a group's pass, in scalar registers, from
+// the vector in which lane n holds twiddle n (one load for all of them, 4
+// VGPRs).
+__device__ __forceinline__ f128 f128_mul_shifts_add(f128 v, f128 lanes, int n, f128 s) {
+  const uint32_t vw[4] = {v.x, v.y, v.z, v.w}, sw[4] = {s.x, s.y, s.z, s.w};
   uint32_t o[4];
-  shortmul::mul(vw, __builtin_amdgcn_readlane(lanes.x, n), __builtin_amdgcn_readlane(lanes.y, n),
-                __builtin_amdgcn_readlane(lanes.z, n), __builtin_amdgcn_readlane(lanes.w, n), o);
+  shortmul::mul_add(vw, __builtin_amdgcn_readlane(lanes.x, n), __builtin_amdgcn_readlane(lanes.y, n),
+                    __builtin_amdgcn_readlane(lanes.z, n), __builtin_amdgcn_readlane(lanes.w, n), sw, o);
   return make_uint4(o[0], o[1], o[2], o[3]);
 }
 
@@ -101,7 +102,7 @@ __device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__
 #pragma unroll
     for (int el = 0; el < nb; ++el) {
       const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-      const f128 u0 = f128_xor(v[e0], f128_mul_shifts(v[e1], twl, (1 << J) - 1 + C));
+      const f128 u0 = f128_mul_shifts_add(v[e1], twl, (1 << J) - 1 + C, v[e0]);
       v[e0] = u0;
       f128_xor_into(v[e1], u0);
       __builtin_amdgcn_sched_barrier(0);
@@ -116,7 +117,7 @@ __device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__
 #pragma unroll
     for (int el = 0; el < nb; ++el) {
       const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<0>(v[e1], lds, NL));
+      const f128 u0 = f128_mul_nibtab_add<0>(v[e1], lds, NL, v[e0]);
       v[e0] = u0;
       f128_xor_into(v[e1], u0);
       __builtin_amdgcn_sched_barrier(0);
@@ -125,7 +126,7 @@ __device__ __forceinline__ void tail_pass_step(f128 (&v)[1 << F], const f128 *__
 #pragma unroll
       for (int el = 0; el < nb; ++el) {
         const int e0 = ((C + 1) << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-        const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<NIBTAB_BYTES>(v[e1], lds, NL));
+        const f128 u0 = f128_mul_nibtab_add<NIBTAB_BYTES>(v[e1], lds, NL, v[e0]);
         v[e0] = u0;
         f128_xor_into(v[e1], u0);
         __builtin_amdgcn_sched_barrier(0);
@@ -210,15 +211,14 @@ __device__ __forceinline__ void fulltab_butterflies(f128 (&v)[G][1 << F], const 
     const int a0 = (C << (F - J)) | la, a1 = a0 | (1 << (F - 1 - J));
     if (STEP == 2 && sl + 1 < SLOTS) {
       const int b0 = (C << (F - J)) | lb, b1 = b0 | (1 << (F - 1 - J));
-      f128 p0, p1;
-      f128_mul2_fulltab<HI>(v[ga][a1], v[gb][b1], tbase, FL, p0, p1);
-      const f128 u0 = f128_xor(v[ga][a0], p0), u1 = f128_xor(v[gb][b0], p1);
+      f128 u0 = v[ga][a0], u1 = v[gb][b0];
+      f128_mul2_fulltab_add<HI>(v[ga][a1], v[gb][b1], tbase, FL, u0, u1);
       v[ga][a0] = u0;
       f128_xor_into(v[ga][a1], u0);
       v[gb][b0] = u1;
       f128_xor_into(v[gb][b1], u1);
     } else {
-      const f128 u0 = f128_xor(v[ga][a0], f128_mul_fulltab<HI>(v[ga][a1], tbase, FL));
+      const f128 u0 = f128_mul_fulltab_add<HI>(v[ga][a1], tbase, FL, v[ga][a0]);
       v[ga][a0] = u0;
       f128_xor_into(v[ga][a1], u0);
     }
@@ -451,7 +451,7 @@ __device__ __forceinline__ void nibshared_step(f128 (&v)[1 << F], const char *ld
 #pragma unroll
     for (int el = 0; el < nb; ++el) {
       const int e0 = (C << (F - J)) | el, e1 = e0 | (1 << (F - 1 - J));
-      const f128 u0 = f128_xor(v[e0], f128_mul_nibtab<(uint32_t)N * NIBTAB_BYTES>(v[e1], lds, NL));
+      const f128 u0 = f128_mul_nibtab_add<(uint32_t)N * NIBTAB_BYTES>(v[e1], lds, NL, v[e0]);
       v[e0] = u0;
       f128_xor_into(v[e1], u0);
       __builtin_amdgcn_sched_barrier(0);

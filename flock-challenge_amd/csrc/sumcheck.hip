@@ -359,12 +359,12 @@ __global__ __launch_bounds__(FT_THREADS) void fold_pair_kernel(const f128 *__res
       const uint64_t x = x0 + u * stride;
       if (x >= half) break;
       if (B) {
-        f128 pa, pb;
-        f128_mul2_fulltab(f128_xor(a0[u], a1[u]), f128_xor(b0[u], b1[u]), (const char *)tab, L, pa, pb);
-        A2[x] = f128_xor(a0[u], pa);
-        B2[x] = f128_xor(b0[u], pb);
+        f128 pa = a0[u], pb = b0[u];
+        f128_mul2_fulltab_add(f128_xor(a0[u], a1[u]), f128_xor(b0[u], b1[u]), (const char *)tab, L, pa, pb);
+        A2[x] = pa;
+        B2[x] = pb;
       } else {
-        A2[x] = f128_xor(a0[u], f128_mul_fulltab(f128_xor(a0[u], a1[u]), (const char *)tab, L));
+        A2[x] = f128_mul_fulltab_add(f128_xor(a0[u], a1[u]), (const char *)tab, L, a0[u]);
       }
     }
   }
@@ -398,8 +398,8 @@ static_assert(SMALL_PAIRS <= (uint64_t)SMALL_THREADS, "one pair per thread in th
 __device__ __forceinline__ void fold4(const f128 *__restrict__ v, uint64_t x, const char *nt, const NibtabLane &NL,
                                       f128 &o0, f128 &o1) {
   const f128 a0 = v[4 * x], a1 = v[4 * x + 1], a2 = v[4 * x + 2], a3 = v[4 * x + 3];
-  o0 = f128_xor(a0, f128_mul_nibtab<0>(f128_xor(a0, a1), nt, NL));
-  o1 = f128_xor(a2, f128_mul_nibtab<0>(f128_xor(a2, a3), nt, NL));
+  o0 = f128_mul_nibtab_add<0>(f128_xor(a0, a1), nt, NL, a0);
+  o1 = f128_mul_nibtab_add<0>(f128_xor(a2, a3), nt, NL, a2);
 }
 
 // XOR of s0, s1 over the workgroup into out[0], out[1] (out[2] = 0 when
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(LC_THREADS) void lc_comb_kernel(LcCsc A, LcCsc B, c
   constexpr uint32_t N_COLS = LC_COLS_PER_WAVE * (LC_THREADS / 64);
   if (tid < N_COLS) {
     const uint32_t c = blockIdx.x + LC_WGS * tid;
-    f128 v = f128_xor(f128_mul_nibtab<0>(cols[tid][0], nt, NibtabLane(tid & 15u)), cols[tid][1]);
+    f128 v = f128_mul_nibtab_add<0>(cols[tid][0], nt, NibtabLane(tid & 15u), cols[tid][1]);
     if (c == pin) f128_xor_into(v, beta);
     comb[c] = v;
   }
@@ -970,15 +970,14 @@ __global__ __launch_bounds__(MUL_THREADS) void ntt_pass_narrow_kernel(f128 *__re
       for (int el = 0; el < (1 << (F - 1 - j)); ++el) {
         const int e0 = (c << (F - j)) | el, e1 = e0 | (1 << (F - 1 - j));
         if (!TRANSPOSED) {
-          const f128 p = UT ? f128_mul_nibtab<0>(v[e1], wave_tab, NL) : f128_mul_comb(v[e1], tab, CL);
-          const f128 u = f128_xor(v[e0], p);
+          const f128 u = UT ? f128_mul_nibtab_add<0>(v[e1], wave_tab, NL, v[e0])
+                            : f128_xor(v[e0], f128_mul_comb(v[e1], tab, CL));
           v[e0] = u;
           v[e1] = f128_xor(v[e1], u);
         } else {
           const f128 sum = f128_xor(v[e0], v[e1]);
           v[e0] = sum;
-          const f128 p = UT ? f128_mul_nibtab<0>(sum, wave_tab, NL) : f128_mul_comb(sum, tab, CL);
-          v[e1] = f128_xor(p, v[e1]);
+          v[e1] = UT ? f128_mul_nibtab_add<0>(sum, wave_tab, NL, v[e1]) : f128_xor(f128_mul_comb(sum, tab, CL), v[e1]);
         }
       }
     }

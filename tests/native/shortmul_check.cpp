@@ -33,6 +33,16 @@ static int check(of128 v, of128 t) {
            (unsigned long long)want.lo);
     return 1;
   }
+  // the seeded form (a butterfly's u0 = s + v t) against s XOR the product
+  const uint64_t slo = next(), shi = next();
+  const uint32_t sw[4] = {(uint32_t)slo, (uint32_t)(slo >> 32), (uint32_t)shi, (uint32_t)(shi >> 32)};
+  flock::shortmul::mul_add(vw, (uint32_t)t.lo, (uint32_t)(t.lo >> 32), (uint32_t)t.hi, (uint32_t)(t.hi >> 32), sw, o);
+  const uint64_t alo = o[0] | ((uint64_t)o[1] << 32), ahi = o[2] | ((uint64_t)o[3] << 32);
+  if (alo != (want.lo ^ slo) || ahi != (want.hi ^ shi)) {
+    printf("mul_add v=%016llx%016llx t=%016llx%016llx: wrong\n", (unsigned long long)v.hi,
+           (unsigned long long)v.lo, (unsigned long long)t.hi, (unsigned long long)t.lo);
+    return 1;
+  }
   return 0;
 }
 
