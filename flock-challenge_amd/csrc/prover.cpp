@@ -696,11 +696,12 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
       if ((rc = P.h2d(dElo, elo.data(), elo.size() * 16))) return rc;
       if ((rc = P.eq_device(&r[7 + ZS_LOG + 1], n - 2 - ZS_LOG, Ehi))) return rc;
     }
+    // both round-1 results come back with one wait
+    std::vector<F128> qab(64 * 128), q(2048);
+    if ((rc = P.d2h2(qab.data(), dout, qab.size() * 16, q.data(), dq, q.size() * 16))) return rc;
     {
       // P^AB(lambda_j) = sum_{x_mid, t} E4(x_mid) phi8(2^t) bitfold of bit t of
       // Y_j: element 4 x_mid + 2 h + t / 4, bit 32 (t % 4) + b for j = 32 h + b
-      std::vector<F128> qab(64 * 128);
-      if ((rc = P.d2h(qab.data(), dout, qab.size() * 16))) return rc;
       const ZcTables &T = zc_tables();
       for (int j = 0; j < 64; ++j) {
         const int h = j >> 5, bit = j & 31;
@@ -714,8 +715,6 @@ int prove_back(Ctx *ctx, hipStream_t st, int m, const flock_lig_config *cfg, con
         r1ab[j] = reduce(acc);
       }
     }
-    std::vector<F128> q(2048);
-    if ((rc = P.d2h(q.data(), dq, 2048 * 16))) return rc;
     HostPhase hp(ctx, "round1_combine");
     std::vector<F128> lo = eq_table(&r[7], 4);
     F128 cs[64];
