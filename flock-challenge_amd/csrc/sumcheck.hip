@@ -188,9 +188,10 @@ __global__ __launch_bounds__(THREADS) void zc_round0_split_kernel(const ulonglon
   f128 *tab = &mt[0][0];
   const CombLane CL(tid);
   const NibtabLane NL(tid & 15u);
-  f128 s1[PER], si[PER];
+  // the sums stay unreduced (256-bit) until the end: no reduction per product
+  F256w s1[PER], si[PER];
 #pragma unroll
-  for (int s = 0; s < PER; ++s) s1[s] = si[s] = f128_zero();
+  for (int s = 0; s < PER; ++s) s1[s] = si[s] = F256w{};
   for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
     const f128 eh = Ehi[xh];
     __syncthreads();  // the previous x_hi's reads of nt and Tw are done (and T is visible)
@@ -204,13 +205,19 @@ __global__ __launch_bounds__(THREADS) void zc_round0_split_kernel(const ulonglon
       const ulonglong2 ar = a[x], br = b[x];
       const f128 a1 = lagrange_row(ar.y, Tw, rot), ai = lagrange_row(ar.x ^ ar.y, Tw, rot);
       const f128 b1 = lagrange_row(br.y, T, rot), bi = lagrange_row(br.x ^ br.y, T, rot);
-      const f128 w1 = f128_mul(a1, b1, tab, CL);
-      const f128 wi = f128_mul(ai, bi, tab, CL);
+      uint32_t w1[8], wi[8];
+      f128_mul_build(a1, tab, CL);
+      f128_comb256(b1, tab, CL, w1);
+      f128_mul_build(ai, tab, CL);
+      f128_comb256(bi, tab, CL, wi);
 #pragma unroll
       for (int q = 0; q < PER; ++q)
         if (q == s) {
-          f128_xor_into(s1[q], w1);
-          f128_xor_into(si[q], wi);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            s1[q].d[i] ^= w1[i];
+            si[q].d[i] ^= wi[i];
+          }
         }
     }
   }
@@ -218,8 +225,8 @@ __global__ __launch_bounds__(THREADS) void zc_round0_split_kernel(const ulonglon
 #pragma unroll
   for (int s = 0; s < PER; ++s) {
     f128_mul_build(Elo[s * THREADS + tid], tab, CL);
-    f128_xor_into(g1, f128_mul_comb(s1[s], tab, CL));
-    f128_xor_into(gi, f128_mul_comb(si[s], tab, CL));
+    f128_xor_into(g1, f128_mul_comb(f128_reduce256w(s1[s]), tab, CL));
+    f128_xor_into(gi, f128_mul_comb(f128_reduce256w(si[s]), tab, CL));
   }
   __syncthreads();  // every lane's table reads are done before the reduction reuses mt
   wg_xor_reduce(g1, tab, &partial[2 * blockIdx.x]);
