@@ -129,16 +129,28 @@ __global__ __launch_bounds__(THREADS) void zc_round_split_kernel(const f128 *__r
   f128 s1[PER], si[PER];
 #pragma unroll
   for (int s = 0; s < PER; ++s) s1[s] = si[s] = f128_zero();
+  // the next pair's loads are issued before this pair's products (the index
+  // after the last one clamped to it, so no branch)
+  f128 na0, na1, nb0, nb1;
+  auto load = [&](uint64_t x) {
+    na0 = A[2 * x];
+    na1 = A[2 * x + 1];
+    nb0 = B[2 * x];
+    nb1 = B[2 * x + 1];
+  };
+  if (blockIdx.x < n_hi) load(((uint64_t)blockIdx.x << LOG) + tid);
   for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
     const f128 eh = Ehi[xh];
     if (Ehi_next && !(xh & 1) && tid == 0) Ehi_next[xh >> 1] = f128_xor(eh, Ehi[xh + 1]);
     __syncthreads();  // the previous x_hi's table reads are done
     if (tid < 256) build_nibtab256<0>(eh, nt, tid);
     __syncthreads();
+    const uint64_t xh_next = xh + gridDim.x < n_hi ? xh + gridDim.x : xh;
 #pragma unroll 1
     for (int s = 0; s < PER; ++s) {
       const uint64_t x = (xh << LOG) + (uint64_t)s * THREADS + tid;
-      const f128 a0 = A[2 * x], a1 = A[2 * x + 1], b0 = B[2 * x], b1 = B[2 * x + 1];
+      const f128 a0 = na0, a1 = na1, b0 = nb0, b1 = nb1;
+      load(s + 1 < PER ? x + THREADS : (xh_next == xh ? x : (xh_next << LOG) + tid));
       const f128 p1 = f128_mul(a1, b1, tab, CL);
       const f128 pi = f128_mul(f128_xor(a0, a1), f128_xor(b0, b1), tab, CL);
       const f128 w1 = f128_mul_nibtab<0>(p1, nt, NL), wi = f128_mul_nibtab<0>(pi, nt, NL);
@@ -192,6 +204,13 @@ __global__ __launch_bounds__(THREADS) void zc_round0_split_kernel(const ulonglon
   F256w s1[PER], si[PER];
 #pragma unroll
   for (int s = 0; s < PER; ++s) s1[s] = si[s] = F256w{};
+  // the next row pair's loads are issued before this pair's products (the
+  // index after the last one clamped to it, so no branch)
+  ulonglong2 nar, nbr;
+  if (blockIdx.x < n_hi) {
+    nar = a[((uint64_t)blockIdx.x << LOG) + tid];
+    nbr = b[((uint64_t)blockIdx.x << LOG) + tid];
+  }
   for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
     const f128 eh = Ehi[xh];
     __syncthreads();  // the previous x_hi's reads of nt and Tw are done (and T is visible)
@@ -199,10 +218,14 @@ __global__ __launch_bounds__(THREADS) void zc_round0_split_kernel(const ulonglon
     __syncthreads();
     if (tid < 256) Tw[tid] = f128_mul_nibtab<0>(T[tid], nt, NL);
     __syncthreads();
+    const uint64_t xh_next = xh + gridDim.x < n_hi ? xh + gridDim.x : xh;
 #pragma unroll 1
     for (int s = 0; s < PER; ++s) {
       const uint64_t x = (xh << LOG) + (uint64_t)s * THREADS + tid;
-      const ulonglong2 ar = a[x], br = b[x];
+      const ulonglong2 ar = nar, br = nbr;
+      const uint64_t xn = s + 1 < PER ? x + THREADS : (xh_next == xh ? x : (xh_next << LOG) + tid);
+      nar = a[xn];
+      nbr = b[xn];
       const f128 a1 = lagrange_row(ar.y, Tw, rot), ai = lagrange_row(ar.x ^ ar.y, Tw, rot);
       const f128 b1 = lagrange_row(br.y, T, rot), bi = lagrange_row(br.x ^ br.y, T, rot);
       uint32_t w1[8], wi[8];
@@ -665,8 +688,25 @@ __global__ __launch_bounds__(MUL_THREADS, 2) void lig_msg_kernel(const f128 *__r
   const CombLane CL(threadIdx.x);
   // the sums stay unreduced (256-bit) until the end; y = u0 + sum f1 b1
   F256w u0 = {}, u2 = {}, y1 = {};
-  for (uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x; x < half; x += (uint64_t)gridDim.x * MUL_THREADS) {
-    const f128 f0 = f[2 * x], f1 = f[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+  // the next pair's loads are issued before this pair's products (its index
+  // clamped to this one at the end, so no branch), so their latency hides
+  // behind the products
+  const uint64_t stride = (uint64_t)gridDim.x * MUL_THREADS;
+  uint64_t x = (uint64_t)blockIdx.x * MUL_THREADS + threadIdx.x;
+  f128 nf0, nf1, nb0, nb1;
+  if (x < half) {
+    nf0 = f[2 * x];
+    nf1 = f[2 * x + 1];
+    nb0 = b[2 * x];
+    nb1 = b[2 * x + 1];
+  }
+  for (; x < half; x += stride) {
+    const f128 f0 = nf0, f1 = nf1, b0 = nb0, b1 = nb1;
+    const uint64_t xn = x + stride < half ? x + stride : x;
+    nf0 = f[2 * xn];
+    nf1 = f[2 * xn + 1];
+    nb0 = b[2 * xn];
+    nb1 = b[2 * xn + 1];
     f128_mul_build(f0, tab, CL);
     f128_mul_comb_acc(b0, tab, CL, u0);
     f128_mul_build(f128_xor(f0, f1), tab, CL);
