@@ -729,7 +729,9 @@ int lig_msg_device(Ctx *ctx, hipStream_t st, const void *f, const void *b, uint6
     return FLOCK_E_ARG;
   }
   ScopedPhase ph(ctx, "lig_msg", st);
-  unsigned g = grid_for(half, MUL_THREADS, 4096);
+  // at most the 1024 workgroups the CUs hold at once (four each, set by LDS):
+  // every workgroup starts at once and the partials to reduce stay few
+  unsigned g = grid_for(half, MUL_THREADS, 1024);
   f128 *part = (f128 *)ctx->get_scratch("lig.msg.partial", (size_t)g * 3 * sizeof(f128));
   if (!part) return FLOCK_E_OOM;
   if (with_eval)
